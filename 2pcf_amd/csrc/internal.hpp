@@ -1,0 +1,208 @@
+// internal.hpp -- shared records, device helpers and host plumbing of the
+// gfx950 pair-counting library.  Not part of the public C-ABI (tpcf_gpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tpcf_gpu.h"
+
+namespace tpcf {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kWavesPerBlock = 4;  // pair kernel: 4 independent waves per workgroup
+constexpr int kChunk = 64;         // lenses per chunk = one lens per lane
+constexpr int kSlots = 8;          // per-XCD copies of the patch-pair accumulators
+constexpr int kNQ = 5;             // accumulated FP64 quantities: Theta, G1, G2, W, K
+constexpr uint32_t kSegMax = 1024; // longest source segment (bounds per-segment FP32 sums)
+
+enum Kind { SHEAR = 0, KAPPA = 1 };
+
+// One source, sorted into (patch, cell) order.  Eight values of the input
+// precision: x, y, w, w*(1+m), s1*w, s2*w, 2*s1*w, 2*s2*w where (s1,s2) are
+// (e1,e2) for shear and (kappa_E,kappa_B) for kappa (kappa uses w*(1+m) = w).
+template <typename T>
+struct alignas(sizeof(T) * 8) SrcRec {
+    T x, y, w, wk, s1w, s2w, s1w2, s2w2;
+};
+
+// A non-empty (source patch, cell) run of sorted sources with its tight
+// bounding box and the sums used when the whole run falls into one bin.
+template <typename T>
+struct alignas(16) Seg {
+    uint32_t start, count, pb, pad;
+    T x0, x1, y0, y1;
+    T zmin, zpad;
+    double W, K, S1, S2;
+};
+
+// Up to 64 consecutive sorted lenses of one lens patch.
+template <typename T>
+struct alignas(16) Chunk {
+    uint32_t start, count, pa, pad;
+    T bx0, bx1, by0, by1;
+    T zmax, zpad;
+    double wsum, asum;
+};
+
+// Uniform cell grid laid over the sources.  Cell of a coordinate:
+// clamp(floor((x - x0) * inv), 0, n - 1) in FP64 -- a monotone function, so
+// conservative coordinate bounds map to conservative cell bounds.
+struct Grid {
+    double x0, y0, cell, inv;
+    int32_t nx, ny;
+    int64_t ncells;
+};
+
+__host__ __device__ inline int32_t cell_of(double v, double v0, double inv, int32_t n)
+{
+    double f = floor((v - v0) * inv);
+    if (!(f >= 0.0)) return 0;   // also catches NaN
+    if (f >= (double)(n - 1)) return n - 1;
+    return (int32_t)f;
+}
+
+// ---- wavefront reductions (DPP row shifts + row broadcasts, gfx9 family) ----
+template <int CTRL, int ROWMASK = 0xf, bool BC = true>
+__device__ inline int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWMASK, 0xf, BC);
+}
+
+// Sum over all 64 lanes; every lane must be active.  Result is uniform.
+__device__ inline float wave_sum(float v)
+{
+    v += __int_as_float(dpp_i<0x111>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x112>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x114>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x118>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x142, 0xa, false>(__float_as_int(v)));
+    v += __int_as_float(dpp_i<0x143, 0xc, false>(__float_as_int(v)));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+template <int CTRL, int ROWMASK = 0xf, bool BC = true>
+__device__ inline double dpp_d(double v)
+{
+    int2 p = *reinterpret_cast<int2 *>(&v);
+    p.x = dpp_i<CTRL, ROWMASK, BC>(p.x);
+    p.y = dpp_i<CTRL, ROWMASK, BC>(p.y);
+    return *reinterpret_cast<double *>(&p);
+}
+
+__device__ inline double wave_sum(double v)
+{
+    v += dpp_d<0x111>(v);
+    v += dpp_d<0x112>(v);
+    v += dpp_d<0x114>(v);
+    v += dpp_d<0x118>(v);
+    v += dpp_d<0x142, 0xa, false>(v);
+    v += dpp_d<0x143, 0xc, false>(v);
+    int2 p = *reinterpret_cast<int2 *>(&v);
+    p.x = __builtin_amdgcn_readlane(p.x, 63);
+    p.y = __builtin_amdgcn_readlane(p.y, 63);
+    return *reinterpret_cast<double *>(&p);
+}
+
+// ---- host plumbing ----
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct ProbeCounters {
+    unsigned long long seg[3];
+    unsigned long long pairs[3];
+};
+
+}  // namespace tpcf
+
+struct tpcf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipEvent_t ev[6] = {};
+    double opt_cell = 0.0;
+    double opt_cell_target = 48.0;
+    bool opt_counters = false;
+    tpcf_stats stats{};
+    std::string err;
+    // named device buffers, grown on demand and reused between calls
+    enum {
+        B_IN0, B_IN1, B_IN2, B_IN3, B_IN4, B_IN5, B_IN6, B_IN7, B_IN8, B_IN9, B_IN10, B_IN11,
+        B_PATCH_A, B_PATCH_B,
+        B_PARTIAL, B_ERR,
+        B_SKEY, B_SKEY2, B_SIDX, B_SIDX2, B_COUNTS, B_CSTART, B_SEGFIRST, B_SREC, B_SZ, B_SEGS, B_PBOX,
+        B_LKEY, B_LKEY2, B_LIDX, B_LIDX2, B_PCOUNT, B_CHOFF, B_PASTART,
+        B_LX, B_LY, B_LZ, B_LA, B_LW, B_CHUNKS,
+        B_PP, B_PPN, B_MARKS, B_OUT, B_TMP, B_PROBE, B_E2,
+        B_COUNT
+    };
+    tpcf::DevBuf buf[B_COUNT];
+};
+
+namespace tpcf {
+
+int fail(tpcf_ctx *ctx, int code, const std::string &msg);
+int ensure(tpcf_ctx *ctx, int which, size_t bytes, void **ptr);
+
+#define TPCF_HIP(ctx, call)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ::tpcf::fail((ctx), TPCF_ERR_HIP,                                     \
+                                std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define TPCF_TRY(expr)          \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_) return rc_;    \
+    } while (0)
+
+// Arguments of one correlation call after validation; pointers are device
+// pointers of the input precision T.
+template <typename T>
+struct CallArgs {
+    int kind;
+    int64_t n_a, n_b;
+    const T *xa, *ya, *za, *vala, *wa;
+    const T *xb, *yb, *zb, *s1b, *s2b, *wb, *mb;   // zb, mb NULL for kappa
+    const int32_t *patch_a, *patch_b;              // device, nullable
+    int32_t n_pa, n_pb;
+    const tpcf_bins *bins;
+    bool marks;
+    bool verbose;
+};
+
+// Device-side result of the preparation stage.
+template <typename T>
+struct Prepared {
+    Grid grid;
+    int64_t n_src;         // sources kept
+    int64_t n_lens;        // lenses kept
+    int64_t n_chunks;
+    int64_t n_segs;
+    int64_t key_space;     // n_pb * ncells
+    const SrcRec<T> *srec;
+    const T *sz;           // sorted source z (shear)
+    const Seg<T> *segs;
+    const uint32_t *seg_first;   // [key_space + 1]
+    const int4 *pbox;            // per source patch: cx0, cx1, cy0, cy1 (cx0 > cx1 when empty)
+    const Chunk<T> *chunks;
+    const T *lx, *ly, *lz, *la, *lw;
+    const uint32_t *lidx;        // sorted lens -> original index
+};
+
+template <typename T>
+int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p);
+
+template <typename T>
+int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double *pp, unsigned long long *ppn,
+              double *marks);
+
+}  // namespace tpcf

@@ -1,0 +1,474 @@
+// prepare.hip -- device-side replacement of the reference's kd-tree build
+// (grow_tree, src/kdtree.f90:73-205, called at src/tpcf.f90:255-257).
+//
+// Instead of a pointer tree the catalogues are sorted on the device:
+//  * sources by key (patch_B, cell) on a uniform grid: every non-empty
+//    (patch, cell) run becomes a "segment" with a tight bounding box and the
+//    sums needed when all its pairs fall into one bin;
+//  * lenses by (patch_A, Morton code) and cut into chunks of up to 64 lenses
+//    (one lens per lane), each with its bounding box.
+// Sources masked by the reference (z_B < 0, src/tpcf.f90:306-309) and lenses
+// it skips (w_A == 0, src/tpcf.f90:285 / :474; z_A < 0 for shear) are dropped
+// here, which removes the same pairs.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace tpcf {
+
+namespace {
+
+constexpr int kBlk = 256;
+
+template <typename U>
+__global__ void k_fill(U *p, U v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename U>
+void launch_fill(hipStream_t s, U *p, U v, int64_t n)
+{
+    if (n <= 0) return;
+    int64_t blocks = std::min<int64_t>((n + kBlk - 1) / kBlk, 8192);
+    hipLaunchKernelGGL(k_fill<U>, dim3((unsigned)blocks), dim3(kBlk), 0, s, p, v, n);
+}
+
+// Per-block min/max of x and y and count of non-finite coordinates.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_bbox(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
+                                               double *__restrict__ out)
+{
+    double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY, bad = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double vx = (double)x[i], vy = (double)y[i];
+        if (!isfinite(vx) || !isfinite(vy)) {
+            bad += 1.0;
+            continue;
+        }
+        mnx = fmin(mnx, vx); mxx = fmax(mxx, vx);
+        mny = fmin(mny, vy); mxy = fmax(mxy, vy);
+    }
+    __shared__ double sh[5][kBlk];
+    sh[0][threadIdx.x] = mnx; sh[1][threadIdx.x] = mxx; sh[2][threadIdx.x] = mny; sh[3][threadIdx.x] = mxy;
+    sh[4][threadIdx.x] = bad;
+    __syncthreads();
+    for (int s = kBlk / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sh[0][threadIdx.x] = fmin(sh[0][threadIdx.x], sh[0][threadIdx.x + s]);
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + s]);
+            sh[2][threadIdx.x] = fmin(sh[2][threadIdx.x], sh[2][threadIdx.x + s]);
+            sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + s]);
+            sh[4][threadIdx.x] += sh[4][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) out[blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Source sort key: pb * ncells + cell, or key_space for a dropped source.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_src_keys(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
+                                                   const T *__restrict__ z, const int32_t *__restrict__ patch,
+                                                   int32_t n_pb, Grid g, uint32_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ idx, uint32_t *__restrict__ counts,
+                                                   int32_t *__restrict__ err)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t drop_key = (uint32_t)(g.ncells * n_pb);
+    uint32_t key;
+    int32_t pb = patch ? patch[i] : 0;
+    if (pb < 0 || pb >= n_pb) {
+        atomicOr(err, 1);
+        pb = 0;
+    }
+    if (z && z[i] < (T)0) {
+        key = drop_key;
+    } else {
+        const int32_t cx = cell_of((double)x[i], g.x0, g.inv, g.nx);
+        const int32_t cy = cell_of((double)y[i], g.y0, g.inv, g.ny);
+        key = (uint32_t)((int64_t)pb * g.ncells + (int64_t)cy * g.nx + cx);
+    }
+    keys[i] = key;
+    idx[i] = (uint32_t)i;
+    atomicAdd(&counts[key], 1u);
+}
+
+// Number of segments of each key: runs longer than kSegMax are split.
+__global__ void k_nsegs(int64_t key_space, const uint32_t *__restrict__ counts, uint32_t *__restrict__ nseg)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k <= key_space; k += (int64_t)gridDim.x * blockDim.x)
+        nseg[k] = (k < key_space) ? (counts[k] + kSegMax - 1) / kSegMax : 0u;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_src_gather(int64_t n, int kind, const uint32_t *__restrict__ order,
+                                                     const T *__restrict__ x, const T *__restrict__ y,
+                                                     const T *__restrict__ z, const T *__restrict__ s1,
+                                                     const T *__restrict__ s2, const T *__restrict__ w,
+                                                     const T *__restrict__ m, SrcRec<T> *__restrict__ rec,
+                                                     T *__restrict__ zs)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = order[i];
+    SrcRec<T> r;
+    r.x = x[j];
+    r.y = y[j];
+    r.w = w[j];
+    r.wk = (kind == SHEAR && m) ? w[j] * ((T)1 + m[j]) : w[j];
+    r.s1w = s1[j] * r.w;
+    r.s2w = s2[j] * r.w;
+    r.s1w2 = (T)2 * r.s1w;
+    r.s2w2 = (T)2 * r.s2w;
+    rec[i] = r;
+    if (zs) zs[i] = z[j];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_segs(int64_t key_space, Grid g, const uint32_t *__restrict__ counts,
+                                               const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ seg_first,
+                                               const SrcRec<T> *__restrict__ rec, const T *__restrict__ zs,
+                                               Seg<T> *__restrict__ segs, int4 *__restrict__ pbox)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= key_space) return;
+    const uint32_t total = counts[k];
+    if (total == 0) return;
+    const uint32_t pb = (uint32_t)(k / g.ncells);
+    for (uint32_t part = 0; part * kSegMax < total; ++part) {
+        Seg<T> s;
+        s.start = cstart[k] + part * kSegMax;
+        s.count = min(kSegMax, total - part * kSegMax);
+        s.pb = pb;
+        s.pad = 0;
+        T x0 = rec[s.start].x, x1 = x0, y0 = rec[s.start].y, y1 = y0;
+        T zmin = zs ? zs[s.start] : (T)0;
+        double W = 0, K = 0, S1 = 0, S2 = 0;
+        for (uint32_t q = s.start; q < s.start + s.count; ++q) {
+            const SrcRec<T> r = rec[q];
+            x0 = r.x < x0 ? r.x : x0; x1 = r.x > x1 ? r.x : x1;
+            y0 = r.y < y0 ? r.y : y0; y1 = r.y > y1 ? r.y : y1;
+            if (zs) zmin = fmin(zmin, zs[q]);
+            W += (double)r.w; K += (double)r.wk; S1 += (double)r.s1w; S2 += (double)r.s2w;
+        }
+        s.x0 = x0; s.x1 = x1; s.y0 = y0; s.y1 = y1;
+        s.zmin = zmin; s.zpad = (T)0;
+        s.W = W; s.K = K; s.S1 = S1; s.S2 = S2;
+        segs[seg_first[k] + part] = s;
+    }
+    const int64_t cell = k % g.ncells;
+    const int32_t cx = (int32_t)(cell % g.nx), cy = (int32_t)(cell / g.nx);
+    int4 *b = &pbox[pb];
+    atomicMin(&b->x, cx);
+    atomicMax(&b->y, cx);
+    atomicMin(&b->z, cy);
+    atomicMax(&b->w, cy);
+}
+
+__device__ inline uint32_t spread16(uint32_t v)
+{
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Lens sort key: (pa << 32) | Morton code of the 16-bit quantised position,
+// or (n_pa << 32) for a dropped lens.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_lens_keys(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
+                                                    const T *__restrict__ z, const T *__restrict__ w,
+                                                    const int32_t *__restrict__ patch, int32_t n_pa, double lx0,
+                                                    double ly0, double qs, uint64_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ idx, uint32_t *__restrict__ pcount,
+                                                    int32_t *__restrict__ err)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t pa = patch ? patch[i] : 0;
+    if (pa < 0 || pa >= n_pa) {
+        atomicOr(err, 2);
+        pa = 0;
+    }
+    const bool drop = (w[i] == (T)0) || (z && z[i] < (T)0);
+    uint64_t key;
+    if (drop) {
+        key = (uint64_t)n_pa << 32;
+    } else {
+        double fx = ((double)x[i] - lx0) * qs, fy = ((double)y[i] - ly0) * qs;
+        uint32_t qx = (uint32_t)fmin(fmax(fx, 0.0), 65535.0);
+        uint32_t qy = (uint32_t)fmin(fmax(fy, 0.0), 65535.0);
+        key = ((uint64_t)pa << 32) | (uint64_t)(spread16(qx) | (spread16(qy) << 1));
+        atomicAdd(&pcount[pa], 1u);
+    }
+    keys[i] = key;
+    idx[i] = (uint32_t)i;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_lens_gather(int64_t n, const uint32_t *__restrict__ order,
+                                                      const T *__restrict__ x, const T *__restrict__ y,
+                                                      const T *__restrict__ z, const T *__restrict__ val,
+                                                      const T *__restrict__ w, T *__restrict__ lx, T *__restrict__ ly,
+                                                      T *__restrict__ lz, T *__restrict__ la, T *__restrict__ lw)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = order[i];
+    lx[i] = x[j];
+    ly[i] = y[j];
+    lz[i] = z ? z[j] : (T)0;
+    la[i] = val[j] * w[j];
+    lw[i] = w[j];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_chunks(int64_t n_chunks, int32_t n_pa, const uint32_t *__restrict__ ch_off,
+                                                 const uint32_t *__restrict__ pa_start,
+                                                 const uint32_t *__restrict__ pcount, const T *__restrict__ lx,
+                                                 const T *__restrict__ ly, const T *__restrict__ lz,
+                                                 const T *__restrict__ lw, const T *__restrict__ la,
+                                                 Chunk<T> *__restrict__ chunks)
+{
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= n_chunks) return;
+    int lo = 0, hi = n_pa;   // ch_off[lo] <= c < ch_off[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if ((int64_t)ch_off[mid] <= c) lo = mid; else hi = mid;
+    }
+    const uint32_t local = (uint32_t)(c - ch_off[lo]);
+    Chunk<T> ch;
+    ch.start = pa_start[lo] + local * kChunk;
+    ch.count = std::min<uint32_t>(kChunk, pcount[lo] - local * kChunk);
+    ch.pa = (uint32_t)lo;
+    ch.pad = 0;
+    T bx0 = lx[ch.start], bx1 = bx0, by0 = ly[ch.start], by1 = by0, zmax = lz[ch.start];
+    double ws = 0, as = 0;
+    for (uint32_t q = ch.start; q < ch.start + ch.count; ++q) {
+        bx0 = lx[q] < bx0 ? lx[q] : bx0; bx1 = lx[q] > bx1 ? lx[q] : bx1;
+        by0 = ly[q] < by0 ? ly[q] : by0; by1 = ly[q] > by1 ? ly[q] : by1;
+        zmax = fmax(zmax, lz[q]);
+        ws += (double)lw[q];
+        as += (double)la[q];
+    }
+    ch.bx0 = bx0; ch.bx1 = bx1; ch.by0 = by0; ch.by1 = by1;
+    ch.zmax = zmax; ch.zpad = (T)0;
+    ch.wsum = ws; ch.asum = as;
+    chunks[c] = ch;
+}
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlk - 1) / kBlk); }
+
+inline int bitlen(uint64_t v)
+{
+    int b = 0;
+    while (v) { ++b; v >>= 1; }
+    return b;
+}
+
+}  // namespace
+
+template <typename T>
+int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
+{
+    hipStream_t s = ctx->stream;
+    const bool shear = (a.kind == SHEAR);
+    const double x_max = (double)a.bins->x_max;
+
+    // ---- bounding boxes and finiteness of both catalogues ----
+    const unsigned nbs = (unsigned)std::min<int64_t>(1024, (a.n_b + kBlk - 1) / kBlk);
+    const unsigned nbl = (unsigned)std::min<int64_t>(1024, (a.n_a + kBlk - 1) / kBlk);
+    double *partial;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_PARTIAL, sizeof(double) * 5 * (nbs + nbl), (void **)&partial));
+    hipLaunchKernelGGL(k_bbox<T>, dim3(nbs), dim3(kBlk), 0, s, a.n_b, a.xb, a.yb, partial);
+    hipLaunchKernelGGL(k_bbox<T>, dim3(nbl), dim3(kBlk), 0, s, a.n_a, a.xa, a.ya, partial + 5 * nbs);
+    std::vector<double> hp(5 * (nbs + nbl));
+    TPCF_HIP(ctx, hipMemcpyAsync(hp.data(), partial, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
+    TPCF_HIP(ctx, hipStreamSynchronize(s));
+    double sb[5] = {INFINITY, -INFINITY, INFINITY, -INFINITY, 0}, lb[5] = {INFINITY, -INFINITY, INFINITY, -INFINITY, 0};
+    for (unsigned b = 0; b < nbs + nbl; ++b) {
+        double *d = (b < nbs) ? sb : lb;
+        const double *q = &hp[5 * b];
+        d[0] = std::min(d[0], q[0]); d[1] = std::max(d[1], q[1]);
+        d[2] = std::min(d[2], q[2]); d[3] = std::max(d[3], q[3]);
+        d[4] += q[4];
+    }
+    if (sb[4] > 0 || lb[4] > 0) return fail(ctx, TPCF_ERR_NONFINITE, "non-finite coordinate in the input catalogues");
+
+    // ---- source cell grid ----
+    const double ext_x = sb[1] - sb[0], ext_y = sb[3] - sb[2];
+    double cell = ctx->opt_cell;
+    if (!(cell > 0.0)) {
+        const double floor_ext = x_max / 64.0;
+        const double area = std::max(ext_x, floor_ext) * std::max(ext_y, floor_ext);
+        cell = std::sqrt(ctx->opt_cell_target * area / (double)std::max<int64_t>(a.n_b, 1));
+        cell = std::min(std::max(cell, x_max / 512.0), 4.0 * x_max);
+    }
+    Grid g;
+    for (int attempt = 0;; ++attempt) {
+        cell = std::max(cell, std::max(ext_x, ext_y) / 16384.0);
+        g.x0 = sb[0];
+        g.y0 = sb[2];
+        g.cell = cell;
+        g.inv = 1.0 / cell;
+        g.nx = (int32_t)std::floor(ext_x * g.inv) + 1;
+        g.ny = (int32_t)std::floor(ext_y * g.inv) + 1;
+        g.ncells = (int64_t)g.nx * g.ny;
+        if (g.ncells * a.n_pb < ((int64_t)1 << 31)) break;
+        if (attempt > 40) return fail(ctx, TPCF_ERR_GRID, "cell grid too large");
+        cell *= 1.25;
+    }
+    const int64_t key_space = g.ncells * a.n_pb;
+    p.grid = g;
+    p.key_space = key_space;
+
+    // ---- sources: keys, histogram, sort ----
+    const int64_t nb = a.n_b;
+    uint32_t *skey, *skey2, *sidx, *sidx2, *counts, *cstart, *seg_first;
+    int32_t *err;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SKEY, sizeof(uint32_t) * nb, (void **)&skey));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SKEY2, sizeof(uint32_t) * nb, (void **)&skey2));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SIDX, sizeof(uint32_t) * nb, (void **)&sidx));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SIDX2, sizeof(uint32_t) * nb, (void **)&sidx2));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_COUNTS, sizeof(uint32_t) * (key_space + 2), (void **)&counts));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_CSTART, sizeof(uint32_t) * (key_space + 2), (void **)&cstart));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SEGFIRST, sizeof(uint32_t) * (key_space + 2), (void **)&seg_first));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_ERR, sizeof(int32_t) * 4, (void **)&err));
+    launch_fill<uint32_t>(s, counts, 0u, key_space + 1);
+    launch_fill<int32_t>(s, err, 0, 4);
+    hipLaunchKernelGGL(k_src_keys<T>, dim3(blocks_for(nb)), dim3(kBlk), 0, s, nb, a.xb, a.yb, shear ? a.zb : nullptr,
+                       a.patch_b, a.n_pb, g, skey, sidx, counts, err);
+
+    size_t tmp_sort = 0, tmp_scan = 0, tmp_lsort = 0;
+    const int sbits = std::max(1, bitlen((uint64_t)key_space));
+    TPCF_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tmp_sort, skey, skey2, sidx, sidx2, (size_t)nb, 0, sbits, s));
+    TPCF_HIP(ctx, rocprim::exclusive_scan(nullptr, tmp_scan, counts, cstart, 0u, (size_t)key_space + 1,
+                                          rocprim::plus<uint32_t>(), s));
+    const int lbits = 32 + std::max(1, bitlen((uint64_t)a.n_pa));
+    {
+        uint64_t *k64 = nullptr;
+        TPCF_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tmp_lsort, k64, k64, sidx, sidx2, (size_t)a.n_a, 0, lbits, s));
+    }
+    void *tmp;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_TMP, std::max(std::max(tmp_sort, tmp_scan), tmp_lsort) + 256, &tmp));
+    size_t tb = tmp_sort;
+    TPCF_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, skey, skey2, sidx, sidx2, (size_t)nb, 0, sbits, s));
+    tb = tmp_scan;
+    TPCF_HIP(ctx, rocprim::exclusive_scan(tmp, tb, counts, cstart, 0u, (size_t)key_space + 1,
+                                          rocprim::plus<uint32_t>(), s));
+    // segments per key, scanned into the first segment index of each key
+    uint32_t *flags;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_OUT, sizeof(uint32_t) * (key_space + 2), (void **)&flags));
+    hipLaunchKernelGGL(k_nsegs, dim3((unsigned)std::min<int64_t>(8192, blocks_for(key_space + 1))), dim3(kBlk), 0, s,
+                       key_space, counts, flags);
+    tb = tmp_scan;
+    TPCF_HIP(ctx, rocprim::exclusive_scan(tmp, tb, flags, seg_first, 0u, (size_t)key_space + 1,
+                                          rocprim::plus<uint32_t>(), s));
+    uint32_t hcnt[2];
+    int32_t herr = 0;
+    TPCF_HIP(ctx, hipMemcpyAsync(&hcnt[0], cstart + key_space, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    TPCF_HIP(ctx, hipMemcpyAsync(&hcnt[1], seg_first + key_space, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    TPCF_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TPCF_HIP(ctx, hipStreamSynchronize(s));
+    if (herr & 1) return fail(ctx, TPCF_ERR_PATCH, "source patch label outside [0, n_patch_B)");
+    p.n_src = hcnt[0];
+    p.n_segs = hcnt[1];
+
+    SrcRec<T> *srec;
+    T *sz = nullptr;
+    Seg<T> *segs;
+    int4 *pbox;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SREC, sizeof(SrcRec<T>) * std::max<int64_t>(p.n_src, 1), (void **)&srec));
+    if (shear) TPCF_TRY(ensure(ctx, tpcf_ctx::B_SZ, sizeof(T) * std::max<int64_t>(p.n_src, 1), (void **)&sz));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SEGS, sizeof(Seg<T>) * std::max<int64_t>(p.n_segs, 1), (void **)&segs));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_PBOX, sizeof(int4) * a.n_pb, (void **)&pbox));
+    if (p.n_src > 0)
+        hipLaunchKernelGGL(k_src_gather<T>, dim3(blocks_for(p.n_src)), dim3(kBlk), 0, s, p.n_src, a.kind, sidx2, a.xb,
+                           a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb, srec, sz);
+    launch_fill<int4>(s, pbox, make_int4(INT_MAX, -1, INT_MAX, -1), a.n_pb);
+    hipLaunchKernelGGL(k_segs<T>, dim3(blocks_for(key_space)), dim3(kBlk), 0, s, key_space, g, counts, cstart,
+                       seg_first, srec, sz, segs, pbox);
+    p.srec = srec;
+    p.sz = sz;
+    p.segs = segs;
+    p.seg_first = seg_first;
+    p.pbox = pbox;
+
+    // ---- lenses: keys, sort, chunks ----
+    const int64_t na = a.n_a;
+    uint64_t *lkey, *lkey2;
+    uint32_t *lidx, *lidx2, *pcount, *choff, *pastart;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LKEY, sizeof(uint64_t) * na, (void **)&lkey));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LKEY2, sizeof(uint64_t) * na, (void **)&lkey2));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LIDX, sizeof(uint32_t) * na, (void **)&lidx));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LIDX2, sizeof(uint32_t) * na, (void **)&lidx2));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_PCOUNT, sizeof(uint32_t) * (a.n_pa + 1), (void **)&pcount));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_CHOFF, sizeof(uint32_t) * (a.n_pa + 1), (void **)&choff));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_PASTART, sizeof(uint32_t) * (a.n_pa + 1), (void **)&pastart));
+    launch_fill<uint32_t>(s, pcount, 0u, a.n_pa + 1);
+    const double lext = std::max(std::max(lb[1] - lb[0], lb[3] - lb[2]), 1e-30);
+    hipLaunchKernelGGL(k_lens_keys<T>, dim3(blocks_for(na)), dim3(kBlk), 0, s, na, a.xa, a.ya, shear ? a.za : nullptr,
+                       a.wa, a.patch_a, a.n_pa, lb[0], lb[2], 65535.0 / lext, lkey, lidx, pcount, err);
+    tb = tmp_lsort;
+    TPCF_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, lkey, lkey2, lidx, lidx2, (size_t)na, 0, lbits, s));
+    std::vector<uint32_t> hpc(a.n_pa);
+    TPCF_HIP(ctx, hipMemcpyAsync(hpc.data(), pcount, sizeof(uint32_t) * a.n_pa, hipMemcpyDeviceToHost, s));
+    TPCF_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TPCF_HIP(ctx, hipStreamSynchronize(s));
+    if (herr & 2) return fail(ctx, TPCF_ERR_PATCH, "lens patch label outside [0, n_patch_A)");
+    std::vector<uint32_t> hoff(a.n_pa + 1), hstart(a.n_pa + 1);
+    int64_t nl = 0, nch = 0;
+    for (int32_t q = 0; q < a.n_pa; ++q) {
+        hstart[q] = (uint32_t)nl;
+        hoff[q] = (uint32_t)nch;
+        nl += hpc[q];
+        nch += (hpc[q] + kChunk - 1) / kChunk;
+    }
+    hstart[a.n_pa] = (uint32_t)nl;
+    hoff[a.n_pa] = (uint32_t)nch;
+    p.n_lens = nl;
+    p.n_chunks = nch;
+    TPCF_HIP(ctx, hipMemcpyAsync(choff, hoff.data(), sizeof(uint32_t) * (a.n_pa + 1), hipMemcpyHostToDevice, s));
+    TPCF_HIP(ctx, hipMemcpyAsync(pastart, hstart.data(), sizeof(uint32_t) * (a.n_pa + 1), hipMemcpyHostToDevice, s));
+
+    T *lx, *ly, *lz, *la, *lw;
+    Chunk<T> *chunks;
+    const int64_t nl1 = std::max<int64_t>(nl, 1);
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LX, sizeof(T) * nl1, (void **)&lx));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LY, sizeof(T) * nl1, (void **)&ly));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LZ, sizeof(T) * nl1, (void **)&lz));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LA, sizeof(T) * nl1, (void **)&la));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_LW, sizeof(T) * nl1, (void **)&lw));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_CHUNKS, sizeof(Chunk<T>) * std::max<int64_t>(nch, 1), (void **)&chunks));
+    if (nl > 0) {
+        hipLaunchKernelGGL(k_lens_gather<T>, dim3(blocks_for(nl)), dim3(kBlk), 0, s, nl, lidx2, a.xa, a.ya,
+                           shear ? a.za : nullptr, a.vala, a.wa, lx, ly, lz, la, lw);
+        hipLaunchKernelGGL(k_chunks<T>, dim3(blocks_for(nch)), dim3(kBlk), 0, s, nch, a.n_pa, choff, pastart, pcount,
+                           lx, ly, lz, lw, la, chunks);
+    }
+    TPCF_HIP(ctx, hipGetLastError());
+    p.chunks = chunks;
+    p.lx = lx; p.ly = ly; p.lz = lz; p.la = la; p.lw = lw;
+    p.lidx = lidx2;
+    return TPCF_OK;
+}
+
+template int prepare<float>(tpcf_ctx *, const CallArgs<float> &, Prepared<float> &);
+template int prepare<double>(tpcf_ctx *, const CallArgs<double> &, Prepared<double> &);
+
+// fill helpers used by the other translation units
+void fill_u64(hipStream_t s, unsigned long long *p, int64_t n) { launch_fill<unsigned long long>(s, p, 0ull, n); }
+void fill_f64(hipStream_t s, double *p, int64_t n) { launch_fill<double>(s, p, 0.0, n); }
+
+}  // namespace tpcf

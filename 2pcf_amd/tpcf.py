@@ -1,0 +1,326 @@
+"""Host-side mirror of the reference's ``tpcf`` / ``utils`` module interface for
+the correlation call, running on the gfx950 library through its C-ABI.
+
+Reference (tilmantroester/2pcf):
+  * ``binning`` / ``create_bins`` / ``find_bin_index``   src/utils.f90:18-24, :139-230, :101-137
+  * ``shape_catalog``, ``lens_catalog``, ``kappa_map``, ``lensing_survey_field``,
+    ``shear_2pcf_results``                                 src/tpcf.f90:17-43
+  * ``shear_2pcf`` -> tangential_shear_tpcf             src/tpcf.f90:134-160, :190-378
+  * ``kappa_2pcf`` -> scalar_kappa_tpcf                 src/tpcf.f90:162-188, :380-545
+  * field stacking                                       src/main.f90:678-695
+
+Differences from the reference, kept on purpose: marks can be switched off
+(``marks=False``; the reference always computes them because
+``calculate_marks`` is never set, src/tpcf.f90:146); per-patch-pair sums and
+raw FP64 sums are returned as well; errors raise ``TpcfError`` instead of
+stopping the program.  ``num_threads`` and ``use_kd_tree`` are accepted and
+have no effect (the pair set is the reference's "every pair within x_max").
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _native as N
+from ._native import IHS, KAPPA, LIN, LOG, SHEAR, SQRT, TpcfError  # noqa: F401  (re-exported)
+
+lin_binning, log_binning, ihs_binning, sqrt_binning = LIN, LOG, IHS, SQRT
+
+
+# ---------------------------------------------------------------- binning ----
+@dataclass
+class Binning:
+    """type(binning), src/utils.f90:18-24; ``bin_edges`` are the FP64 edges the
+    library bins with (squared-edge comparison in FP64)."""
+    x_min: float
+    x_max: float
+    bin_size: float
+    n_bin: int
+    spacing: int
+    ihs_theta: float
+    bin_edges: np.ndarray
+    bin_centers: np.ndarray
+    c: N.Bins = field(repr=False, default=None)
+
+
+def create_bins(x_min: float, x_max: float, n_bin: int, spacing: int = LOG, p: float = 1.0) -> Binning:
+    """create_bins(x_min=, x_max=, n_bin=, p=, spacing=), src/utils.f90:139-230."""
+    b = N.Bins()
+    rc = N.lib().tpcf_create_bins(x_min, x_max, n_bin, spacing, p, C.byref(b))
+    if rc:
+        raise TpcfError(rc, "ill-defined binning")
+    n = b.n_bin
+    return Binning(x_min=b.x_min, x_max=b.x_max, bin_size=b.bin_size, n_bin=n, spacing=b.spacing,
+                   ihs_theta=b.ihs_theta, bin_edges=np.array(b.edges[: n + 1]),
+                   bin_centers=np.array(b.bin_centers[:n], dtype=np.float32), c=b)
+
+
+def find_bin_index(x: float, bins: Binning) -> int:
+    """find_bin_index, src/utils.f90:101-137: 1-based bin, -1 outside."""
+    k = N.lib().tpcf_find_bin_index(float(x), C.byref(bins.c))
+    return k + 1 if k >= 0 else -1
+
+
+# ---------------------------------------------------------------- catalogues ----
+@dataclass
+class ShapeCatalog:       # src/tpcf.f90:24-27
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    e1: np.ndarray
+    e2: np.ndarray
+    m: np.ndarray
+    w: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.x)
+
+
+@dataclass
+class KappaMap:           # src/tpcf.f90:29-32
+    x: np.ndarray
+    y: np.ndarray
+    kappa_E: np.ndarray
+    kappa_B: np.ndarray
+    w: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.x)
+
+
+@dataclass
+class LensCatalog:        # src/tpcf.f90:34-37
+    x: np.ndarray
+    y: np.ndarray
+    val: np.ndarray
+    w: np.ndarray
+    z: Optional[np.ndarray] = None
+
+    @property
+    def n(self) -> int:
+        return len(self.x)
+
+
+@dataclass
+class LensingSurveyField:  # src/tpcf.f90:39-43
+    lenses: LensCatalog
+    shapes: Optional[ShapeCatalog] = None
+    convergence: Optional[KappaMap] = None
+
+
+@dataclass
+class Shear2pcfResults:   # src/tpcf.f90:17-22 plus raw and per-patch-pair sums
+    n_bin: int
+    theta: np.ndarray
+    mean_theta: np.ndarray
+    g_t: np.ndarray
+    g_x: np.ndarray
+    weights: np.ndarray
+    n_pairs: np.ndarray
+    sum_theta: np.ndarray
+    sum_gt: np.ndarray
+    sum_gx: np.ndarray
+    sum_w: np.ndarray
+    sum_k: np.ndarray
+    # per-lens marks, shape (n_bin, n_A) in Fortran order like real(n_bin, n_A)
+    xi_E_marks: Optional[np.ndarray] = None
+    xi_B_marks: Optional[np.ndarray] = None
+    K_marks: Optional[np.ndarray] = None
+    w_marks: Optional[np.ndarray] = None
+    marks_weight: Optional[np.ndarray] = None
+    # per-patch-pair raw sums, shape (P_B, P_A, n_bin) == real(n_bin, P_A, P_B)
+    pp_n: Optional[np.ndarray] = None
+    pp_theta: Optional[np.ndarray] = None
+    pp_gt: Optional[np.ndarray] = None
+    pp_gx: Optional[np.ndarray] = None
+    pp_w: Optional[np.ndarray] = None
+    pp_k: Optional[np.ndarray] = None
+    stats: Optional[dict] = None
+
+
+# ---------------------------------------------------------------- context ----
+class Context:
+    """Owns a ``tpcf_ctx`` (device, stream, reusable device buffers)."""
+
+    def __init__(self, device: int = -1):
+        self._h = C.c_void_p()
+        rc = N.lib().tpcf_ctx_create(device, C.byref(self._h))
+        if rc:
+            raise TpcfError(rc, "tpcf_ctx_create failed")
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().tpcf_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def set_option(self, option: int, value: float) -> None:
+        rc = N.lib().tpcf_set_option(self._h, option, float(value))
+        if rc:
+            raise TpcfError(rc, self.last_error())
+
+    def set_stream(self, stream_handle: int | None) -> None:
+        N.lib().tpcf_set_stream(self._h, C.c_void_p(stream_handle) if stream_handle else None)
+
+    def last_error(self) -> str:
+        s = N.lib().tpcf_last_error(self._h)
+        return s.decode() if s else ""
+
+    def stats(self) -> dict:
+        st = N.Stats()
+        N.lib().tpcf_get_stats(self._h, C.byref(st))
+        return st.as_dict()
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context()
+    return _default_ctx
+
+
+# ---------------------------------------------------------------- core call ----
+def _host(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str = "f32", marks: bool = False,
+              patch_A=None, patch_B=None, n_patch_A: int = 1, n_patch_B: int = 1, real_bytes: int = 4,
+              device_inputs: bool = False, verbose: bool = False, ctx: Optional[Context] = None,
+              n_A: Optional[int] = None, n_B: Optional[int] = None) -> Shear2pcfResults:
+    """One call of the C-ABI.  ``lens``: x, y, val, w (+ z for shear);
+    ``src``: x, y, w + (z, e1, e2, m) for shear or (kappa_E, kappa_B) for kappa.
+    With ``device_inputs`` the values are device addresses (ints) of arrays of
+    the given precision and ``n_A``/``n_B`` give the sizes."""
+    ctx = ctx or default_context()
+    L = N.lib()
+    dt = np.float32 if precision == "f32" else np.float64
+    keep = []
+
+    def ptr(v, dtype=dt):
+        if v is None:
+            return None
+        if device_inputs:
+            return C.c_void_p(int(v))
+        a = _host(v, dtype)
+        keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+    if not device_inputs:
+        n_A = len(lens["x"])
+        n_B = len(src["x"])
+    nb = bins.n_bin
+    rt = np.float32 if real_bytes == 4 else np.float64
+    res = {k: np.zeros(nb, dtype=rt) for k in ("theta", "mean_theta", "g_t", "g_x", "weights")}
+    res["n_pairs"] = np.zeros(nb, dtype=np.int64)
+    for k in ("sum_theta", "sum_gt", "sum_gx", "sum_w", "sum_k"):
+        res[k] = np.zeros(nb, dtype=np.float64)
+    o = N.Out()
+    o.real_bytes = real_bytes
+    for k, v in res.items():
+        setattr(o, k, v.ctypes.data_as(C.c_void_p))
+    if marks:
+        for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
+            res[k] = np.zeros((n_A, nb), dtype=rt)
+            setattr(o, k, res[k].ctypes.data_as(C.c_void_p))
+        res["marks_weight"] = np.zeros(n_A, dtype=rt)
+        o.marks_weight = res["marks_weight"].ctypes.data_as(C.c_void_p)
+    patches = (patch_A is not None) or (patch_B is not None)
+    if patches:
+        P_A = n_patch_A if patch_A is not None else 1
+        P_B = n_patch_B if patch_B is not None else 1
+        res["pp_n"] = np.zeros((P_B, P_A, nb), dtype=np.int64)
+        o.pp_n = res["pp_n"].ctypes.data_as(C.c_void_p)
+        for k in ("pp_theta", "pp_gt", "pp_gx", "pp_w", "pp_k"):
+            res[k] = np.zeros((P_B, P_A, nb), dtype=np.float64)
+            setattr(o, k, res[k].ctypes.data_as(C.c_void_p))
+    flags = (N.FLAG_DEVICE_INPUTS if device_inputs else 0) | (N.FLAG_VERBOSE if verbose else 0)
+    pa = ptr(patch_A, np.int32) if patch_A is not None else None
+    pb = ptr(patch_B, np.int32) if patch_B is not None else None
+    if kind == SHEAR:
+        fn = L.tpcf_shear_f32 if precision == "f32" else L.tpcf_shear_f64
+        rc = fn(ctx.handle, n_A, ptr(lens["x"]), ptr(lens["y"]), ptr(lens["z"]), ptr(lens["val"]), ptr(lens["w"]),
+                n_B, ptr(src["x"]), ptr(src["y"]), ptr(src["z"]), ptr(src["e1"]), ptr(src["e2"]), ptr(src["w"]),
+                ptr(src["m"]), C.byref(bins.c), flags, pa, pb, n_patch_A, n_patch_B, C.byref(o))
+    else:
+        fn = L.tpcf_kappa_f32 if precision == "f32" else L.tpcf_kappa_f64
+        rc = fn(ctx.handle, n_A, ptr(lens["x"]), ptr(lens["y"]), ptr(lens["val"]), ptr(lens["w"]),
+                n_B, ptr(src["x"]), ptr(src["y"]), ptr(src["kappa_E"]), ptr(src["kappa_B"]), ptr(src["w"]),
+                C.byref(bins.c), flags, pa, pb, n_patch_A, n_patch_B, C.byref(o))
+    if rc:
+        raise TpcfError(rc, ctx.last_error())
+    out = Shear2pcfResults(n_bin=nb, stats=ctx.stats(), **{k: v for k, v in res.items() if not k.endswith("marks")
+                                                             and k != "marks_weight"})
+    if marks:
+        for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
+            setattr(out, k, res[k].T)   # (n_bin, n_A), Fortran order, no copy
+        out.marks_weight = res["marks_weight"]
+    return out
+
+
+def _precision_of(arr) -> str:
+    return "f64" if np.asarray(arr).dtype == np.float64 else "f32"
+
+
+def shear_2pcf(field: LensingSurveyField, bins: Binning, results=None, spherical_coords: bool = False,
+               num_threads: int = 1, use_kd_tree: bool = True, verbose: bool = False, *, marks: bool = True,
+               precision: Optional[str] = None, patch_A=None, patch_B=None, n_patch_A: int = 1, n_patch_B: int = 1,
+               ctx: Optional[Context] = None) -> Shear2pcfResults:
+    """shear_2pcf(field, bins, results, spherical_coords, num_threads,
+    use_kd_tree, verbose), src/tpcf.f90:134-160.  Returns the results object
+    (the reference fills its ``results`` argument)."""
+    if spherical_coords:
+        raise TpcfError(N.ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build")
+    L, S = field.lenses, field.shapes
+    prec = precision or _precision_of(L.x)
+    lens = dict(x=L.x, y=L.y, z=L.z if L.z is not None else np.zeros(L.n), val=L.val, w=L.w)
+    src = dict(x=S.x, y=S.y, z=S.z, e1=S.e1, e2=S.e2, w=S.w, m=S.m)
+    return correlate(SHEAR, lens, src, bins, precision=prec, marks=marks, patch_A=patch_A, patch_B=patch_B,
+                     n_patch_A=n_patch_A, n_patch_B=n_patch_B, real_bytes=4 if prec == "f32" else 8,
+                     verbose=verbose, ctx=ctx)
+
+
+def kappa_2pcf(field: LensingSurveyField, bins: Binning, results=None, spherical_coords: bool = False,
+               num_threads: int = 1, use_kd_tree: bool = True, verbose: bool = False, *, marks: bool = True,
+               precision: Optional[str] = None, patch_A=None, patch_B=None, n_patch_A: int = 1, n_patch_B: int = 1,
+               ctx: Optional[Context] = None) -> Shear2pcfResults:
+    """kappa_2pcf(field, bins, results, spherical_coords, num_threads,
+    use_kd_tree, verbose), src/tpcf.f90:162-188."""
+    if spherical_coords:
+        raise TpcfError(N.ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build")
+    L, K = field.lenses, field.convergence
+    prec = precision or _precision_of(L.x)
+    lens = dict(x=L.x, y=L.y, val=L.val, w=L.w)
+    src = dict(x=K.x, y=K.y, kappa_E=K.kappa_E, kappa_B=K.kappa_B, w=K.w)
+    return correlate(KAPPA, lens, src, bins, precision=prec, marks=marks, patch_A=patch_A, patch_B=patch_B,
+                     n_patch_A=n_patch_A, n_patch_B=n_patch_B, real_bytes=4 if prec == "f32" else 8,
+                     verbose=verbose, ctx=ctx)
+
+
+def stack_fields(results: list[Shear2pcfResults]) -> dict:
+    """Weight-stacking of per-field results, src/main.f90:678-695:
+    g = sum_f W_f g_f / sum_f W_f, n_pairs = sum_f N_f, mean_theta likewise."""
+    W = sum(np.asarray(r.weights, dtype=np.float64) for r in results)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        g_t = sum(np.asarray(r.weights, np.float64) * np.asarray(r.g_t, np.float64) for r in results) / W
+        g_x = sum(np.asarray(r.weights, np.float64) * np.asarray(r.g_x, np.float64) for r in results) / W
+        mt = sum(np.asarray(r.weights, np.float64) * np.asarray(r.mean_theta, np.float64) for r in results) / W
+    return dict(theta=results[0].theta, g_t=g_t, g_x=g_x, weights=W,
+                n_pairs=sum(np.asarray(r.n_pairs) for r in results), mean_theta=mt)

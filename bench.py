@@ -1,0 +1,159 @@
+"""Benchmark of the pair-counting hot path on one MI355X.
+
+A step is one full correlation call (tpcf_shear_f32 / tpcf_kappa_*) over the
+workload with the catalogues already resident in HBM: cell sort, segment and
+chunk tables, the pair kernel, the patch-pair reduction and the copy of the
+per-bin and per-patch-pair results to the host.  ``value`` is in-bin pairs per
+second of wall-clock time over the timed steps.
+
+Default workload: 20M kappa lenses x 20M shear sources on a 3000'x3000'
+lognormal-clustered patch, 30 log bins 0.5'-250', 128 patches, FP32 geometry
+with FP64 re-binning of pairs within 1e-5 of an edge and FP64 sums.
+
+The roofline line is for the dominant kernel (the pair kernel): achieved
+TFLOP/s = in-bin pairs x FLOPs/pair / pair-kernel time (HIP events on the
+library stream), against the MI355X FP32 vector peak.  The CPU baseline is
+the host-core port of the reference algorithm (kd-tree, FP32, OpenMP) timed on
+a lens subsample against all sources.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_PAIR = {0: 33, 1: 17}     # shear-scalar, kappa-kappa (algorithmic count)
+FP32_PEAK_TFLOPS = 157.3            # MI355X published FP32 vector peak
+METRIC = "pairs/sec on 1 MI355X vs host-core ref; FP32+FP64-edge vs FP64 at ≤1e-6 rel"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, pairs_per_lens: float, budget_s: float, threads: int) -> dict:
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle   # the host-core port of the reference algorithm (checker family)
+
+    rng = np.random.default_rng(12345)
+    n_a = len(wl.lens["x"])
+    spacing = 1
+    # calibrate on a few lenses, then size the sample for ~budget_s of pair work
+    probe = min(n_a, max(8, int(2e7 / max(pairs_per_lens, 1.0))))
+    sel = rng.choice(n_a, size=probe, replace=False)
+    sub = {k: v[sel] for k, v in wl.lens.items()}
+    t0 = time.perf_counter()
+    r = oracle.ref_port(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
+    dt = time.perf_counter() - t0 - r["tree_seconds"]
+    rate = max(r["n_pairs"].sum(), 1) / max(dt, 1e-6)
+    n_s = int(min(n_a, max(probe, budget_s * rate / max(pairs_per_lens, 1.0))))
+    sel = rng.choice(n_a, size=n_s, replace=False)
+    sub = {k: v[sel] for k, v in wl.lens.items()}
+    t0 = time.perf_counter()
+    r = oracle.ref_port(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
+    total = time.perf_counter() - t0
+    pair_time = total - r["tree_seconds"]
+    pairs = int(r["n_pairs"].sum())
+    return {"value": pairs / pair_time, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"{n_s} random lenses of {n_a} against all {len(wl.src['x'])} sources; "
+                      f"{pairs} in-bin pairs in {pair_time:.2f} s; kd-tree build over the sources "
+                      f"{r['tree_seconds']:.2f} s reported apart",
+            "tree_build_s": r["tree_seconds"]}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="sk_20Mx20M_lognormal_f32")
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink sizes (constant density); 1 = full")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cell", type=float, default=0.0, help="source cell side (0: automatic)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    tp = importlib.import_module("2pcf_amd")
+    synth = importlib.import_module("2pcf_amd.synth")
+    t0 = time.perf_counter()
+    wl = synth.config(args.config, args.scale)
+    log(f"workload {wl.name} generated in {time.perf_counter() - t0:.1f} s: "
+        f"{len(wl.lens['x'])} lenses x {len(wl.src['x'])} sources")
+    prec = "f64" if wl.lens["x"].dtype == np.float64 else "f32"
+    dev = torch.device("cuda:0")
+    dl = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in wl.lens.items()}
+    ds = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in wl.src.items()}
+    pa = torch.from_numpy(wl.patch_A).to(dev) if wl.patch_A is not None else None
+    pb = torch.from_numpy(wl.patch_B).to(dev) if wl.patch_B is not None else None
+    torch.cuda.synchronize()
+    bins = tp.create_bins(wl.x_min, wl.x_max, wl.n_bin, tp.LOG)
+    ctx = tp.Context(0)
+    if args.cell > 0:
+        ctx.set_option(1, args.cell)
+
+    def step():
+        return tp.correlate(wl.kind, {k: v.data_ptr() for k, v in dl.items()},
+                            {k: v.data_ptr() for k, v in ds.items()}, bins, precision=prec,
+                            patch_A=pa.data_ptr() if pa is not None else None,
+                            patch_B=pb.data_ptr() if pb is not None else None,
+                            n_patch_A=wl.n_patch, n_patch_B=wl.n_patch, real_bytes=4, device_inputs=True,
+                            ctx=ctx, n_A=len(wl.lens["x"]), n_B=len(wl.src["x"]))
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"warmup {i}: {r.stats['ms_total']:.1f} ms (pairs kernel {r.stats['ms_pairs']:.1f} ms)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms, pairs, stats = [], 0, None
+    for i in range(args.steps):
+        r = step()
+        kernel_ms.append(r.stats["ms_pairs"])
+        pairs += int(r.n_pairs.sum())
+        stats = r.stats
+        log(f"step {i}: {r.stats['ms_total']:.1f} ms (prepare {r.stats['ms_prepare']:.1f}, "
+            f"pairs kernel {r.stats['ms_pairs']:.1f}, finish {r.stats['ms_finish']:.1f})")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = pairs / elapsed
+    per_step_pairs = pairs / args.steps
+    k_ms = float(np.mean(kernel_ms))
+    achieved = per_step_pairs * FLOPS_PER_PAIR[wl.kind] / (k_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": prec, "data": "synthetic",
+        "config": {"workload": wl.name, "n_lens": len(wl.lens["x"]), "n_source": len(wl.src["x"]),
+                   "n_bin": wl.n_bin, "theta": [wl.x_min, wl.x_max], "patches": wl.n_patch,
+                   "kind": "shear-scalar" if wl.kind == 0 else "kappa-kappa", "scale": args.scale,
+                   "pairs_per_step": per_step_pairs, "cell_size": stats["cell_size"],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": FLOPS_PER_PAIR[wl.kind]},
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        ppl = per_step_pairs / max(stats["n_lens_used"], 1)
+        out["cpu_baseline"] = cpu_baseline(wl, ppl, args.cpu_seconds, threads)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,186 @@
+/*
+ * tpcf_gpu.h -- C-ABI of the MI355X (gfx950) pair-counting library.
+ *
+ * Drop-in for the correlation call of tilmantroester/2pcf:
+ *   call shear_2pcf(field, bins, results, spherical_coords, num_threads, use_kd_tree, verbose)
+ *   call kappa_2pcf(field, bins, results, spherical_coords, num_threads, use_kd_tree, verbose)
+ * (src/tpcf.f90:134-160 and :162-188), whose work is done by
+ * tangential_shear_tpcf (src/tpcf.f90:190-378) and scalar_kappa_tpcf
+ * (src/tpcf.f90:380-545).  The reference has no FFI; the Fortran module
+ * tpcf_gpu (2pcf_amd/fortran/tpcf_gpu.f90) binds these entry points with
+ * ISO_C_BINDING and keeps the reference's subroutine signatures.
+ *
+ * Conventions
+ *  - Arrays are caller-owned, contiguous, 0-based.  Two-dimensional outputs
+ *    use Fortran's column-major order: marks are [n_A][n_bin] with the bin
+ *    index fastest (the reference's real(n_bin, n_A), src/tpcf.f90:204-205),
+ *    per-patch-pair sums are [P_B][P_A][n_bin] (real(n_bin, P_A, P_B)).
+ *  - Every entry point is synchronous: it returns after the outputs are
+ *    written.  A context must not be used by two host threads at once.
+ *  - Status codes replace the reference's print/stop error handling; the
+ *    library never exits the process.
+ *  - Outputs are overwritten, never accumulated (the reference zeroes its
+ *    intent(out) arrays, src/tpcf.f90:242-252).
+ */
+#ifndef TPCF_GPU_H
+#define TPCF_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Largest n_bin the kernels support (one bin edge per wavefront lane). */
+#define TPCF_MAX_BIN 63
+
+/* Bin spacing enum of src/utils.f90:7. */
+enum tpcf_spacing { TPCF_LIN = 0, TPCF_LOG = 1, TPCF_IHS = 2, TPCF_SQRT = 3 };
+
+enum tpcf_status {
+    TPCF_OK = 0,
+    TPCF_ERR_ARG = 1,          /* bad size, NULL required pointer, bad flag */
+    TPCF_ERR_BINS = 2,         /* ill-defined binning (src/utils.f90:161-206 stops) */
+    TPCF_ERR_ALLOC = 3,        /* device allocation failed */
+    TPCF_ERR_HIP = 4,          /* a HIP runtime call failed */
+    TPCF_ERR_UNSUPPORTED = 5,  /* option this build does not implement */
+    TPCF_ERR_NONFINITE = 6,    /* a coordinate is NaN or Inf */
+    TPCF_ERR_NODEVICE = 7,     /* no usable GPU */
+    TPCF_ERR_PATCH = 8,        /* patch label outside [0, n_patch) */
+    TPCF_ERR_GRID = 9          /* catalogue extent too large for the cell grid */
+};
+
+/* Flags of the correlation calls. */
+enum tpcf_flags {
+    TPCF_DEVICE_INPUTS = 1u << 0,  /* input arrays (and patch labels) are device pointers */
+    TPCF_SPHERICAL = 1u << 1,      /* spherical_coords = .true. (not implemented: TPCF_ERR_UNSUPPORTED) */
+    TPCF_VERBOSE = 1u << 2,        /* verbose = .true.: print a summary to stdout */
+    TPCF_NO_KD_TREE = 1u << 3      /* use_kd_tree = .false.: accepted; the pair set is identical */
+};
+
+/* type(binning), src/utils.f90:18-24, plus the FP64 edges used for binning. */
+typedef struct tpcf_bins {
+    float x_min, x_max, bin_size;
+    int32_t n_bin;
+    int32_t spacing;               /* enum tpcf_spacing */
+    float ihs_theta;
+    double edges[TPCF_MAX_BIN + 1];    /* edges[0] = x_min, edges[n_bin] = x_max */
+    float bin_centers[TPCF_MAX_BIN];   /* arithmetic midpoints, src/utils.f90:229 */
+} tpcf_bins;
+
+/* Outputs.  Each pointer may be NULL (that output is skipped).  "real"
+ * outputs have real_bytes bytes per element: 4 for default real, 8 for a
+ * -fdefault-real-8 build.  Pointers are host pointers. */
+typedef struct tpcf_out {
+    int32_t real_bytes;
+    /* type(shear_2pcf_results), src/tpcf.f90:17-22, per bin [n_bin] */
+    void *theta;          /* bin centres (src/tpcf.f90:151 / :179) */
+    void *mean_theta;     /* Theta/W, NaN for an empty bin (src/tpcf.f90:369) */
+    void *g_t;            /* G_t/K where K>0 else 0 (kappa: xi_E) */
+    void *g_x;            /* G_x/K where K>0 else 0 (kappa: xi_B) */
+    void *weights;        /* W */
+    int64_t *n_pairs;
+    /* raw FP64 sums per bin [n_bin]: Theta, G_t, G_x, W and K (K is internal
+     * in the reference, exported here for stacking and bootstrap) */
+    double *sum_theta, *sum_gt, *sum_gx, *sum_w, *sum_k;
+    /* per-lens marks [n_A][n_bin] and marks_weight [n_A] (src/tpcf.f90:354-358) */
+    void *xi_E_marks, *xi_B_marks, *K_marks, *w_marks, *marks_weight;
+    /* per-patch-pair raw sums [P_B][P_A][n_bin]; summed over P_B they are the
+     * per-block sums of src/tpcf.f90:1130-1137 */
+    int64_t *pp_n;
+    double *pp_theta, *pp_gt, *pp_gx, *pp_w, *pp_k;
+} tpcf_out;
+
+/* Timings and counters of the last correlation call on a context. */
+typedef struct tpcf_stats {
+    double ms_total;        /* whole call, host wall clock */
+    double ms_upload;       /* H2D copies (0 with TPCF_DEVICE_INPUTS) */
+    double ms_prepare;      /* bbox, sort into cells, segment and chunk tables (HIP events) */
+    double ms_pairs;        /* the pair kernel (HIP events on the library stream) */
+    double ms_finish;       /* reduction, normalisation and D2H */
+    int64_t n_pairs;        /* in-bin pairs counted */
+    int64_t n_lens_used;    /* lenses with w != 0 (and z >= 0 for shear) */
+    int64_t n_source_used;  /* sources kept (z >= 0 for shear) */
+    int64_t n_chunks;       /* lens chunks of up to 64 lenses */
+    int64_t n_segments;     /* non-empty (cell, patch) source segments */
+    double cell_size;       /* source cell side used */
+    int64_t seg_visits[3];  /* fast / edge / multi segment visits (only with TPCF_OPT_COUNTERS) */
+    int64_t pair_visits[3]; /* lens-source pairs examined per class (same) */
+} tpcf_stats;
+
+/* Options for tpcf_set_option. */
+enum tpcf_option {
+    TPCF_OPT_CELL_SIZE = 1,     /* > 0: source cell side; 0: automatic (default) */
+    TPCF_OPT_COUNTERS = 2,      /* != 0: collect seg_visits / pair_visits (slower) */
+    TPCF_OPT_CELL_TARGET = 3    /* automatic cell size: mean sources per cell (default 48) */
+};
+
+typedef struct tpcf_ctx tpcf_ctx;
+
+/* create_bins(x_min=, x_max=, n_bin=, p=, spacing=), src/utils.f90:139-230. */
+int tpcf_create_bins(float x_min, float x_max, int32_t n_bin, int32_t spacing, float ihs_theta,
+                     tpcf_bins *bins);
+
+/* find_bin_index (src/utils.f90:101-137) under the library's bin definition:
+ * 0-based bin k with edges[k]^2 <= x^2 < edges[k+1]^2 evaluated in FP64,
+ * x == x_max -> n_bin-1, -1 outside [x_min, x_max]. */
+int32_t tpcf_find_bin_index(double x, const tpcf_bins *bins);
+
+/* Context: device, stream and reusable device buffers.  device < 0 picks the
+ * current device. */
+int tpcf_ctx_create(int32_t device, tpcf_ctx **ctx);
+int tpcf_ctx_destroy(tpcf_ctx *ctx);
+int tpcf_set_option(tpcf_ctx *ctx, int32_t option, double value);
+/* Use a caller-owned hipStream_t (NULL restores the library's own stream). */
+int tpcf_set_stream(tpcf_ctx *ctx, void *hip_stream);
+int tpcf_get_stats(const tpcf_ctx *ctx, tpcf_stats *stats);
+const char *tpcf_last_error(const tpcf_ctx *ctx);
+
+/* shear_2pcf / tangential_shear_tpcf (src/tpcf.f90:134-160, :190-378), FP32
+ * inputs: lenses x,y,z,val,w [n_A]; sources x,y,z,e1,e2,w,m [n_B].  Geometry
+ * and per-pair products in FP32, pairs within 1e-5 (relative, on d^2) of a bin
+ * edge re-binned in FP64, sums in FP64.  patch_A / patch_B (nullable) are
+ * 0-based labels in [0, n_patch_A) / [0, n_patch_B). */
+int tpcf_shear_f32(tpcf_ctx *ctx,
+                   int64_t n_A, const float *x_A, const float *y_A, const float *z_A,
+                   const float *val_A, const float *w_A,
+                   int64_t n_B, const float *x_B, const float *y_B, const float *z_B,
+                   const float *e1_B, const float *e2_B, const float *w_B, const float *m_B,
+                   const tpcf_bins *bins, uint32_t flags,
+                   const int32_t *patch_A, const int32_t *patch_B,
+                   int32_t n_patch_A, int32_t n_patch_B, tpcf_out *out);
+
+/* kappa_2pcf / scalar_kappa_tpcf (src/tpcf.f90:162-188, :380-545), FP32
+ * inputs: A x,y,val,w [n_A]; B x,y,kappa_E,kappa_B,w [n_B]. */
+int tpcf_kappa_f32(tpcf_ctx *ctx,
+                   int64_t n_A, const float *x_A, const float *y_A, const float *val_A, const float *w_A,
+                   int64_t n_B, const float *x_B, const float *y_B, const float *kappa_E,
+                   const float *kappa_B, const float *w_B,
+                   const tpcf_bins *bins, uint32_t flags,
+                   const int32_t *patch_A, const int32_t *patch_B,
+                   int32_t n_patch_A, int32_t n_patch_B, tpcf_out *out);
+
+/* FP64 twins (the reference built with -fdefault-real-8): FP64 geometry,
+ * binning and sums throughout. */
+int tpcf_shear_f64(tpcf_ctx *ctx,
+                   int64_t n_A, const double *x_A, const double *y_A, const double *z_A,
+                   const double *val_A, const double *w_A,
+                   int64_t n_B, const double *x_B, const double *y_B, const double *z_B,
+                   const double *e1_B, const double *e2_B, const double *w_B, const double *m_B,
+                   const tpcf_bins *bins, uint32_t flags,
+                   const int32_t *patch_A, const int32_t *patch_B,
+                   int32_t n_patch_A, int32_t n_patch_B, tpcf_out *out);
+
+int tpcf_kappa_f64(tpcf_ctx *ctx,
+                   int64_t n_A, const double *x_A, const double *y_A, const double *val_A, const double *w_A,
+                   int64_t n_B, const double *x_B, const double *y_B, const double *kappa_E,
+                   const double *kappa_B, const double *w_B,
+                   const tpcf_bins *bins, uint32_t flags,
+                   const int32_t *patch_A, const int32_t *patch_B,
+                   int32_t n_patch_A, int32_t n_patch_B, tpcf_out *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPCF_GPU_H */

@@ -1,0 +1,266 @@
+"""ctypes front-end of the CPU oracle (``tpcf_oracle.c``) and of the host-core
+timing port (``ref_port.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` -- always as the checker of the HIP
+path, never as the thing measured or shipped.  The product package does not
+import this module.
+
+Reference behaviour restated (tilmantroester/2pcf):
+  * ``edges``      -- create_bins, src/utils.f90:139-230 (FP64 closed form)
+  * ``bin_of_d2``  -- find_bin_index, src/utils.f90:101-137 (squared form)
+  * ``correlate``  -- tangential_shear_tpcf src/tpcf.f90:190-378 and
+                      scalar_kappa_tpcf src/tpcf.f90:380-545, every pair
+                      visited, FP64 geometry, long-double sums
+  * ``finalize``   -- src/tpcf.f90:369-376 / :536-543
+  * ``ref_port``   -- the reference algorithm (kd-tree, FP32, OpenMP) for timing
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+SHEAR, KAPPA = 0, 1
+LIN, LOG, IHS, SQRT = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class _OrcArgs(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("n_bin", C.c_int32), ("n_a", C.c_int64), ("n_b", C.c_int64),
+        ("xa", _dp), ("ya", _dp), ("za", _dp), ("vala", _dp), ("wa", _dp),
+        ("xb", _dp), ("yb", _dp), ("zb", _dp), ("s1b", _dp), ("s2b", _dp), ("wb", _dp), ("mb", _dp),
+        ("edges", _dp), ("patch_a", _i32p), ("patch_b", _i32p), ("n_pa", C.c_int32), ("n_pb", C.c_int32),
+        ("brute", C.c_int32), ("n_threads", C.c_int32),
+    ]
+
+
+class _OrcOut(C.Structure):
+    _fields_ = [
+        ("n_pairs", _i64p), ("theta", _dp), ("gt", _dp), ("gx", _dp), ("w", _dp), ("k", _dp),
+        ("xi_e", _dp), ("xi_b", _dp), ("k_marks", _dp), ("w_marks", _dp), ("marks_weight", _dp),
+        ("pp_n", _i64p), ("pp_theta", _dp), ("pp_gt", _dp), ("pp_gx", _dp), ("pp_w", _dp), ("pp_k", _dp),
+        ("gt_abs", _dp), ("gx_abs", _dp),
+    ]
+
+
+class _RpBins(C.Structure):
+    _fields_ = [("x_min", C.c_float), ("x_max", C.c_float), ("bin_size", C.c_float),
+                ("ihs_theta", C.c_float), ("n_bin", C.c_int32), ("spacing", C.c_int32)]
+
+
+class _RpArgs(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("n_a", C.c_int32), ("n_b", C.c_int32),
+        ("xa", _fp), ("ya", _fp), ("za", _fp), ("vala", _fp), ("wa", _fp),
+        ("xb", _fp), ("yb", _fp), ("zb", _fp), ("s1b", _fp), ("s2b", _fp), ("wb", _fp), ("mb", _fp),
+        ("bins", _RpBins), ("n_threads", C.c_int32), ("use_tree", C.c_int32),
+    ]
+
+
+class _RpOut(C.Structure):
+    _fields_ = [("mean_theta", _fp), ("g_t", _fp), ("g_x", _fp), ("weights", _fp), ("n_pairs", _i64p),
+                ("xi_e", _fp), ("xi_b", _fp), ("k_marks", _fp), ("w_marks", _fp), ("marks_weight", _fp),
+                ("tree_seconds", C.c_double)]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _lib.orc_edges.argtypes = [C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, _dp]
+        _lib.orc_edges.restype = C.c_int
+        _lib.orc_edges_squared.argtypes = [_dp, C.c_int, _dp]
+        _lib.orc_bin_of_d2_array.argtypes = [C.c_int64, _dp, _dp, C.c_int, _i32p]
+        _lib.orc_correlate.argtypes = [C.POINTER(_OrcArgs), C.POINTER(_OrcOut)]
+        _lib.orc_correlate.restype = C.c_int
+        _lib.orc_finalize.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        _lib.rp_create_bins.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.POINTER(_RpBins)]
+        _lib.rp_create_bins.restype = C.c_int
+        _lib.rp_find_bin_array.argtypes = [C.c_int64, _fp, C.POINTER(_RpBins), _i32p]
+        _lib.rp_correlate.argtypes = [C.POINTER(_RpArgs), C.POINTER(_RpOut)]
+        _lib.rp_correlate.restype = C.c_int
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def edges(spacing: int, x_min: float, x_max: float, n_bin: int, ihs_theta: float = 1.0) -> np.ndarray:
+    e = np.zeros(n_bin + 1, dtype=np.float64)
+    rc = lib().orc_edges(spacing, x_min, x_max, n_bin, ihs_theta, _ptr(e, _dp))
+    if rc:
+        raise ValueError("ill-defined binning")
+    return e
+
+
+def bin_of_d2(d2, edges_: np.ndarray) -> np.ndarray:
+    d2 = _d(np.atleast_1d(d2))
+    e = _d(edges_)
+    n = len(e) - 1
+    e2 = np.zeros_like(e)
+    lib().orc_edges_squared(_ptr(e, _dp), n, _ptr(e2, _dp))
+    out = np.zeros(len(d2), dtype=np.int32)
+    lib().orc_bin_of_d2_array(len(d2), _ptr(d2, _dp), _ptr(e2, _dp), n, _ptr(out, _i32p))
+    return out
+
+
+def correlate(kind: int, lens: dict, src: dict, edges_: np.ndarray, patch_a=None, patch_b=None,
+              n_pa: int = 1, n_pb: int = 1, brute: bool = False, marks: bool = False,
+              patches: bool = False, n_threads: int | None = None) -> dict:
+    """Raw FP64 sums.  ``lens`` has x, y, val, w (+ z for shear); ``src`` has
+    x, y, w and e1, e2, m, z (shear) or kappa_E, kappa_B (kappa)."""
+    L = lib()
+    e = _d(edges_)
+    n_bin = len(e) - 1
+    keep = []
+
+    def arr(dct, key, n):
+        v = dct.get(key)
+        a = _d(v) if v is not None else np.zeros(n, dtype=np.float64)
+        keep.append(a)
+        return a
+
+    n_a = len(lens["x"])
+    n_b = len(src["x"])
+    a = _OrcArgs()
+    a.kind = kind
+    a.n_bin = n_bin
+    a.n_a = n_a
+    a.n_b = n_b
+    a.xa, a.ya = _ptr(arr(lens, "x", n_a), _dp), _ptr(arr(lens, "y", n_a), _dp)
+    a.za = _ptr(arr(lens, "z", n_a), _dp)
+    a.vala, a.wa = _ptr(arr(lens, "val", n_a), _dp), _ptr(arr(lens, "w", n_a), _dp)
+    a.xb, a.yb = _ptr(arr(src, "x", n_b), _dp), _ptr(arr(src, "y", n_b), _dp)
+    a.wb = _ptr(arr(src, "w", n_b), _dp)
+    if kind == SHEAR:
+        a.zb = _ptr(arr(src, "z", n_b), _dp)
+        a.s1b, a.s2b = _ptr(arr(src, "e1", n_b), _dp), _ptr(arr(src, "e2", n_b), _dp)
+        a.mb = _ptr(arr(src, "m", n_b), _dp)
+    else:
+        a.zb = _ptr(arr(src, "z", n_b), _dp)
+        a.s1b, a.s2b = _ptr(arr(src, "kappa_E", n_b), _dp), _ptr(arr(src, "kappa_B", n_b), _dp)
+        a.mb = _ptr(arr(src, "m", n_b), _dp)
+    a.edges = _ptr(e, _dp)
+    pa = pb = None
+    if patches:
+        pa = np.ascontiguousarray(patch_a if patch_a is not None else np.zeros(n_a), dtype=np.int32)
+        pb = np.ascontiguousarray(patch_b if patch_b is not None else np.zeros(n_b), dtype=np.int32)
+        a.patch_a, a.patch_b = _ptr(pa, _i32p), _ptr(pb, _i32p)
+    a.n_pa, a.n_pb = n_pa, n_pb
+    a.brute = 1 if brute else 0
+    a.n_threads = n_threads or min(16, os.cpu_count() or 1)
+
+    res = {
+        "n_pairs": np.zeros(n_bin, dtype=np.int64),
+        "theta": np.zeros(n_bin), "gt": np.zeros(n_bin), "gx": np.zeros(n_bin),
+        "w": np.zeros(n_bin), "k": np.zeros(n_bin), "gt_abs": np.zeros(n_bin), "gx_abs": np.zeros(n_bin),
+    }
+    o = _OrcOut()
+    o.n_pairs = _ptr(res["n_pairs"], _i64p)
+    for key in ("theta", "gt", "gx", "w", "k", "gt_abs", "gx_abs"):
+        setattr(o, key, _ptr(res[key], _dp))
+    if marks:
+        for key in ("xi_e", "xi_b", "k_marks", "w_marks"):
+            res[key] = np.zeros((n_a, n_bin))
+            setattr(o, key, _ptr(res[key], _dp))
+        res["marks_weight"] = np.zeros(n_a)
+        o.marks_weight = _ptr(res["marks_weight"], _dp)
+    if patches:
+        res["pp_n"] = np.zeros((n_pb, n_pa, n_bin), dtype=np.int64)
+        o.pp_n = _ptr(res["pp_n"], _i64p)
+        for key in ("pp_theta", "pp_gt", "pp_gx", "pp_w", "pp_k"):
+            res[key] = np.zeros((n_pb, n_pa, n_bin))
+            setattr(o, key, _ptr(res[key], _dp))
+    rc = L.orc_correlate(C.byref(a), C.byref(o))
+    if rc:
+        raise RuntimeError(f"oracle failed with code {rc}")
+    fin = finalize(res)
+    res.update(fin)
+    return res
+
+
+def finalize(res: dict) -> dict:
+    n_bin = len(res["theta"])
+    mt, gt, gx = np.zeros(n_bin), np.zeros(n_bin), np.zeros(n_bin)
+    lib().orc_finalize(n_bin, *[_ptr(_d(res[k]), _dp) for k in ("theta", "gt", "gx", "w", "k")],
+                       _ptr(mt, _dp), _ptr(gt, _dp), _ptr(gx, _dp))
+    return {"mean_theta": mt, "g_t": gt, "g_x": gx, "weights": np.asarray(res["w"], dtype=np.float64)}
+
+
+def rp_bins(x_min, x_max, n_bin, spacing, ihs_theta=1.0):
+    b = _RpBins()
+    if lib().rp_create_bins(x_min, x_max, n_bin, spacing, ihs_theta, C.byref(b)):
+        raise ValueError("ill-defined binning")
+    return b
+
+
+def rp_find_bin(x, x_min, x_max, n_bin, spacing, ihs_theta=1.0) -> np.ndarray:
+    """Reference FP32 find_bin_index (1-based, -1 outside)."""
+    b = rp_bins(x_min, x_max, n_bin, spacing, ihs_theta)
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float32)
+    out = np.zeros(len(x), dtype=np.int32)
+    lib().rp_find_bin_array(len(x), _ptr(x, _fp), C.byref(b), _ptr(out, _i32p))
+    return out
+
+
+def ref_port(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ihs_theta=1.0,
+             n_threads: int = 1, use_tree: bool = True, marks: bool = False) -> dict:
+    """The reference algorithm (kd-tree, FP32) for CPU timing."""
+    keep = []
+
+    def f(dct, key, n):
+        v = dct.get(key)
+        a = np.ascontiguousarray(v if v is not None else np.zeros(n), dtype=np.float32)
+        keep.append(a)
+        return _ptr(a, _fp)
+
+    n_a, n_b = len(lens["x"]), len(src["x"])
+    a = _RpArgs()
+    a.kind, a.n_a, a.n_b = kind, n_a, n_b
+    a.xa, a.ya, a.za, a.vala, a.wa = (f(lens, k, n_a) for k in ("x", "y", "z", "val", "w"))
+    s1, s2 = ("e1", "e2") if kind == SHEAR else ("kappa_E", "kappa_B")
+    a.xb, a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb = (f(src, k, n_b) for k in ("x", "y", "z", s1, s2, "w", "m"))
+    a.bins = rp_bins(x_min, x_max, n_bin, spacing, ihs_theta)
+    a.n_threads, a.use_tree = n_threads, 1 if use_tree else 0
+    res = {k: np.zeros(n_bin, dtype=np.float32) for k in ("mean_theta", "g_t", "g_x", "weights")}
+    res["n_pairs"] = np.zeros(n_bin, dtype=np.int64)
+    o = _RpOut()
+    for k in ("mean_theta", "g_t", "g_x", "weights"):
+        setattr(o, k, _ptr(res[k], _fp))
+    o.n_pairs = _ptr(res["n_pairs"], _i64p)
+    if marks:
+        for k in ("xi_e", "xi_b", "k_marks", "w_marks"):
+            res[k] = np.zeros((n_a, n_bin), dtype=np.float32)
+            setattr(o, k, _ptr(res[k], _fp))
+        res["marks_weight"] = np.zeros(n_a, dtype=np.float32)
+        o.marks_weight = _ptr(res["marks_weight"], _fp)
+    rc = lib().rp_correlate(C.byref(a), C.byref(o))
+    if rc:
+        raise RuntimeError(f"ref_port failed with code {rc}")
+    res["tree_seconds"] = o.tree_seconds
+    return res

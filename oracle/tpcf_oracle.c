@@ -1,0 +1,367 @@
+/*
+ * tpcf_oracle.c -- CPU FP64 oracle for 2pcf's binned pair counts.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The product library never links, loads or calls
+ * this file.  It is loaded (through oracle/oracle.py) by tests/, by
+ * __graft_entry__.smoke() and by the cpu_baseline leg of bench.py, always as
+ * the checker of the HIP path, never as the thing measured or shipped.
+ *
+ * What it restates (reference = tilmantroester/2pcf, Fortran):
+ *   orc_edges            bin edges of create_bins, src/utils.f90:139-230, as an
+ *                        FP64 closed form (lin / log / ihs / sqrt spacing, the
+ *                        enum of src/utils.f90:7); first edge = x_min and last
+ *                        edge = x_max exactly.
+ *   orc_bin_of_d2        find_bin_index, src/utils.f90:101-137, on squared
+ *                        separations: bin k <=> e_k^2 <= d^2 < e_{k+1}^2, with
+ *                        d^2 == e_n^2 (x == x_max) going to the last bin and
+ *                        anything outside [x_min, x_max] dropped.
+ *   orc_correlate        the per-lens loops of tangential_shear_tpcf
+ *                        (src/tpcf.f90:266-359) and scalar_kappa_tpcf
+ *                        (src/tpcf.f90:456-526): masks, per-bin sums, per-lens
+ *                        marks, with every pair visited as with --no-kd-tree
+ *                        (src/tpcf.f90:293-295), plus per-patch-pair sums whose
+ *                        sum over source patches is the per-block sum of
+ *                        src/tpcf.f90:1130-1137.
+ *   orc_finalize         the normalisation of src/tpcf.f90:369-376 / :536-543
+ *                        (mean_theta = Theta/W with no guard, g = G/K where K>0).
+ *
+ * Arithmetic: coordinates arrive as double (FP32 inputs are widened exactly);
+ * dx = x_A - x_B and dy = y_A - y_B are formed in FP64 and the squared
+ * separation is fma(dx, dx, dy*dy).  Sums are accumulated in long double.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { ORC_LIN = 0, ORC_LOG = 1, ORC_IHS = 2, ORC_SQRT = 3 };
+enum { ORC_SHEAR = 0, ORC_KAPPA = 1 };
+
+typedef struct {
+    int32_t kind;                 /* ORC_SHEAR or ORC_KAPPA */
+    int32_t n_bin;
+    int64_t n_a, n_b;
+    const double *xa, *ya, *za, *vala, *wa;          /* za unused for kappa */
+    const double *xb, *yb, *zb, *s1b, *s2b, *wb, *mb; /* shear: e1,e2,m ; kappa: kappa_E,kappa_B */
+    const double *edges;          /* n_bin+1 edges (not squared) */
+    const int32_t *patch_a, *patch_b; /* nullable, 0-based labels */
+    int32_t n_pa, n_pb;
+    int32_t brute;                /* 1: every pair; 0: exact cell pruning */
+    int32_t n_threads;
+} orc_args;
+
+typedef struct {
+    int64_t *n_pairs;             /* [n_bin] */
+    double *theta, *gt, *gx, *w, *k;  /* raw per-bin sums [n_bin] */
+    double *xi_e, *xi_b, *k_marks, *w_marks; /* nullable [n_a][n_bin], bin fastest */
+    double *marks_weight;         /* nullable [n_a] */
+    int64_t *pp_n;                /* nullable [n_pb][n_pa][n_bin] */
+    double *pp_theta, *pp_gt, *pp_gx, *pp_w, *pp_k;
+    double *gt_abs, *gx_abs;      /* nullable [n_bin]: sums of |term|, the scale of a signed sum */
+} orc_out;
+
+/* Transform of each spacing (src/utils.f90:83-90 for ihs) and its inverse. */
+static double tr_fwd(int spacing, double x, double p)
+{
+    switch (spacing) {
+    case ORC_LIN: return x;
+    case ORC_LOG: return log2(x);
+    case ORC_IHS: return asinh(p * x) / p;
+    default: return sqrt(x);
+    }
+}
+
+static double tr_inv(int spacing, double t, double p)
+{
+    switch (spacing) {
+    case ORC_LIN: return t;
+    case ORC_LOG: return exp2(t);
+    case ORC_IHS: return sinh(p * t) / p;
+    default: return t * t;
+    }
+}
+
+/* create_bins (src/utils.f90:139-230) with x_min, x_max, n_bin given: edge i is
+ * the inverse transform of T(x_min) + i*(T(x_max)-T(x_min))/n_bin.  The log
+ * spacing is written with log2/exp2 and x_min factored out so that power-of-two
+ * ratios give exact edges. */
+int orc_edges(int spacing, float x_min, float x_max, int n_bin, float ihs_theta, double *edges)
+{
+    if (n_bin < 1 || !(x_max > x_min) || !(x_min >= 0.0f)) return 1;
+    if (spacing < ORC_LIN || spacing > ORC_SQRT) return 1;
+    if (spacing == ORC_LOG && !(x_min > 0.0f)) return 1;
+    const double lo = (double)x_min, hi = (double)x_max, p = (double)ihs_theta;
+    if (spacing == ORC_IHS && !(p > 0.0)) return 1;
+    edges[0] = lo;
+    edges[n_bin] = hi;
+    if (spacing == ORC_LOG) {
+        const double span = log2(hi / lo);
+        for (int i = 1; i < n_bin; ++i) edges[i] = lo * exp2(((double)i * span) / (double)n_bin);
+    } else {
+        const double t0 = tr_fwd(spacing, lo, p), t1 = tr_fwd(spacing, hi, p);
+        for (int i = 1; i < n_bin; ++i)
+            edges[i] = tr_inv(spacing, t0 + ((double)i * (t1 - t0)) / (double)n_bin, p);
+    }
+    return 0;
+}
+
+/* Squared-edge table used by orc_bin_of_d2. */
+void orc_edges_squared(const double *edges, int n_bin, double *e2)
+{
+    for (int i = 0; i <= n_bin; ++i) e2[i] = edges[i] * edges[i];
+}
+
+/* find_bin_index (src/utils.f90:101-137) on squared separations. */
+int orc_bin_of_d2(double d2, const double *e2, int n_bin)
+{
+    if (d2 < e2[0] || d2 > e2[n_bin]) return -1;   /* x < x_min or x > x_max */
+    if (d2 == e2[n_bin]) return n_bin - 1;          /* x == x_max -> last bin */
+    int lo = 0, hi = n_bin;                          /* e2[lo] <= d2 < e2[hi] */
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (d2 >= e2[mid]) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+typedef struct {
+    long double theta, gt, gx, w, k, gt_abs, gx_abs;
+    int64_t n;
+} acc6;
+
+static void acc_add(acc6 *dst, const acc6 *src)
+{
+    dst->theta += src->theta; dst->gt += src->gt; dst->gx += src->gx;
+    dst->w += src->w; dst->k += src->k; dst->n += src->n;
+    dst->gt_abs += src->gt_abs; dst->gx_abs += src->gx_abs;
+}
+
+/* Exact pruning structure over B: square cells no smaller than x_max, so every
+ * pair with separation <= x_max lies in the 3x3 block around the lens cell. */
+typedef struct {
+    double x0, y0, cs;
+    int64_t nx, ny;
+    int64_t *start;   /* [nx*ny+1] */
+    int64_t *idx;     /* [n_b] source indices sorted by cell */
+} orc_grid;
+
+static int grid_build(orc_grid *g, const orc_args *a, double x_max)
+{
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (int64_t j = 0; j < a->n_b; ++j) {
+        if (a->xb[j] < xmin) xmin = a->xb[j];
+        if (a->xb[j] > xmax) xmax = a->xb[j];
+        if (a->yb[j] < ymin) ymin = a->yb[j];
+        if (a->yb[j] > ymax) ymax = a->yb[j];
+    }
+    double cs = x_max * 1.001 + 1e-300;
+    const double ext = fmax(xmax - xmin, ymax - ymin);
+    if (ext / cs > 4096.0) cs = ext / 4096.0;
+    g->x0 = xmin; g->y0 = ymin; g->cs = cs;
+    g->nx = (int64_t)floor((xmax - xmin) / cs) + 1;
+    g->ny = (int64_t)floor((ymax - ymin) / cs) + 1;
+    const int64_t nc = g->nx * g->ny;
+    g->start = (int64_t *)calloc((size_t)nc + 1, sizeof(int64_t));
+    g->idx = (int64_t *)malloc(sizeof(int64_t) * (size_t)(a->n_b > 0 ? a->n_b : 1));
+    int64_t *cell = (int64_t *)malloc(sizeof(int64_t) * (size_t)(a->n_b > 0 ? a->n_b : 1));
+    if (!g->start || !g->idx || !cell) { free(cell); return 1; }
+    for (int64_t j = 0; j < a->n_b; ++j) {
+        int64_t cx = (int64_t)floor((a->xb[j] - g->x0) / cs);
+        int64_t cy = (int64_t)floor((a->yb[j] - g->y0) / cs);
+        if (cx < 0) cx = 0;
+        if (cx >= g->nx) cx = g->nx - 1;
+        if (cy < 0) cy = 0;
+        if (cy >= g->ny) cy = g->ny - 1;
+        cell[j] = cy * g->nx + cx;
+        g->start[cell[j] + 1]++;
+    }
+    for (int64_t c = 0; c < nc; ++c) g->start[c + 1] += g->start[c];
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)nc);
+    if (!fill) { free(cell); return 1; }
+    memcpy(fill, g->start, sizeof(int64_t) * (size_t)nc);
+    for (int64_t j = 0; j < a->n_b; ++j) g->idx[fill[cell[j]]++] = j;
+    free(fill);
+    free(cell);
+    return 0;
+}
+
+static void grid_free(orc_grid *g)
+{
+    free(g->start);
+    free(g->idx);
+}
+
+/* One (lens i, source j) pair: src/tpcf.f90:302-358 (shear) / :491-525 (kappa). */
+static inline void visit_pair(const orc_args *a, const double *e2, int64_t i, int64_t j,
+                              acc6 *bins, acc6 *pp_row, long double *mk)
+{
+    const int n_bin = a->n_bin;
+    const int shear = (a->kind == ORC_SHEAR);
+    if (shear && (a->za[i] < 0.0 || a->zb[j] < 0.0)) return;   /* masked objects */
+    const double dx = a->xa[i] - a->xb[j];
+    const double dy = a->ya[i] - a->yb[j];
+    const double d2 = fma(dx, dx, dy * dy);
+    if (shear && a->za[i] > a->zb[j]) return;                  /* source in front of lens */
+    const int b = orc_bin_of_d2(d2, e2, n_bin);
+    if (b < 0) return;
+    const double d = sqrt(d2);
+    const double wA = a->wa[i], valA = a->vala[i], wB = a->wb[j];
+    double t, x, kw;
+    if (shear) {
+        const double e1 = a->s1b[j], e2v = a->s2b[j], m = a->mb[j];
+        const double c2 = 2.0 * (dx / d) * (dx / d) - 1.0;
+        const double s2 = 2.0 * dx * dy / (d * d);
+        t = -(e1 * c2 + e2v * s2) * wB * valA * wA;
+        x = -(-e1 * s2 + e2v * c2) * wB * valA * wA;
+        kw = wB * (1.0 + m) * wA;
+    } else {
+        t = a->s1b[j] * wB * valA * wA;
+        x = a->s2b[j] * wB * valA * wA;
+        kw = wB * wA;
+    }
+    const double ww = wB * wA;
+    acc6 *ab = &bins[b];
+    ab->theta += d * wB * wA; ab->gt += t; ab->gx += x; ab->w += ww; ab->k += kw; ab->n += 1;
+    ab->gt_abs += fabs(t); ab->gx_abs += fabs(x);
+    if (pp_row) {
+        const int32_t pb = a->patch_b ? a->patch_b[j] : 0;
+        acc6 *ap = &pp_row[(size_t)pb * n_bin + b];
+        ap->theta += d * wB * wA; ap->gt += t; ap->gx += x; ap->w += ww; ap->k += kw; ap->n += 1;
+    }
+    if (mk) {
+        long double *m4 = mk + (size_t)4 * b;
+        m4[0] += t; m4[1] += x; m4[2] += kw; m4[3] += ww;
+    }
+}
+
+int orc_correlate(const orc_args *a, orc_out *o)
+{
+    const int n_bin = a->n_bin;
+    if (n_bin < 1 || a->n_a < 0 || a->n_b < 0) return 1;
+    const int want_pp = (o->pp_n != NULL);
+    const int want_marks = (o->xi_e != NULL);
+    const int32_t n_pa = want_pp ? (a->n_pa > 0 ? a->n_pa : 1) : 1;
+    const int32_t n_pb = want_pp ? (a->n_pb > 0 ? a->n_pb : 1) : 1;
+    double *e2 = (double *)malloc(sizeof(double) * (size_t)(n_bin + 1));
+    if (!e2) return 2;
+    orc_edges_squared(a->edges, n_bin, e2);
+
+    orc_grid g;
+    memset(&g, 0, sizeof g);
+    if (!a->brute && grid_build(&g, a, a->edges[n_bin])) { free(e2); grid_free(&g); return 2; }
+
+    int nt = a->n_threads > 0 ? a->n_threads : 1;
+#ifndef _OPENMP
+    nt = 1;
+#endif
+    const size_t pp_len = (size_t)n_pa * n_pb * n_bin;
+    acc6 *tbins = (acc6 *)calloc((size_t)nt * n_bin, sizeof(acc6));
+    acc6 *tpp = want_pp ? (acc6 *)calloc((size_t)nt * pp_len, sizeof(acc6)) : NULL;
+    long double *tmk = want_marks ? (long double *)calloc((size_t)nt * 4 * n_bin, sizeof(long double)) : NULL;
+    int fail = (!tbins || (want_pp && !tpp) || (want_marks && !tmk));
+
+    if (!fail) {
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 16)
+#endif
+        for (int64_t i = 0; i < a->n_a; ++i) {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+            acc6 *bins = tbins + (size_t)tid * n_bin;
+            acc6 *pp_row = NULL;
+            if (want_pp) {
+                const int32_t pa = a->patch_a ? a->patch_a[i] : 0;
+                /* rows of the [n_pb][n_pa][n_bin] table for this lens patch are
+                 * strided; keep a per-thread [n_pa][n_pb][n_bin] copy instead */
+                pp_row = tpp + (size_t)tid * pp_len + (size_t)pa * n_pb * n_bin;
+            }
+            long double *mk = NULL;
+            if (want_marks) {
+                mk = tmk + (size_t)tid * 4 * n_bin;
+                memset(mk, 0, sizeof(long double) * 4 * (size_t)n_bin);
+            }
+            if (a->wa[i] != 0.0) {                     /* src/tpcf.f90:285 / :474 */
+                if (a->brute) {
+                    for (int64_t j = 0; j < a->n_b; ++j) visit_pair(a, e2, i, j, bins, pp_row, mk);
+                } else {
+                    int64_t cx = (int64_t)floor((a->xa[i] - g.x0) / g.cs);
+                    int64_t cy = (int64_t)floor((a->ya[i] - g.y0) / g.cs);
+                    for (int64_t yy = cy - 1; yy <= cy + 1; ++yy) {
+                        if (yy < 0 || yy >= g.ny) continue;
+                        for (int64_t xx = cx - 1; xx <= cx + 1; ++xx) {
+                            if (xx < 0 || xx >= g.nx) continue;
+                            const int64_t c = yy * g.nx + xx;
+                            for (int64_t q = g.start[c]; q < g.start[c + 1]; ++q)
+                                visit_pair(a, e2, i, g.idx[q], bins, pp_row, mk);
+                        }
+                    }
+                }
+            }
+            if (want_marks) {
+                double mw = 0.0;
+                for (int b = 0; b < n_bin; ++b) {
+                    const size_t o2 = (size_t)i * n_bin + b;
+                    o->xi_e[o2] = (double)mk[4 * b + 0];
+                    o->xi_b[o2] = (double)mk[4 * b + 1];
+                    o->k_marks[o2] = (double)mk[4 * b + 2];
+                    o->w_marks[o2] = (double)mk[4 * b + 3];
+                    mw += (double)mk[4 * b + 3];
+                }
+                if (o->marks_weight) o->marks_weight[i] = mw;
+            }
+        }
+    }
+
+    if (!fail) {
+        for (int b = 0; b < n_bin; ++b) {
+            acc6 s;
+            memset(&s, 0, sizeof s);
+            for (int t = 0; t < nt; ++t) acc_add(&s, &tbins[(size_t)t * n_bin + b]);
+            o->n_pairs[b] = s.n;
+            o->theta[b] = (double)s.theta; o->gt[b] = (double)s.gt; o->gx[b] = (double)s.gx;
+            o->w[b] = (double)s.w; o->k[b] = (double)s.k;
+            if (o->gt_abs) o->gt_abs[b] = (double)s.gt_abs;
+            if (o->gx_abs) o->gx_abs[b] = (double)s.gx_abs;
+        }
+        if (want_pp) {
+            for (int32_t pa = 0; pa < n_pa; ++pa)
+                for (int32_t pb = 0; pb < n_pb; ++pb)
+                    for (int b = 0; b < n_bin; ++b) {
+                        acc6 s;
+                        memset(&s, 0, sizeof s);
+                        for (int t = 0; t < nt; ++t)
+                            acc_add(&s, &tpp[(size_t)t * pp_len + ((size_t)pa * n_pb + pb) * n_bin + b]);
+                        const size_t o3 = ((size_t)pb * n_pa + pa) * n_bin + b;
+                        o->pp_n[o3] = s.n;
+                        o->pp_theta[o3] = (double)s.theta; o->pp_gt[o3] = (double)s.gt;
+                        o->pp_gx[o3] = (double)s.gx; o->pp_w[o3] = (double)s.w; o->pp_k[o3] = (double)s.k;
+                    }
+        }
+    }
+    free(tbins); free(tpp); free(tmk); free(e2);
+    if (!a->brute) grid_free(&g);
+    return fail ? 2 : 0;
+}
+
+/* Normalisation of src/tpcf.f90:369-376 (shear) / :536-543 (kappa). */
+void orc_finalize(int n_bin, const double *theta, const double *gt, const double *gx,
+                  const double *w, const double *k,
+                  double *mean_theta, double *g_t, double *g_x)
+{
+    for (int b = 0; b < n_bin; ++b) {
+        mean_theta[b] = theta[b] / w[b];              /* NaN for an empty bin, as the reference */
+        if (k[b] > 0.0) { g_t[b] = gt[b] / k[b]; g_x[b] = gx[b] / k[b]; }
+        else { g_t[b] = 0.0; g_x[b] = 0.0; }
+    }
+}
+
+/* Vector form of orc_bin_of_d2 for the tests. */
+void orc_bin_of_d2_array(int64_t n, const double *d2, const double *e2, int n_bin, int32_t *out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_bin_of_d2(d2[i], e2, n_bin);
+}

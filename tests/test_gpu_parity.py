@@ -1,0 +1,252 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU FP64 oracle.
+
+Bar: pair counts per bin and per patch pair bit-exact; positive sums (Theta, W,
+K) within 1e-6 relative in every bin (FP32 path; 1e-12 for FP64); signed sums
+(G_t, G_x, xi_E, xi_B) within 1e-6 of the bin's sum of |terms| (a signed sum of
+zero-mean terms can cancel, so its plain relative error is ill-conditioned).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHEAR, KAPPA = 0, 1
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+def uniform_case(seed, n_a, n_b, side, kind, z=False, zero_w=False):
+    r = _rng(seed)
+    f = np.float32
+    lens = dict(x=r.uniform(0, side, n_a).astype(f), y=r.uniform(0, side, n_a).astype(f),
+                val=r.normal(0, 0.02, n_a).astype(f), w=r.uniform(0.5, 1.5, n_a).astype(f),
+                z=(r.uniform(-0.2, 1.0, n_a) if z else np.zeros(n_a)).astype(f))
+    if zero_w:
+        lens["w"][r.random(n_a) < 0.1] = 0
+    if kind == SHEAR:
+        src = dict(x=r.uniform(0, side, n_b).astype(f), y=r.uniform(0, side, n_b).astype(f),
+                   z=(r.uniform(-0.2, 1.2, n_b) if z else np.ones(n_b)).astype(f),
+                   e1=r.normal(0, 0.27, n_b).astype(f), e2=r.normal(0, 0.27, n_b).astype(f),
+                   w=r.uniform(0.5, 1.5, n_b).astype(f), m=r.normal(0, 0.02, n_b).astype(f))
+    else:
+        src = dict(x=r.uniform(0, side, n_b).astype(f), y=r.uniform(0, side, n_b).astype(f),
+                   kappa_E=r.normal(0, 0.02, n_b).astype(f), kappa_B=r.normal(0, 0.02, n_b).astype(f),
+                   w=r.uniform(0.5, 1.5, n_b).astype(f))
+    if zero_w:
+        src["w"][r.random(n_b) < 0.1] = 0
+    return lens, src
+
+
+def check(tp, orc, kind, lens, src, bins, *, precision="f32", marks=False, patch_A=None, patch_B=None,
+          n_pa=1, n_pb=1, ctx=None, tol=None):
+    if tol is None:
+        tol = 1e-6 if precision == "f32" else 1e-12
+    g = tp.correlate(kind, lens, src, bins, precision=precision, marks=marks, patch_A=patch_A, patch_B=patch_B,
+                     n_patch_A=n_pa, n_patch_B=n_pb, real_bytes=8, ctx=ctx)
+    patches = patch_A is not None or patch_B is not None
+    o = orc.correlate(kind, lens, src, bins.bin_edges, patch_A, patch_B, n_pa, n_pb, marks=marks, patches=patches)
+    print("\npairs per bin:", g.n_pairs.tolist())
+    print("stats:", {k: g.stats[k] for k in ("n_chunks", "n_segments", "cell_size", "ms_pairs")})
+    np.testing.assert_array_equal(g.n_pairs, o["n_pairs"])
+    errs = {}
+    for gk, ok in (("sum_theta", "theta"), ("sum_w", "w"), ("sum_k", "k")):
+        den = np.where(o[ok] != 0, np.abs(o[ok]), 1.0)
+        errs[gk] = float(np.max(np.abs(g.__dict__[gk] - o[ok]) / den))
+    for gk, ok, ak in (("sum_gt", "gt", "gt_abs"), ("sum_gx", "gx", "gx_abs")):
+        den = np.where(o[ak] != 0, o[ak], 1.0)
+        errs[gk] = float(np.max(np.abs(g.__dict__[gk] - o[ok]) / den))
+    print("max errors:", errs)
+    for k, v in errs.items():
+        assert v <= tol, (k, v)
+    # normalised outputs follow from the sums (src/tpcf.f90:369-376)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        mt = o["theta"] / o["w"]
+    np.testing.assert_array_equal(np.isnan(g.mean_theta), np.isnan(mt))
+    if patches:
+        np.testing.assert_array_equal(g.pp_n, o["pp_n"])
+        for gk, ok in (("pp_theta", "pp_theta"), ("pp_w", "pp_w"), ("pp_k", "pp_k")):
+            den = np.where(o[ok] != 0, np.abs(o[ok]), 1.0)
+            assert np.max(np.abs(getattr(g, gk) - o[ok]) / den) <= tol, gk
+        scale = np.maximum(np.abs(o["pp_gt"]), 1e-300)
+        # per patch pair the |terms| scale is not available: use the per-bin scale
+        for gk, ok, ak in (("pp_gt", "pp_gt", "gt_abs"), ("pp_gx", "pp_gx", "gx_abs")):
+            assert np.max(np.abs(getattr(g, gk) - o[ok]) / np.maximum(o[ak], 1e-300)) <= tol, gk
+        del scale
+    if marks:
+        # marks come back as (n_bin, n_A) views; compare against the oracle's (n_A, n_bin)
+        for gk, ok in (("xi_E_marks", "xi_e"), ("xi_B_marks", "xi_b"), ("K_marks", "k_marks"),
+                       ("w_marks", "w_marks")):
+            gm = np.asarray(getattr(g, gk)).T
+            om = o[ok]
+            scale = np.abs(om).sum(axis=1, keepdims=True) + 1e-300
+            err = np.max(np.abs(gm - om) / scale)
+            assert err <= 10 * tol, (gk, err)
+        np.testing.assert_allclose(g.marks_weight, o["marks_weight"], rtol=10 * tol, atol=1e-300)
+    return g, o
+
+
+@pytest.mark.parametrize("kind", [KAPPA, SHEAR])
+def test_uniform_log_bins(tp, orc, ctx, kind):
+    lens, src = uniform_case(11 + kind, 4000, 5000, 100.0, kind)
+    bins = tp.create_bins(0.5, 30.0, 15, tp.LOG)
+    check(tp, orc, kind, lens, src, bins, ctx=ctx)
+
+
+@pytest.mark.parametrize("kind", [KAPPA, SHEAR])
+def test_patches_and_marks(tp, orc, ctx, kind):
+    lens, src = uniform_case(21 + kind, 3000, 3500, 80.0, kind)
+    r = _rng(5)
+    pa = r.integers(0, 4, len(lens["x"])).astype(np.int32)
+    pb = (np.floor(src["x"] / 80.0 * 3).clip(0, 2)).astype(np.int32)
+    bins = tp.create_bins(1.0, 25.0, 12, tp.LOG)
+    check(tp, orc, kind, lens, src, bins, marks=True, patch_A=pa, patch_B=pb, n_pa=4, n_pb=3, ctx=ctx)
+
+
+def test_kappa_auto_same_catalogue(tp, orc, ctx):
+    r = _rng(31)
+    n = 6000
+    x, y = r.uniform(0, 120, n).astype(np.float32), r.uniform(0, 120, n).astype(np.float32)
+    k = r.normal(0, 0.02, n).astype(np.float32)
+    w = r.uniform(0.5, 1.5, n).astype(np.float32)
+    lens = dict(x=x, y=y, val=k, w=w)
+    src = dict(x=x, y=y, kappa_E=k, kappa_B=r.normal(0, 0.02, n).astype(np.float32), w=w)
+    bins = tp.create_bins(1.0, 40.0, 20, tp.LOG)
+    g, o = check(tp, orc, KAPPA, lens, src, bins, ctx=ctx)
+    assert g.n_pairs.sum() % 2 == 0   # ordered pairs are counted twice (src/tpcf.f90:266, :486)
+
+
+@pytest.mark.parametrize("spacing", [0, 2, 3])
+def test_other_spacings(tp, orc, ctx, spacing):
+    lens, src = uniform_case(41 + spacing, 3000, 3000, 60.0, KAPPA)
+    bins = tp.create_bins(0.0 if spacing != 2 else 0.5, 20.0, 10, spacing, 0.7)
+    check(tp, orc, KAPPA, lens, src, bins, ctx=ctx)
+
+
+def test_shear_clustered(tp, orc, ctx):
+    synth = __import__("importlib").import_module("2pcf_amd.synth")
+    wl = synth.make_workload("t", 30000, 30000, 300.0, 0, 0.5, 30.0, 20, 6, seed=77, clustered=True, npix=256,
+                             fwhm=10.0)
+    bins = tp.create_bins(wl.x_min, wl.x_max, wl.n_bin, tp.LOG)
+    check(tp, orc, SHEAR, wl.lens, wl.src, bins, patch_A=wl.patch_A, patch_B=wl.patch_B, n_pa=6, n_pb=6,
+          ctx=ctx)
+
+
+def test_shear_masks_and_zero_weights(tp, orc, ctx):
+    lens, src = uniform_case(51, 3000, 4000, 70.0, SHEAR, z=True, zero_w=True)
+    bins = tp.create_bins(0.5, 20.0, 12, tp.LOG)
+    check(tp, orc, SHEAR, lens, src, bins, marks=True, ctx=ctx)
+
+
+@pytest.mark.parametrize("kind", [KAPPA, SHEAR])
+def test_fp64_path(tp, orc, ctx, kind):
+    lens, src = uniform_case(61 + kind, 3000, 3000, 60.0, kind)
+    lens = {k: v.astype(np.float64) + (1e-9 if k in ("x", "y") else 0) for k, v in lens.items()}
+    src = {k: v.astype(np.float64) for k, v in src.items()}
+    bins = tp.create_bins(0.5, 20.0, 12, tp.LOG)
+    check(tp, orc, kind, lens, src, bins, precision="f64", marks=True, ctx=ctx)
+
+
+@pytest.mark.parametrize("kind", [KAPPA, SHEAR])
+def test_exact_edge_hits(tp, orc, ctx, kind):
+    # integer lattices: many separations are exactly on integer bin edges and
+    # on x_max, so FP32 d^2 equals the edge and the FP64 re-test decides
+    g1 = np.arange(0, 24, dtype=np.float32)
+    X, Y = np.meshgrid(g1, g1)
+    x, y = X.ravel(), Y.ravel()
+    n = len(x)
+    r = _rng(71)
+    lens = dict(x=x, y=y, val=r.normal(0, 0.02, n).astype(np.float32), w=r.uniform(0.5, 1.5, n).astype(np.float32),
+                z=np.zeros(n, np.float32))
+    if kind == KAPPA:
+        src = dict(x=x, y=y, kappa_E=lens["val"], kappa_B=r.normal(0, 0.02, n).astype(np.float32), w=lens["w"])
+        bins = tp.create_bins(0.0, 10.0, 10, tp.LIN)
+    else:
+        src = dict(x=x, y=y, z=np.ones(n, np.float32), e1=r.normal(0, 0.27, n).astype(np.float32),
+                   e2=r.normal(0, 0.27, n).astype(np.float32), w=r.uniform(0.5, 1.5, n).astype(np.float32),
+                   m=np.zeros(n, np.float32))
+        bins = tp.create_bins(1.0, 10.0, 9, tp.LIN)
+    g, o = check(tp, orc, kind, lens, src, bins, ctx=ctx)
+    # d = 5 (3-4-5 triangles) sits on the edge at 5: the pair belongs to the bin above
+    assert g.n_pairs[-1] > 0
+
+
+def test_power_of_two_log_edges(tp, orc, ctx):
+    g1 = np.arange(0, 40, dtype=np.float32)
+    X, Y = np.meshgrid(g1, g1)
+    x, y = X.ravel(), Y.ravel()
+    n = len(x)
+    lens = dict(x=x, y=y, val=np.ones(n, np.float32), w=np.ones(n, np.float32))
+    src = dict(x=x, y=y, kappa_E=np.ones(n, np.float32), kappa_B=np.zeros(n, np.float32), w=np.ones(n, np.float32))
+    bins = tp.create_bins(1.0, 32.0, 5, tp.LOG)   # edges 1, 2, 4, 8, 16, 32 exactly
+    g, o = check(tp, orc, KAPPA, lens, src, bins, ctx=ctx)
+    np.testing.assert_allclose(g.g_t[g.n_pairs > 0], 1.0, rtol=1e-12)   # constant kappa field: xi = 1
+
+
+def test_empty_inputs(tp, ctx):
+    bins = tp.create_bins(1.0, 10.0, 5, tp.LOG)
+    lens = dict(x=np.zeros(0, np.float32), y=np.zeros(0, np.float32), val=np.zeros(0, np.float32),
+                w=np.zeros(0, np.float32))
+    src = dict(x=np.ones(3, np.float32), y=np.ones(3, np.float32), kappa_E=np.ones(3, np.float32),
+               kappa_B=np.ones(3, np.float32), w=np.ones(3, np.float32))
+    g = tp.correlate(KAPPA, lens, src, bins, ctx=ctx)
+    assert g.n_pairs.sum() == 0 and np.all(np.isnan(g.mean_theta)) and np.all(g.g_t == 0)
+
+
+def test_error_paths(tp, ctx):
+    bins = tp.create_bins(1.0, 10.0, 5, tp.LOG)
+    lens, src = uniform_case(81, 100, 100, 20.0, KAPPA)
+    lens["x"][3] = np.nan
+    with pytest.raises(tp.TpcfError) as e:
+        tp.correlate(KAPPA, lens, src, bins, ctx=ctx)
+    assert e.value.code == 6
+    lens, src = uniform_case(82, 100, 100, 20.0, KAPPA)
+    with pytest.raises(tp.TpcfError) as e:
+        tp.correlate(KAPPA, lens, src, bins, patch_A=np.full(100, 5, np.int32), n_patch_A=4, ctx=ctx)
+    assert e.value.code == 8
+    field = tp.LensingSurveyField(lenses=tp.LensCatalog(**lens),
+                                  convergence=tp.KappaMap(**src))
+    with pytest.raises(tp.TpcfError) as e:
+        tp.kappa_2pcf(field, bins, None, True, 1, True, False)
+    assert e.value.code == 5
+    # the context stays usable after errors
+    g = tp.kappa_2pcf(field, bins, None, False, 1, True, False)
+    assert g.n_pairs.sum() > 0
+
+
+def test_linearity_and_weight_scaling(tp, ctx):
+    lens, src = uniform_case(91, 20000, 20000, 200.0, SHEAR)
+    bins = tp.create_bins(0.5, 40.0, 16, tp.LOG)
+    a = tp.correlate(SHEAR, lens, src, bins, ctx=ctx, real_bytes=8)
+    lens2 = dict(lens, val=lens["val"] * 2)
+    b = tp.correlate(SHEAR, lens2, src, bins, ctx=ctx, real_bytes=8)
+    np.testing.assert_array_equal(a.n_pairs, b.n_pairs)
+    np.testing.assert_allclose(b.sum_gt, 2 * a.sum_gt, rtol=1e-12)
+    np.testing.assert_allclose(b.sum_gx, 2 * a.sum_gx, rtol=1e-12)
+    src3 = dict(src, w=src["w"] * 4)
+    c = tp.correlate(SHEAR, lens, src3, bins, ctx=ctx, real_bytes=8)
+    np.testing.assert_allclose(c.g_t, a.g_t, rtol=1e-9)
+    np.testing.assert_allclose(c.sum_w, 4 * a.sum_w, rtol=1e-12)
+
+
+def test_zero_separation_with_x_min_zero(tp, orc, ctx):
+    # coincident lens/source pairs with x_min = 0 land in bin 0 with an
+    # undefined position angle: the reference's g_t/g_x of bin 0 are NaN
+    r = _rng(101)
+    n = 500
+    x = r.uniform(0, 30, n).astype(np.float32)
+    y = r.uniform(0, 30, n).astype(np.float32)
+    lens = dict(x=x, y=y, val=r.normal(0, 0.02, n).astype(np.float32), w=r.uniform(0.5, 1.5, n).astype(np.float32),
+                z=np.zeros(n, np.float32))
+    src = dict(x=x.copy(), y=y.copy(), z=np.ones(n, np.float32), e1=r.normal(0, 0.27, n).astype(np.float32),
+               e2=r.normal(0, 0.27, n).astype(np.float32), w=r.uniform(0.5, 1.5, n).astype(np.float32),
+               m=np.zeros(n, np.float32))
+    bins = tp.create_bins(0.0, 5.0, 5, tp.LIN)
+    g = tp.correlate(SHEAR, lens, src, bins, ctx=ctx, real_bytes=8)
+    o = orc.correlate(SHEAR, lens, src, bins.bin_edges)
+    np.testing.assert_array_equal(g.n_pairs, o["n_pairs"])
+    assert np.isnan(o["gt"][0]) and np.isnan(g.sum_gt[0]) and np.isnan(g.sum_gx[0])
+    assert np.all(np.isfinite(g.sum_gt[1:]))
+    np.testing.assert_allclose(g.sum_w, o["w"], rtol=1e-6)
+    np.testing.assert_allclose(g.sum_gt[1:], o["gt"][1:], rtol=0, atol=1e-6 * o["gt_abs"][1:].max())
