@@ -28,7 +28,7 @@ FLAG_SPHERICAL = 1 << 1
 FLAG_VERBOSE = 1 << 2
 FLAG_NO_KD_TREE = 1 << 3
 
-OPT_CELL_SIZE, OPT_COUNTERS, OPT_CELL_TARGET = 1, 2, 3
+OPT_CELL_SIZE, OPT_COUNTERS, OPT_CELL_TARGET, OPT_SPLIT = 1, 2, 3, 4
 
 # every entry point the header declares
 EXPORTS = (
@@ -73,6 +73,7 @@ class Stats(C.Structure):
         ("ms_finish", C.c_double), ("n_pairs", C.c_int64), ("n_lens_used", C.c_int64),
         ("n_source_used", C.c_int64), ("n_chunks", C.c_int64), ("n_segments", C.c_int64),
         ("cell_size", C.c_double), ("seg_visits", C.c_int64 * 3), ("pair_visits", C.c_int64 * 3),
+        ("n_items", C.c_int64),
     ]
 
     def as_dict(self) -> dict:

@@ -258,7 +258,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
             fill_f64(s, marks, 4 * p.n_lens * n_bin);
         }
         TPCF_TRY(run_pairs<T>(ctx, a, p, pp, ppn, marks));
-        if (p.n_chunks == 0) {
+        if (p.n_items == 0) {
             TPCF_HIP(ctx, hipEventRecord(ctx->ev[2], s));
             TPCF_HIP(ctx, hipEventRecord(ctx->ev[3], s));
         }
@@ -366,6 +366,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
     st.n_lens_used = p.n_lens;
     st.n_source_used = p.n_src;
     st.n_chunks = p.n_chunks;
+    st.n_items = p.n_items;
     st.n_segments = p.n_segs;
     st.cell_size = p.grid.cell;
     if (ctx->opt_counters && !empty) {
@@ -519,6 +520,10 @@ int tpcf_set_option(tpcf_ctx *ctx, int32_t option, double value)
         return TPCF_OK;
     case TPCF_OPT_COUNTERS:
         ctx->opt_counters = (value != 0.0);
+        return TPCF_OK;
+    case TPCF_OPT_SPLIT:
+        if (!(value > 0.0) || !std::isfinite(value)) return fail(ctx, TPCF_ERR_ARG, "split must be > 0");
+        ctx->opt_split = value;
         return TPCF_OK;
     case TPCF_OPT_CELL_TARGET:
         if (!(value > 0.0) || !std::isfinite(value)) return fail(ctx, TPCF_ERR_ARG, "cell target must be > 0");

@@ -66,6 +66,58 @@ __host__ __device__ inline int32_t cell_of(double v, double v0, double inv, int3
     return (int32_t)f;
 }
 
+// Cell rows a lens box [bx0,bx1]x[by0,by1] can reach within x_max (plus the
+// slack mg), before clipping to a patch.
+__host__ __device__ inline void row_span(const Grid &g, double by0, double by1, double xm, double mg, int32_t &cy0,
+                                         int32_t &cy1)
+{
+    cy0 = cell_of(by0 - xm - mg, g.y0, g.inv, g.ny);
+    cy1 = cell_of(by1 + xm + mg, g.y0, g.inv, g.ny);
+}
+
+// Cell columns of row cy that can hold a source within x_max of the box;
+// false when the whole row is out of reach.  Conservative: every source in
+// reach lies in [cx0, cx1] because cell_of is monotone.
+__host__ __device__ inline bool col_span(const Grid &g, double bx0, double bx1, double by0, double by1, int32_t cy,
+                                         double xm, double mg, int32_t &cx0, int32_t &cx1)
+{
+    const double ry0 = g.y0 + (double)cy * g.cell - mg, ry1 = g.y0 + (double)(cy + 1) * g.cell + mg;
+    const double dyl = fmax(0.0, fmax(ry0 - by1, by0 - ry1));
+    if (dyl > xm) return false;
+    const double hw = sqrt(fmax(xm * xm - dyl * dyl, 0.0)) + mg;
+    cx0 = cell_of(bx0 - hw, g.x0, g.inv, g.nx);
+    cx1 = cell_of(bx1 + hw, g.x0, g.inv, g.nx);
+    return true;
+}
+
+// Hilbert-curve index of a point on a 65536 x 65536 lattice (the classic
+// iterative quadrant-rotation construction): consecutive indices are
+// neighbouring lattice points, so runs of the sorted order are compact.
+__host__ __device__ inline uint32_t hilbert16(uint32_t x, uint32_t y)
+{
+    uint32_t d = 0;
+    for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = 0xffffu - x;
+                y = 0xffffu - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+// One unit of pair-kernel work: rows (cy - row0) % n_split == split of a chunk.
+struct Item {
+    uint32_t chunk;
+    uint16_t split, n_split;
+};
+
 // ---- wavefront reductions (DPP row shifts + row broadcasts, gfx9 family) ----
 template <int CTRL, int ROWMASK = 0xf, bool BC = true>
 __device__ inline int dpp_i(int v)
@@ -128,6 +180,7 @@ struct tpcf_ctx {
     hipEvent_t ev[6] = {};
     double opt_cell = 0.0;
     double opt_cell_target = 48.0;
+    double opt_split = 1.0;       // work-item size in units of the mean chunk work
     bool opt_counters = false;
     tpcf_stats stats{};
     std::string err;
@@ -140,6 +193,7 @@ struct tpcf_ctx {
         B_LKEY, B_LKEY2, B_LIDX, B_LIDX2, B_PCOUNT, B_CHOFF, B_PASTART,
         B_LX, B_LY, B_LZ, B_LA, B_LW, B_CHUNKS,
         B_PP, B_PPN, B_MARKS, B_OUT, B_TMP, B_PROBE, B_E2,
+        B_BLK, B_BLKCH, B_WORK, B_ITEMS,
         B_COUNT
     };
     tpcf::DevBuf buf[B_COUNT];
@@ -196,6 +250,10 @@ struct Prepared {
     const Chunk<T> *chunks;
     const T *lx, *ly, *lz, *la, *lw;
     const uint32_t *lidx;        // sorted lens -> original index
+    const uint32_t *cstart;      // [key_space + 1] first sorted source of each key
+    const Item *items;           // work items of the pair kernel
+    int64_t n_items;
+    double margin;               // slack of the cell-range bounds
 };
 
 template <typename T>

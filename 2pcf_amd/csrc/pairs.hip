@@ -1,16 +1,18 @@
 // pairs.hip -- the pair kernel: the inner loops of tangential_shear_tpcf
 // (src/tpcf.f90:266-359) and scalar_kappa_tpcf (src/tpcf.f90:456-526).
 //
-// One wavefront owns one lens chunk (up to 64 lenses, one per lane) and walks
-// the source segments within x_max of the chunk's bounding box, source patch
-// by source patch, row by row (cell rows of the grid).  Source records are
+// One wavefront owns one work item: a lens chunk (up to 64 lenses, one per
+// lane) and a round-robin share of its cell rows.  It walks the source
+// segments within x_max of the chunk's bounding box, source patch by source
+// patch, row by row.  Source records are
 // wave-uniform, so they are read with scalar loads and every lane pairs its
 // lens with the same source.  Each (chunk, segment) pair is classified from
 // the two bounding boxes:
 //   fast  -- every pair lies in one bin: no per-pair binning; W, K and the
 //            pair count come from the segment sums (and G for kappa);
 //   edge  -- pairs straddle one bin edge: per-pair FP32 test against the
-//            squared edge, FP64 re-test for pairs within the band;
+//            squared edge, FP64 re-test for pairs within the band; only the
+//            upper bin's share is summed per pair;
 //   multi -- several edges (or a z test): windows of two bins, bounded tests.
 // Each segment's pairs are summed per lane in the input precision (at most
 // kSegMax terms), then added to per-lane FP64 sums for a window of two
@@ -30,7 +32,7 @@ void fill_f64(hipStream_t s, double *p, int64_t n);
 template <typename T>
 struct PairArgs {
     Grid g;
-    int64_t n_chunks;
+    int64_t n_items;
     int64_t per_xcd;       // blocks per XCD of the remapped grid
     int32_t n_pa, n_pb, n_bin, counters;
     double x_max;          // outer edge (FP64)
@@ -87,17 +89,22 @@ __device__ inline bool below_edge(const PairArgs<T> &A, int e, T d2, T xa, T ya,
 }
 
 // Per-wave window state: two adjacent bins (wk, wk+1) of one source patch.
+// The per-lane FP64 window sums live in LDS, lane-private: acc[q][lane] for
+// q = Theta, G1, G2, W, K, N of bin wk (q) and bin wk+1 (kQ + q), plus the
+// count of in-bin zero-separation pairs (shear with x_min == 0).
+constexpr int kQ = 6;
+constexpr int kWinQ = 2 * kQ + 1;
+enum { QT = 0, QG1 = 1, QG2 = 2, QW = 3, QK = 4, QN = 5 };
+
 struct Win {
     int pb, wk;
     bool open, used0, used1, edge;
-    double aT[2], aG1[2], aG2[2], aW[2], aK[2], aN[2];   // per-lane FP64 sums
-    double aZ;   // shear, x_min == 0: in-bin pairs at zero separation (bin 0 = window slot 0)
+    double *acc;   // this wave's [kWinQ][kWave] block in LDS
+    double *tab;   // this wave's [TPCF_MAX_BIN + 1][kQ] per-bin sums (lane b owns bin b)
 };
 
-// Per-lane FP64 per-bin sums of the current source patch (lane b holds bin b).
-struct Table {
-    double T, G1, G2, W, K, N;
-};
+__device__ __forceinline__ double &wacc(const Win &w, int q, int lane) { return w.acc[q * kWave + lane]; }
+__device__ __forceinline__ double &wtab(const Win &w, int bin, int q) { return w.tab[bin * kQ + q]; }
 
 template <typename T>
 struct LaneData {
@@ -107,13 +114,10 @@ struct LaneData {
     int lane;
 };
 
-__device__ __forceinline__ void win_clear(Win &w)
+__device__ __forceinline__ void win_clear(Win &w, int lane)
 {
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        w.aT[s] = 0; w.aG1[s] = 0; w.aG2[s] = 0; w.aW[s] = 0; w.aK[s] = 0; w.aN[s] = 0;
-    }
-    w.aZ = 0;
+    for (int q = 0; q < kWinQ; ++q) wacc(w, q, lane) = 0.0;
     w.open = false;
     w.used0 = w.used1 = false;
     w.edge = false;
@@ -122,10 +126,10 @@ __device__ __forceinline__ void win_clear(Win &w)
 // Reduce the window's per-lane sums across the wave into the per-bin table
 // (and, with marks, add each lane's sums to its lens' marks).
 template <typename T, int KIND, bool MARKS>
-__device__ __forceinline__ void win_flush(Win &w, Table &t, const LaneData<T> &L, int n_bin,
-                                          double *__restrict__ marks)
+__device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bin, double *__restrict__ marks)
 {
     if (!w.open) return;
+    const int lane = L.lane;
     const double dwa = (double)L.wa, daa = (double)L.aa;
     const double gs = (KIND == SHEAR) ? -1.0 : 1.0;   // G_t -= ... for shear, src/tpcf.f90:348-349
 #pragma unroll
@@ -133,48 +137,125 @@ __device__ __forceinline__ void win_flush(Win &w, Table &t, const LaneData<T> &L
         const int bin = w.wk + s;
         const bool used = s ? w.used1 : w.used0;
         if (!used || bin < 0 || bin >= n_bin) continue;
-        const double sT = wave_sum(dwa * w.aT[s]);
+        const int o = s * kQ;
+        const double aT = wacc(w, o + QT, lane), aG1 = wacc(w, o + QG1, lane), aG2 = wacc(w, o + QG2, lane);
+        const double sT = wave_sum(dwa * aT);
         double sG1 = 0, sG2 = 0, sW = 0, sK = 0, sN = 0;
         if (KIND == SHEAR || MARKS || w.edge) {
-            sG1 = gs * wave_sum(daa * w.aG1[s]);
-            sG2 = gs * wave_sum(daa * w.aG2[s]);
+            sG1 = gs * wave_sum(daa * aG1);
+            sG2 = gs * wave_sum(daa * aG2);
         }
+        double aW = 0, aK = 0;
         if (MARKS || w.edge) {
-            sW = wave_sum(dwa * w.aW[s]);
-            sK = wave_sum(dwa * w.aK[s]);
+            aW = wacc(w, o + QW, lane);
+            aK = wacc(w, o + QK, lane);
+            sW = wave_sum(dwa * aW);
+            sK = wave_sum(dwa * aK);
         }
-        if (w.edge) sN = wave_sum((double)L.vmask * w.aN[s]);
-        if (KIND == SHEAR && bin == 0 && w.edge && wave_sum((double)L.vmask * w.aZ) > 0.0) {
+        if (w.edge) sN = wave_sum((double)L.vmask * wacc(w, o + QN, lane));
+        if (KIND == SHEAR && bin == 0 && w.edge && wave_sum((double)L.vmask * wacc(w, 2 * kQ, lane)) > 0.0) {
             // the position angle of a zero-separation pair is 0/0: the reference's
             // G_t and G_x of that bin become NaN (src/tpcf.f90:343-349)
             sG1 = __builtin_nan("");
             sG2 = __builtin_nan("");
         }
-        if (L.lane == bin) {
-            t.T += sT; t.G1 += sG1; t.G2 += sG2; t.W += sW; t.K += sK; t.N += sN;
+        if (lane == bin) {
+            wtab(w, bin, QT) += sT;
+            wtab(w, bin, QG1) += sG1;
+            wtab(w, bin, QG2) += sG2;
+            wtab(w, bin, QW) += sW;
+            wtab(w, bin, QK) += sK;
+            wtab(w, bin, QN) += sN;
         }
         if (MARKS && L.valid) {
+            // a lens' chunk may be split over several waves: add atomically
             double *mk = marks + ((size_t)L.li * n_bin + bin) * 4;
-            mk[0] += gs * daa * w.aG1[s];
-            mk[1] += gs * daa * w.aG2[s];
-            mk[2] += dwa * w.aK[s];
-            mk[3] += dwa * w.aW[s];
+            unsafeAtomicAdd(&mk[0], gs * daa * aG1);
+            unsafeAtomicAdd(&mk[1], gs * daa * aG2);
+            unsafeAtomicAdd(&mk[2], dwa * aK);
+            unsafeAtomicAdd(&mk[3], dwa * aW);
         }
     }
-    win_clear(w);
+    win_clear(w, lane);
 }
 
 template <typename T, int KIND, bool MARKS>
-__device__ __forceinline__ void win_open(Win &w, Table &t, const LaneData<T> &L, int n_bin, double *marks, int pb,
-                                         int k)
+__device__ __forceinline__ void win_open(Win &w, const LaneData<T> &L, int n_bin, double *marks, int pb, int k)
 {
-    if (w.open && (pb != w.pb || k != w.wk)) win_flush<T, KIND, MARKS>(w, t, L, n_bin, marks);
+    if (w.open && (pb != w.pb || k != w.wk)) win_flush<T, KIND, MARKS>(w, L, n_bin, marks);
     w.pb = pb;
     w.wk = k;
     w.open = true;
 }
 
-// Per-pair pass over a segment for the window bins wk (mask m) and wk+1 (m2).
+// Edge pass: the segment straddles one edge (between window bins wk and
+// wk+1).  Per pair only the share of bin wk+1 is accumulated next to the
+// segment totals; bin wk gets total - share (W, K, N and kappa's G from the
+// exact segment sums).
+template <typename T, int KIND>
+__device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
+                                          const SrcRec<T> *__restrict__ r, const Seg<T> &sg, uint32_t count)
+{
+    const int mid_e = w.wk + 1;
+    const bool track0 = (KIND == SHEAR) && A.xmin_zero && (w.wk == 0);
+    T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, hN = 0, sZ = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+        const SrcRec<T> q = r[j];
+        const T dx = L.xa - q.x, dy = L.ya - q.y;
+        const T dy2 = dy * dy;
+        const T d2 = fma(dx, dx, dy2);
+        const T m2 = below_edge<T>(A, mid_e, d2, L.xa, L.ya, q.x, q.y) ? (T)0 : (T)1;
+        T d, inv;
+        Geo<T>::dinv(d2, d, inv);
+        d = (d2 > (T)0) ? d : (T)0;
+        const T tv = d * q.w;
+        sT += tv;
+        hT = fma(tv, m2, hT);
+        if (KIND == SHEAR) {
+            inv = (d2 > (T)0) ? inv : (T)0;
+            if (track0) sZ += (d2 == (T)0) ? (T)1 - m2 : (T)0;
+            const T u = fma(dx, dx, -dy2);
+            const T v = dx * dy;
+            const T g1 = fma(q.s1w, u, q.s2w2 * v) * inv;
+            const T g2 = fma(q.s2w, u, -(q.s1w2 * v)) * inv;
+            sG1 += g1;
+            sG2 += g2;
+            hG1 = fma(g1, m2, hG1);
+            hG2 = fma(g2, m2, hG2);
+        } else {
+            hG1 = fma(q.s1w, m2, hG1);
+            hG2 = fma(q.s2w, m2, hG2);
+        }
+        hW = fma(q.w, m2, hW);
+        hK = fma(q.wk, m2, hK);
+        hN += m2;
+    }
+    const int lane = L.lane;
+    const double dhT = (double)hT, dhG1 = (double)hG1, dhG2 = (double)hG2, dhW = (double)hW, dhK = (double)hK;
+    wacc(w, QT, lane) += (double)sT - dhT;
+    wacc(w, kQ + QT, lane) += dhT;
+    if (KIND == SHEAR) {
+        wacc(w, QG1, lane) += (double)sG1 - dhG1;
+        wacc(w, QG2, lane) += (double)sG2 - dhG2;
+    } else {
+        wacc(w, QG1, lane) += sg.S1 - dhG1;
+        wacc(w, QG2, lane) += sg.S2 - dhG2;
+    }
+    wacc(w, kQ + QG1, lane) += dhG1;
+    wacc(w, kQ + QG2, lane) += dhG2;
+    wacc(w, QW, lane) += sg.W - dhW;
+    wacc(w, kQ + QW, lane) += dhW;
+    wacc(w, QK, lane) += sg.K - dhK;
+    wacc(w, kQ + QK, lane) += dhK;
+    wacc(w, QN, lane) += (double)count - (double)hN;
+    wacc(w, kQ + QN, lane) += (double)hN;
+    if (KIND == SHEAR) wacc(w, 2 * kQ, lane) += (double)sZ;
+    w.used0 = w.used1 = true;
+    w.edge = true;
+}
+
+// General pass over a segment for the window bins wk (mask m) and wk+1 (m2),
+// with bounds on both sides of the window and the z test when needed.
 template <typename T, int KIND, bool BOUNDED, bool ZCHECK>
 __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                          const SrcRec<T> *__restrict__ r, const T *__restrict__ zr, uint32_t count,
@@ -233,13 +314,14 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
         sN0 += m;
         sN1 += m2;
     }
-    w.aT[0] += (double)sT0; w.aT[1] += (double)sT1;
-    w.aG1[0] += (double)sG10; w.aG1[1] += (double)sG11;
-    w.aG2[0] += (double)sG20; w.aG2[1] += (double)sG21;
-    w.aW[0] += (double)sW0; w.aW[1] += (double)sW1;
-    w.aK[0] += (double)sK0; w.aK[1] += (double)sK1;
-    w.aN[0] += (double)sN0; w.aN[1] += (double)sN1;
-    w.aZ += (double)sZ;
+    const int lane = L.lane;
+    wacc(w, QT, lane) += (double)sT0; wacc(w, kQ + QT, lane) += (double)sT1;
+    wacc(w, QG1, lane) += (double)sG10; wacc(w, kQ + QG1, lane) += (double)sG11;
+    wacc(w, QG2, lane) += (double)sG20; wacc(w, kQ + QG2, lane) += (double)sG21;
+    wacc(w, QW, lane) += (double)sW0; wacc(w, kQ + QW, lane) += (double)sW1;
+    wacc(w, QK, lane) += (double)sK0; wacc(w, kQ + QK, lane) += (double)sK1;
+    wacc(w, QN, lane) += (double)sN0; wacc(w, kQ + QN, lane) += (double)sN1;
+    if (KIND == SHEAR) wacc(w, 2 * kQ, lane) += (double)sZ;
     w.used0 = w.used1 = true;
     w.edge = true;
 }
@@ -249,16 +331,19 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
     const PairArgs<T> A, const Chunk<T> *__restrict__ chunks, const T *__restrict__ lx, const T *__restrict__ ly,
     const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
     const uint32_t *__restrict__ seg_first, const int4 *__restrict__ pbox, const SrcRec<T> *__restrict__ srec,
-    const T *__restrict__ sz, const T *__restrict__ e2lane, double *__restrict__ pp,
+    const T *__restrict__ sz, const T *__restrict__ e2lane, const Item *__restrict__ items, double *__restrict__ pp,
     unsigned long long *__restrict__ ppn, double *__restrict__ marks, unsigned long long *__restrict__ probe)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t b = blockIdx.x;
     const int slot = (int)(b % kSlots);
-    const int64_t ci = ((b % kSlots) * A.per_xcd + b / kSlots) * kWavesPerBlock + wv;
-    if (ci >= A.n_chunks) return;
-    const Chunk<T> ch = chunks[ci];
+    // consecutive items (neighbouring chunks) stay on one XCD and share its L2
+    const int64_t it = ((b % kSlots) * A.per_xcd + b / kSlots) * kWavesPerBlock + wv;
+    if (it >= A.n_items) return;
+    const Item item = items[it];
+    const Chunk<T> ch = chunks[item.chunk];
+    const int split = item.split, n_split = item.n_split;
     LaneData<T> L;
     L.lane = lane;
     L.valid = lane < (int)ch.count;
@@ -272,27 +357,34 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
     const int n_bin = A.n_bin;
     const T e2_lane = e2lane[lane];   // squared edge `lane` (+inf past n_bin)
 
+    __shared__ double s_acc[kWavesPerBlock][kWinQ * kWave];
+    __shared__ double s_tab[kWavesPerBlock][(TPCF_MAX_BIN + 1) * kQ];
     Win w;
-    win_clear(w);
+    w.acc = s_acc[wv];
+    w.tab = s_tab[wv];
+    win_clear(w, lane);
     w.pb = -1;
     w.wk = -3;
-    Table t = {0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < kQ; ++q) wtab(w, lane, q) = 0.0;
 
     const double xm = A.x_max, mg = A.margin;
     const Grid g = A.g;
+    int ry0, ry1;
+    row_span(g, (double)ch.by0, (double)ch.by1, xm, mg, ry0, ry1);
     for (int pb = 0; pb < A.n_pb; ++pb) {
         const int4 pbx = pbox[pb];
         if (pbx.x > pbx.y) continue;   // no sources in this patch
-        const int cy0 = max(cell_of((double)ch.by0 - xm - mg, g.y0, g.inv, g.ny), pbx.z);
-        const int cy1 = min(cell_of((double)ch.by1 + xm + mg, g.y0, g.inv, g.ny), pbx.w);
+        const int cy0 = max(ry0, pbx.z);
+        const int cy1 = min(ry1, pbx.w);
         if (cy0 > cy1) continue;
-        for (int cy = cy0; cy <= cy1; ++cy) {
-            const double ry0 = g.y0 + (double)cy * g.cell - mg, ry1 = g.y0 + (double)(cy + 1) * g.cell + mg;
-            const double dyl = fmax(0.0, fmax(ry0 - (double)ch.by1, (double)ch.by0 - ry1));
-            if (dyl > xm) continue;
-            const double hw = sqrt(fmax(xm * xm - dyl * dyl, 0.0)) + mg;
-            const int cx0 = max(cell_of((double)ch.bx0 - hw, g.x0, g.inv, g.nx), pbx.x);
-            const int cx1 = min(cell_of((double)ch.bx1 + hw, g.x0, g.inv, g.nx), pbx.y);
+        // this item's rows: (cy - ry0) % n_split == split
+        const int first = cy0 + ((split - (cy0 - ry0)) % n_split + n_split) % n_split;
+        for (int cy = first; cy <= cy1; cy += n_split) {
+            int cx0, cx1;
+            if (!col_span(g, (double)ch.bx0, (double)ch.bx1, (double)ch.by0, (double)ch.by1, cy, xm, mg, cx0, cx1))
+                continue;
+            cx0 = max(cx0, pbx.x);
+            cx1 = min(cx1, pbx.y);
             if (cx0 > cx1) continue;
             const int64_t key0 = (int64_t)pb * g.ncells + (int64_t)cy * g.nx + cx0;
             const uint32_t s_begin = __builtin_amdgcn_readfirstlane(seg_first[key0]);
@@ -317,7 +409,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                     cls = 0;
                     // ---- fast: the whole segment is in bin kmin ----
                     if (!(w.open && pb == w.pb && (kmin == w.wk || kmin == w.wk + 1)))
-                        win_open<T, KIND, MARKS>(w, t, L, n_bin, marks, pb, kmin);
+                        win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, kmin);
                     T sT = 0, sG1 = 0, sG2 = 0;
                     if (KIND == SHEAR) {
 #pragma unroll 4
@@ -343,45 +435,47 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                             sT = fma(Geo<T>::dist(d2), q.w, sT);
                         }
                     }
-                    const int s = kmin - w.wk;
-                    if (s == 0) {
-                        w.aT[0] += (double)sT; w.aG1[0] += (double)sG1; w.aG2[0] += (double)sG2;
-                        w.used0 = true;
-                        if (MARKS) {
-                            w.aW[0] += sg.W; w.aK[0] += sg.K;
-                            if (KIND == KAPPA) { w.aG1[0] += sg.S1; w.aG2[0] += sg.S2; }
-                        }
-                    } else {
-                        w.aT[1] += (double)sT; w.aG1[1] += (double)sG1; w.aG2[1] += (double)sG2;
-                        w.used1 = true;
-                        if (MARKS) {
-                            w.aW[1] += sg.W; w.aK[1] += sg.K;
-                            if (KIND == KAPPA) { w.aG1[1] += sg.S1; w.aG2[1] += sg.S2; }
+                    const int o = (kmin - w.wk) * kQ;
+                    if (o == 0) w.used0 = true; else w.used1 = true;
+                    wacc(w, o + QT, lane) += (double)sT;
+                    if (KIND == SHEAR) {
+                        wacc(w, o + QG1, lane) += (double)sG1;
+                        wacc(w, o + QG2, lane) += (double)sG2;
+                    }
+                    if (MARKS) {
+                        wacc(w, o + QW, lane) += sg.W;
+                        wacc(w, o + QK, lane) += sg.K;
+                        if (KIND == KAPPA) {
+                            wacc(w, o + QG1, lane) += sg.S1;
+                            wacc(w, o + QG2, lane) += sg.S2;
                         }
                     }
                     // pair count (and, without marks, W, K and kappa's G) from the segment sums
                     if (lane == kmin) {
-                        t.N += (double)cnt * (double)ch.count;
+                        wtab(w, kmin, QN) += (double)cnt * (double)ch.count;
                         if (!MARKS) {
-                            t.W += ch.wsum * sg.W;
-                            t.K += ch.wsum * sg.K;
-                            if (KIND == KAPPA) { t.G1 += ch.asum * sg.S1; t.G2 += ch.asum * sg.S2; }
+                            wtab(w, kmin, QW) += ch.wsum * sg.W;
+                            wtab(w, kmin, QK) += ch.wsum * sg.K;
+                            if (KIND == KAPPA) {
+                                wtab(w, kmin, QG1) += ch.asum * sg.S1;
+                                wtab(w, kmin, QG2) += ch.asum * sg.S2;
+                            }
                         }
                     }
                 } else if (kmax == kmin + 1 && !zc) {
                     cls = 1;
                     // ---- edge: one bin edge inside the distance range ----
-                    win_open<T, KIND, MARKS>(w, t, L, n_bin, marks, pb, kmin);
-                    win_pass<T, KIND, false, false>(w, L, A, r, nullptr, cnt, n_bin);
+                    win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, kmin);
+                    edge_pass<T, KIND>(w, L, A, r, sg, cnt);
                 } else {
                     cls = 2;
                     // ---- multi: two-bin windows over [kmin, kmax], bounded tests ----
                     const T *zr = (KIND == SHEAR) ? sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
                     for (int w0 = kmin; w0 <= kmax; w0 += 2) {
-                        win_open<T, KIND, MARKS>(w, t, L, n_bin, marks, pb, w0);
+                        win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, w0);
                         if (zc) win_pass<T, KIND, true, true>(w, L, A, r, zr, cnt, n_bin);
                         else win_pass<T, KIND, true, false>(w, L, A, r, zr, cnt, n_bin);
-                        win_flush<T, KIND, MARKS>(w, t, L, n_bin, marks);
+                        win_flush<T, KIND, MARKS>(w, L, n_bin, marks);
                     }
                 }
                 if (A.counters && lane == 0) {
@@ -390,18 +484,18 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                 }
             }
         }
-        win_flush<T, KIND, MARKS>(w, t, L, n_bin, marks);
+        win_flush<T, KIND, MARKS>(w, L, n_bin, marks);
         // hand the patch's per-bin sums to the patch-pair accumulators
-        if (lane < n_bin && t.N > 0) {
+        if (lane < n_bin && wtab(w, lane, QN) > 0.0) {
             const size_t o = (((size_t)slot * A.n_pa + ch.pa) * A.n_pb + pb) * n_bin + lane;
-            unsafeAtomicAdd(&pp[o * kNQ + 0], t.T);
-            unsafeAtomicAdd(&pp[o * kNQ + 1], t.G1);
-            unsafeAtomicAdd(&pp[o * kNQ + 2], t.G2);
-            unsafeAtomicAdd(&pp[o * kNQ + 3], t.W);
-            unsafeAtomicAdd(&pp[o * kNQ + 4], t.K);
-            atomicAdd(&ppn[o], (unsigned long long)t.N);
+            unsafeAtomicAdd(&pp[o * kNQ + 0], wtab(w, lane, QT));
+            unsafeAtomicAdd(&pp[o * kNQ + 1], wtab(w, lane, QG1));
+            unsafeAtomicAdd(&pp[o * kNQ + 2], wtab(w, lane, QG2));
+            unsafeAtomicAdd(&pp[o * kNQ + 3], wtab(w, lane, QW));
+            unsafeAtomicAdd(&pp[o * kNQ + 4], wtab(w, lane, QK));
+            atomicAdd(&ppn[o], (unsigned long long)wtab(w, lane, QN));
         }
-        t = Table{0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < kQ; ++q) wtab(w, lane, q) = 0.0;
     }
 }
 
@@ -412,8 +506,8 @@ int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double 
     PairArgs<T> A;
     std::memset(&A, 0, sizeof A);
     A.g = p.grid;
-    A.n_chunks = p.n_chunks;
-    const int64_t nblk = (p.n_chunks + kWavesPerBlock - 1) / kWavesPerBlock;
+    A.n_items = p.n_items;
+    const int64_t nblk = (p.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
     A.per_xcd = (nblk + kSlots - 1) / kSlots;
     A.n_pa = a.n_pa;
     A.n_pb = a.n_pb;
@@ -422,8 +516,7 @@ int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double 
     const int n = A.n_bin;
     A.x_max = a.bins->edges[n];
     A.xmin_zero = (a.bins->edges[0] == 0.0) ? 1 : 0;
-    const double scale = std::fabs(p.grid.x0) + std::fabs(p.grid.y0) + p.grid.cell * (p.grid.nx + p.grid.ny) + A.x_max;
-    A.margin = 1e-9 * scale + 1e-12;
+    A.margin = p.margin;
     if (sizeof(T) == 4) {
         A.cls_lo = (T)(1.0 - 2e-6);
         A.cls_hi = (T)(1.0 + 2e-6);
@@ -457,12 +550,12 @@ int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double 
         TPCF_TRY(ensure(ctx, tpcf_ctx::B_PROBE, sizeof(unsigned long long) * 8, (void **)&probe));
         fill_u64(ctx->stream, probe, 8);
     }
-    if (p.n_chunks == 0) return TPCF_OK;
+    if (p.n_items == 0) return TPCF_OK;
     const dim3 grid((unsigned)(A.per_xcd * kSlots)), block(kWave * kWavesPerBlock);
     TPCF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
 #define TPCF_LAUNCH(K, M)                                                                                         \
     hipLaunchKernelGGL((k_pairs<T, K, M>), grid, block, 0, ctx->stream, A, p.chunks, p.lx, p.ly, p.lz, p.la, p.lw, \
-                       p.segs, p.seg_first, p.pbox, p.srec, p.sz, e2lane, pp, ppn, marks, probe)
+                       p.segs, p.seg_first, p.pbox, p.srec, p.sz, e2lane, p.items, pp, ppn, marks, probe)
     if (a.kind == SHEAR) {
         if (a.marks) TPCF_LAUNCH(SHEAR, true); else TPCF_LAUNCH(SHEAR, false);
     } else {

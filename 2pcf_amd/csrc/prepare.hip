@@ -5,8 +5,10 @@
 //  * sources by key (patch_B, cell) on a uniform grid: every non-empty
 //    (patch, cell) run becomes a "segment" with a tight bounding box and the
 //    sums needed when all its pairs fall into one bin;
-//  * lenses by (patch_A, Morton code) and cut into chunks of up to 64 lenses
-//    (one lens per lane), each with its bounding box.
+//  * lenses by (patch_A, Hilbert index) and cut into compact chunks of up to
+//    64 lenses (one lens per lane), each with its bounding box; chunks that
+//    meet many sources are split into several work items (cell rows dealt
+//    round-robin) so that clustered fields keep the GPU evenly loaded.
 // Sources masked by the reference (z_B < 0, src/tpcf.f90:306-309) and lenses
 // it skips (w_A == 0, src/tpcf.f90:285 / :474; z_A < 0 for shear) are dropped
 // here, which removes the same pairs.
@@ -173,17 +175,7 @@ __global__ __launch_bounds__(kBlk) void k_segs(int64_t key_space, Grid g, const 
     atomicMax(&b->w, cy);
 }
 
-__device__ inline uint32_t spread16(uint32_t v)
-{
-    v &= 0xffffu;
-    v = (v | (v << 8)) & 0x00ff00ffu;
-    v = (v | (v << 4)) & 0x0f0f0f0fu;
-    v = (v | (v << 2)) & 0x33333333u;
-    v = (v | (v << 1)) & 0x55555555u;
-    return v;
-}
-
-// Lens sort key: (pa << 32) | Morton code of the 16-bit quantised position,
+// Lens sort key: (pa << 32) | Hilbert index of the 16-bit quantised position,
 // or (n_pa << 32) for a dropped lens.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_lens_keys(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
@@ -208,7 +200,7 @@ __global__ __launch_bounds__(kBlk) void k_lens_keys(int64_t n, const T *__restri
         double fx = ((double)x[i] - lx0) * qs, fy = ((double)y[i] - ly0) * qs;
         uint32_t qx = (uint32_t)fmin(fmax(fx, 0.0), 65535.0);
         uint32_t qy = (uint32_t)fmin(fmax(fy, 0.0), 65535.0);
-        key = ((uint64_t)pa << 32) | (uint64_t)(spread16(qx) | (spread16(qy) << 1));
+        key = ((uint64_t)pa << 32) | (uint64_t)hilbert16(qx, qy);
         atomicAdd(&pcount[pa], 1u);
     }
     keys[i] = key;
@@ -232,40 +224,87 @@ __global__ __launch_bounds__(kBlk) void k_lens_gather(int64_t n, const uint32_t 
     lw[i] = w[j];
 }
 
+// Greedy cut of one block of sorted lenses (same patch, <= kLensBlock) into
+// chunks of at most 64 lenses whose bounding box side stays <= dmax (a jump of
+// the Hilbert order out of and back into a patch starts a new chunk).
+template <typename T, bool WRITE>
+__global__ __launch_bounds__(kBlk) void k_chunk_blocks(int64_t n_blocks, const uint4 *__restrict__ blocks, T dmax,
+                                                       const T *__restrict__ lx, const T *__restrict__ ly,
+                                                       const T *__restrict__ lz, const T *__restrict__ lw,
+                                                       const T *__restrict__ la, uint32_t *__restrict__ nch,
+                                                       const uint32_t *__restrict__ ch_off,
+                                                       Chunk<T> *__restrict__ chunks)
+{
+    const int64_t bi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (bi >= n_blocks) return;
+    const uint4 blk = blocks[bi];   // start, count, pa
+    const uint32_t end = blk.x + blk.y;
+    uint32_t i = blk.x, c = 0;
+    while (i < end) {
+        const uint32_t s = i;
+        T bx0 = lx[i], bx1 = bx0, by0 = ly[i], by1 = by0;
+        ++i;
+        while (i < end && i - s < (uint32_t)kChunk) {
+            const T nx0 = lx[i] < bx0 ? lx[i] : bx0, nx1 = lx[i] > bx1 ? lx[i] : bx1;
+            const T ny0 = ly[i] < by0 ? ly[i] : by0, ny1 = ly[i] > by1 ? ly[i] : by1;
+            if (nx1 - nx0 > dmax || ny1 - ny0 > dmax) break;
+            bx0 = nx0; bx1 = nx1; by0 = ny0; by1 = ny1;
+            ++i;
+        }
+        if (WRITE) {
+            Chunk<T> ch;
+            ch.start = s;
+            ch.count = i - s;
+            ch.pa = blk.z;
+            ch.pad = 0;
+            ch.bx0 = bx0; ch.bx1 = bx1; ch.by0 = by0; ch.by1 = by1;
+            T zmax = lz[s];
+            double ws = 0, as = 0;
+            for (uint32_t q = s; q < i; ++q) {
+                zmax = fmax(zmax, lz[q]);
+                ws += (double)lw[q];
+                as += (double)la[q];
+            }
+            ch.zmax = zmax;
+            ch.zpad = (T)0;
+            ch.wsum = ws;
+            ch.asum = as;
+            chunks[ch_off[bi] + c] = ch;
+        }
+        ++c;
+    }
+    if (!WRITE) nch[bi] = c;
+}
+
+// Sources a chunk can meet: the sum over its candidate row ranges.
 template <typename T>
-__global__ __launch_bounds__(kBlk) void k_chunks(int64_t n_chunks, int32_t n_pa, const uint32_t *__restrict__ ch_off,
-                                                 const uint32_t *__restrict__ pa_start,
-                                                 const uint32_t *__restrict__ pcount, const T *__restrict__ lx,
-                                                 const T *__restrict__ ly, const T *__restrict__ lz,
-                                                 const T *__restrict__ lw, const T *__restrict__ la,
-                                                 Chunk<T> *__restrict__ chunks)
+__global__ __launch_bounds__(kBlk) void k_chunk_work(int64_t n_chunks, Grid g, int32_t n_pb, double xm, double mg,
+                                                     const Chunk<T> *__restrict__ chunks,
+                                                     const int4 *__restrict__ pbox, const uint32_t *__restrict__ cstart,
+                                                     float *__restrict__ work)
 {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= n_chunks) return;
-    int lo = 0, hi = n_pa;   // ch_off[lo] <= c < ch_off[hi]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if ((int64_t)ch_off[mid] <= c) lo = mid; else hi = mid;
+    const Chunk<T> ch = chunks[c];
+    int32_t ry0, ry1;
+    row_span(g, (double)ch.by0, (double)ch.by1, xm, mg, ry0, ry1);
+    double total = 0.0;
+    for (int32_t pb = 0; pb < n_pb; ++pb) {
+        const int4 b = pbox[pb];
+        if (b.x > b.y) continue;
+        const int32_t cy0 = max(ry0, b.z), cy1 = min(ry1, b.w);
+        for (int32_t cy = cy0; cy <= cy1; ++cy) {
+            int32_t cx0, cx1;
+            if (!col_span(g, (double)ch.bx0, (double)ch.bx1, (double)ch.by0, (double)ch.by1, cy, xm, mg, cx0, cx1))
+                continue;
+            cx0 = max(cx0, b.x);
+            cx1 = min(cx1, b.y);
+            if (cx0 > cx1) continue;
+            const int64_t key0 = (int64_t)pb * g.ncells + (int64_t)cy * g.nx + cx0;
+            total += (double)(cstart[key0 + (cx1 - cx0) + 1] - cstart[key0]);
+        }
     }
-    const uint32_t local = (uint32_t)(c - ch_off[lo]);
-    Chunk<T> ch;
-    ch.start = pa_start[lo] + local * kChunk;
-    ch.count = std::min<uint32_t>(kChunk, pcount[lo] - local * kChunk);
-    ch.pa = (uint32_t)lo;
-    ch.pad = 0;
-    T bx0 = lx[ch.start], bx1 = bx0, by0 = ly[ch.start], by1 = by0, zmax = lz[ch.start];
-    double ws = 0, as = 0;
-    for (uint32_t q = ch.start; q < ch.start + ch.count; ++q) {
-        bx0 = lx[q] < bx0 ? lx[q] : bx0; bx1 = lx[q] > bx1 ? lx[q] : bx1;
-        by0 = ly[q] < by0 ? ly[q] : by0; by1 = ly[q] > by1 ? ly[q] : by1;
-        zmax = fmax(zmax, lz[q]);
-        ws += (double)lw[q];
-        as += (double)la[q];
-    }
-    ch.bx0 = bx0; ch.bx1 = bx1; ch.by0 = by0; ch.by1 = by1;
-    ch.zmax = zmax; ch.zpad = (T)0;
-    ch.wsum = ws; ch.asum = as;
-    chunks[c] = ch;
+    work[c] = (float)total;
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlk - 1) / kBlk); }
@@ -408,14 +447,12 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     // ---- lenses: keys, sort, chunks ----
     const int64_t na = a.n_a;
     uint64_t *lkey, *lkey2;
-    uint32_t *lidx, *lidx2, *pcount, *choff, *pastart;
+    uint32_t *lidx, *lidx2, *pcount;
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_LKEY, sizeof(uint64_t) * na, (void **)&lkey));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_LKEY2, sizeof(uint64_t) * na, (void **)&lkey2));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_LIDX, sizeof(uint32_t) * na, (void **)&lidx));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_LIDX2, sizeof(uint32_t) * na, (void **)&lidx2));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_PCOUNT, sizeof(uint32_t) * (a.n_pa + 1), (void **)&pcount));
-    TPCF_TRY(ensure(ctx, tpcf_ctx::B_CHOFF, sizeof(uint32_t) * (a.n_pa + 1), (void **)&choff));
-    TPCF_TRY(ensure(ctx, tpcf_ctx::B_PASTART, sizeof(uint32_t) * (a.n_pa + 1), (void **)&pastart));
     launch_fill<uint32_t>(s, pcount, 0u, a.n_pa + 1);
     const double lext = std::max(std::max(lb[1] - lb[0], lb[3] - lb[2]), 1e-30);
     hipLaunchKernelGGL(k_lens_keys<T>, dim3(blocks_for(na)), dim3(kBlk), 0, s, na, a.xa, a.ya, shear ? a.za : nullptr,
@@ -427,20 +464,23 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     TPCF_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     TPCF_HIP(ctx, hipStreamSynchronize(s));
     if (herr & 2) return fail(ctx, TPCF_ERR_PATCH, "lens patch label outside [0, n_patch_A)");
-    std::vector<uint32_t> hoff(a.n_pa + 1), hstart(a.n_pa + 1);
-    int64_t nl = 0, nch = 0;
+    // blocks of <= kLensBlock sorted lenses within one patch
+    constexpr uint32_t kLensBlock = 4096;
+    std::vector<uint4> hblk;
+    int64_t nl = 0;
     for (int32_t q = 0; q < a.n_pa; ++q) {
-        hstart[q] = (uint32_t)nl;
-        hoff[q] = (uint32_t)nch;
+        for (uint32_t o = 0; o < hpc[q]; o += kLensBlock)
+            hblk.push_back(make_uint4((uint32_t)nl + o, std::min(kLensBlock, hpc[q] - o), (uint32_t)q, 0u));
         nl += hpc[q];
-        nch += (hpc[q] + kChunk - 1) / kChunk;
     }
-    hstart[a.n_pa] = (uint32_t)nl;
-    hoff[a.n_pa] = (uint32_t)nch;
     p.n_lens = nl;
-    p.n_chunks = nch;
-    TPCF_HIP(ctx, hipMemcpyAsync(choff, hoff.data(), sizeof(uint32_t) * (a.n_pa + 1), hipMemcpyHostToDevice, s));
-    TPCF_HIP(ctx, hipMemcpyAsync(pastart, hstart.data(), sizeof(uint32_t) * (a.n_pa + 1), hipMemcpyHostToDevice, s));
+    const int64_t nblk = (int64_t)hblk.size();
+    uint4 *dblk;
+    uint32_t *dnch;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_BLK, sizeof(uint4) * std::max<int64_t>(nblk, 1), (void **)&dblk));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_BLKCH, sizeof(uint32_t) * 2 * std::max<int64_t>(nblk, 1), (void **)&dnch));
+    if (nblk > 0)
+        TPCF_HIP(ctx, hipMemcpyAsync(dblk, hblk.data(), sizeof(uint4) * nblk, hipMemcpyHostToDevice, s));
 
     T *lx, *ly, *lz, *la, *lw;
     Chunk<T> *chunks;
@@ -450,13 +490,58 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_LZ, sizeof(T) * nl1, (void **)&lz));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_LA, sizeof(T) * nl1, (void **)&la));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_LW, sizeof(T) * nl1, (void **)&lw));
-    TPCF_TRY(ensure(ctx, tpcf_ctx::B_CHUNKS, sizeof(Chunk<T>) * std::max<int64_t>(nch, 1), (void **)&chunks));
+    // largest chunk box side: three times the side holding 64 lenses at the mean density
+    const double lens_area = std::max((lb[1] - lb[0]) * (lb[3] - lb[2]), 1e-30);
+    const T dmax = (T)(3.0 * std::sqrt((double)kChunk * lens_area / (double)std::max<int64_t>(nl, 1)));
+    int64_t nch = 0;
+    std::vector<uint32_t> hnch(nblk), hchoff(nblk);
     if (nl > 0) {
         hipLaunchKernelGGL(k_lens_gather<T>, dim3(blocks_for(nl)), dim3(kBlk), 0, s, nl, lidx2, a.xa, a.ya,
                            shear ? a.za : nullptr, a.vala, a.wa, lx, ly, lz, la, lw);
-        hipLaunchKernelGGL(k_chunks<T>, dim3(blocks_for(nch)), dim3(kBlk), 0, s, nch, a.n_pa, choff, pastart, pcount,
-                           lx, ly, lz, lw, la, chunks);
+        hipLaunchKernelGGL((k_chunk_blocks<T, false>), dim3(blocks_for(nblk)), dim3(kBlk), 0, s, nblk, dblk, dmax, lx,
+                           ly, lz, lw, la, dnch, nullptr, nullptr);
+        TPCF_HIP(ctx, hipMemcpyAsync(hnch.data(), dnch, sizeof(uint32_t) * nblk, hipMemcpyDeviceToHost, s));
+        TPCF_HIP(ctx, hipStreamSynchronize(s));
+        for (int64_t bi = 0; bi < nblk; ++bi) {
+            hchoff[bi] = (uint32_t)nch;
+            nch += hnch[bi];
+        }
+        TPCF_HIP(ctx, hipMemcpyAsync(dnch + nblk, hchoff.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice, s));
     }
+    p.n_chunks = nch;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_CHUNKS, sizeof(Chunk<T>) * std::max<int64_t>(nch, 1), (void **)&chunks));
+    p.margin = 1e-9 * (std::fabs(g.x0) + std::fabs(g.y0) + g.cell * (g.nx + g.ny) + x_max) + 1e-12;
+    Item *items;
+    if (nch > 0) {
+        hipLaunchKernelGGL((k_chunk_blocks<T, true>), dim3(blocks_for(nblk)), dim3(kBlk), 0, s, nblk, dblk, dmax, lx,
+                           ly, lz, lw, la, nullptr, dnch + nblk, chunks);
+        // split chunks that meet many sources into several work items
+        float *dwork;
+        TPCF_TRY(ensure(ctx, tpcf_ctx::B_WORK, sizeof(float) * nch, (void **)&dwork));
+        hipLaunchKernelGGL(k_chunk_work<T>, dim3(blocks_for(nch)), dim3(kBlk), 0, s, nch, g, a.n_pb, x_max, p.margin,
+                           chunks, pbox, cstart, dwork);
+        std::vector<float> hw(nch);
+        TPCF_HIP(ctx, hipMemcpyAsync(hw.data(), dwork, sizeof(float) * nch, hipMemcpyDeviceToHost, s));
+        TPCF_HIP(ctx, hipStreamSynchronize(s));
+        double tot = 0;
+        for (float v : hw) tot += v;
+        const double target = std::max(1.0, ctx->opt_split * tot / (double)nch);
+        std::vector<Item> hit;
+        hit.reserve(nch * 2);
+        for (int64_t c = 0; c < nch; ++c) {
+            const int ns = (int)std::min(1024.0, std::max(1.0, std::ceil(hw[c] / target)));
+            for (int k = 0; k < ns; ++k) hit.push_back(Item{(uint32_t)c, (uint16_t)k, (uint16_t)ns});
+        }
+        p.n_items = (int64_t)hit.size();
+        TPCF_TRY(ensure(ctx, tpcf_ctx::B_ITEMS, sizeof(Item) * hit.size(), (void **)&items));
+        TPCF_HIP(ctx, hipMemcpyAsync(items, hit.data(), sizeof(Item) * hit.size(), hipMemcpyHostToDevice, s));
+        TPCF_HIP(ctx, hipStreamSynchronize(s));
+        p.items = items;
+    } else {
+        p.n_items = 0;
+        p.items = nullptr;
+    }
+    p.cstart = cstart;
     TPCF_HIP(ctx, hipGetLastError());
     p.chunks = chunks;
     p.lx = lx; p.ly = ly; p.lz = lz; p.la = la; p.lw = lw;

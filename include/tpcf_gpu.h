@@ -106,13 +106,15 @@ typedef struct tpcf_stats {
     double cell_size;       /* source cell side used */
     int64_t seg_visits[3];  /* fast / edge / multi segment visits (only with TPCF_OPT_COUNTERS) */
     int64_t pair_visits[3]; /* lens-source pairs examined per class (same) */
+    int64_t n_items;        /* pair-kernel work items (chunks, heavy ones split) */
 } tpcf_stats;
 
 /* Options for tpcf_set_option. */
 enum tpcf_option {
     TPCF_OPT_CELL_SIZE = 1,     /* > 0: source cell side; 0: automatic (default) */
     TPCF_OPT_COUNTERS = 2,      /* != 0: collect seg_visits / pair_visits (slower) */
-    TPCF_OPT_CELL_TARGET = 3    /* automatic cell size: mean sources per cell (default 48) */
+    TPCF_OPT_CELL_TARGET = 3,   /* automatic cell size: mean sources per cell (default 48) */
+    TPCF_OPT_SPLIT = 4          /* work-item size in units of the mean chunk work (default 1) */
 };
 
 typedef struct tpcf_ctx tpcf_ctx;
