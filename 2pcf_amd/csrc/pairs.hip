@@ -88,6 +88,48 @@ __device__ inline bool below_edge(const PairArgs<T> &A, int e, T d2, T xa, T ya,
     }
 }
 
+// The same test with the edge values already in registers.
+template <typename T>
+struct EdgeTest {
+    float ef, band_ef;
+    double ed;
+    __device__ __forceinline__ EdgeTest(const PairArgs<T> &A, int e)
+        : ef(A.e2f[e]), band_ef(A.band * A.e2f[e]), ed(A.e2d[e]) {}
+    __device__ __forceinline__ bool below(T d2, T xa, T ya, T xs, T ys) const
+    {
+        if constexpr (sizeof(T) == 4) {
+            bool p = d2 < ef;
+            if (fabsf(d2 - ef) <= band_ef) {
+                const double dx = (double)xa - (double)xs, dy = (double)ya - (double)ys;
+                p = fma(dx, dx, dy * dy) < ed;
+            }
+            return p;
+        } else {
+            return d2 < ed;
+        }
+    }
+};
+
+// Visits r[0..count) with the next two records already requested while the
+// current two are processed: copying the prefetched records into `c0/c1`
+// makes the wait for them happen before the next scalar loads are issued
+// (scalar loads return out of order, so a later wait would cover both).
+template <typename T, typename F>
+__device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_t count, F &&f)
+{
+    if (count == 0) return;
+    const uint32_t last = count - 1;
+    SrcRec<T> n0 = r[0], n1 = r[min(1u, last)];
+    for (uint32_t j = 0; j < count; j += 2) {
+        const SrcRec<T> c0 = n0, c1 = n1;
+        n0 = r[min(j + 2, last)];
+        n1 = r[min(j + 3, last)];
+        asm volatile("" ::: "memory");   // the next loads stay ahead of this iteration's work
+        f(c0);
+        if (j + 1 < count) f(c1);
+    }
+}
+
 // Per-wave window state: two adjacent bins (wk, wk+1) of one source patch.
 // The per-lane FP64 window sums live in LDS, lane-private: acc[q][lane] for
 // q = Theta, G1, G2, W, K, N of bin wk (q) and bin wk+1 (kQ + q), plus the
@@ -191,45 +233,64 @@ __device__ __forceinline__ void win_open(Win &w, const LaneData<T> &L, int n_bin
 // Edge pass: the segment straddles one edge (between window bins wk and
 // wk+1).  Per pair only the share of bin wk+1 is accumulated next to the
 // segment totals; bin wk gets total - share (W, K, N and kappa's G from the
-// exact segment sums).
-template <typename T, int KIND>
+// exact segment sums).  GUARD: the chunk and segment boxes overlap, so a pair
+// may sit at zero separation (its per-pair values are forced finite); TRACK0:
+// shear with x_min == 0, where such a pair is in bin 0 (reference: NaN).
+template <typename T, int KIND, bool GUARD, bool TRACK0>
 __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                           const SrcRec<T> *__restrict__ r, const Seg<T> &sg, uint32_t count)
 {
-    const int mid_e = w.wk + 1;
-    const bool track0 = (KIND == SHEAR) && A.xmin_zero && (w.wk == 0);
+    const EdgeTest<T> et(A, w.wk + 1);
     T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, hN = 0, sZ = 0;
-    for (uint32_t j = 0; j < count; ++j) {
-        const SrcRec<T> q = r[j];
+    stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
         const T dx = L.xa - q.x, dy = L.ya - q.y;
         const T dy2 = dy * dy;
         const T d2 = fma(dx, dx, dy2);
-        const T m2 = below_edge<T>(A, mid_e, d2, L.xa, L.ya, q.x, q.y) ? (T)0 : (T)1;
+        bool below = d2 < (T)et.ef;
+        if constexpr (sizeof(T) == 4) {
+            // pairs within the band are re-decided in FP64; the branch is
+            // taken by the whole wave only when some lane needs it
+            const bool amb = fabsf(d2 - et.ef) <= et.band_ef;
+            if (__ballot(amb)) {
+                // keeps this rare path a branch (not evaluated for every pair)
+                asm volatile("" ::: "memory");
+                if (amb) {
+                    const double ddx = (double)L.xa - (double)q.x, ddy = (double)L.ya - (double)q.y;
+                    below = fma(ddx, ddx, ddy * ddy) < et.ed;
+                }
+            }
+        } else {
+            below = d2 < et.ed;
+        }
+        const T m2 = below ? (T)0 : (T)1;
         T d, inv;
         Geo<T>::dinv(d2, d, inv);
-        d = (d2 > (T)0) ? d : (T)0;
-        const T tv = d * q.w;
-        sT += tv;
-        hT = fma(tv, m2, hT);
-        if (KIND == SHEAR) {
+        if (GUARD) {
+            d = (d2 > (T)0) ? d : (T)0;
             inv = (d2 > (T)0) ? inv : (T)0;
-            if (track0) sZ += (d2 == (T)0) ? (T)1 - m2 : (T)0;
+        }
+        const T wm = q.w * m2;
+        sT = fma(d, q.w, sT);
+        hT = fma(d, wm, hT);
+        hW += wm;
+        hK = fma(q.wk, m2, hK);
+        hN += m2;
+        if (KIND == SHEAR) {
+            if (TRACK0) sZ += (d2 == (T)0) ? (T)1 - m2 : (T)0;
             const T u = fma(dx, dx, -dy2);
             const T v = dx * dy;
-            const T g1 = fma(q.s1w, u, q.s2w2 * v) * inv;
-            const T g2 = fma(q.s2w, u, -(q.s1w2 * v)) * inv;
-            sG1 += g1;
-            sG2 += g2;
-            hG1 = fma(g1, m2, hG1);
-            hG2 = fma(g2, m2, hG2);
+            const T t1 = fma(q.s1w, u, q.s2w2 * v);
+            const T t2 = fma(q.s2w, u, -(q.s1w2 * v));
+            const T invm = inv * m2;
+            sG1 = fma(t1, inv, sG1);
+            sG2 = fma(t2, inv, sG2);
+            hG1 = fma(t1, invm, hG1);
+            hG2 = fma(t2, invm, hG2);
         } else {
             hG1 = fma(q.s1w, m2, hG1);
             hG2 = fma(q.s2w, m2, hG2);
         }
-        hW = fma(q.w, m2, hW);
-        hK = fma(q.wk, m2, hK);
-        hN += m2;
-    }
+    });
     const int lane = L.lane;
     const double dhT = (double)hT, dhG1 = (double)hG1, dhG2 = (double)hG2, dhW = (double)hW, dhK = (double)hK;
     wacc(w, QT, lane) += (double)sT - dhT;
@@ -249,7 +310,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
     wacc(w, kQ + QK, lane) += dhK;
     wacc(w, QN, lane) += (double)count - (double)hN;
     wacc(w, kQ + QN, lane) += (double)hN;
-    if (KIND == SHEAR) wacc(w, 2 * kQ, lane) += (double)sZ;
+    if (KIND == SHEAR && TRACK0) wacc(w, 2 * kQ, lane) += (double)sZ;
     w.used0 = w.used1 = true;
     w.edge = true;
 }
@@ -367,6 +428,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
     w.wk = -3;
     for (int q = 0; q < kQ; ++q) wtab(w, lane, q) = 0.0;
 
+    uint64_t cseg0 = 0, cseg1 = 0, cseg2 = 0, cpair0 = 0, cpair1 = 0, cpair2 = 0;   // class counters
     const double xm = A.x_max, mg = A.margin;
     const Grid g = A.g;
     int ry0, ry1;
@@ -412,9 +474,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                         win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, kmin);
                     T sT = 0, sG1 = 0, sG2 = 0;
                     if (KIND == SHEAR) {
-#pragma unroll 4
-                        for (uint32_t j = 0; j < cnt; ++j) {
-                            const SrcRec<T> q = r[j];
+                        stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
                             const T dx = L.xa - q.x, dy = L.ya - q.y;
                             const T dy2 = dy * dy;
                             const T d2 = fma(dx, dx, dy2);
@@ -425,15 +485,13 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                             sT = fma(d, q.w, sT);
                             sG1 = fma(fma(q.s1w, u, q.s2w2 * v), inv, sG1);
                             sG2 = fma(fma(q.s2w, u, -(q.s1w2 * v)), inv, sG2);
-                        }
+                        });
                     } else {
-#pragma unroll 4
-                        for (uint32_t j = 0; j < cnt; ++j) {
-                            const SrcRec<T> q = r[j];
+                        stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
                             const T dx = L.xa - q.x, dy = L.ya - q.y;
                             const T d2 = fma(dx, dx, dy * dy);
                             sT = fma(Geo<T>::dist(d2), q.w, sT);
-                        }
+                        });
                     }
                     const int o = (kmin - w.wk) * kQ;
                     if (o == 0) w.used0 = true; else w.used1 = true;
@@ -466,7 +524,13 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                     cls = 1;
                     // ---- edge: one bin edge inside the distance range ----
                     win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, kmin);
-                    edge_pass<T, KIND>(w, L, A, r, sg, cnt);
+                    if (lo2 > (T)0) {
+                        edge_pass<T, KIND, false, false>(w, L, A, r, sg, cnt);
+                    } else if (KIND == SHEAR && A.xmin_zero && kmin == 0) {
+                        edge_pass<T, KIND, true, true>(w, L, A, r, sg, cnt);
+                    } else {
+                        edge_pass<T, KIND, true, false>(w, L, A, r, sg, cnt);
+                    }
                 } else {
                     cls = 2;
                     // ---- multi: two-bin windows over [kmin, kmax], bounded tests ----
@@ -478,9 +542,10 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                         win_flush<T, KIND, MARKS>(w, L, n_bin, marks);
                     }
                 }
-                if (A.counters && lane == 0) {
-                    atomicAdd(&probe[cls], 1ull);
-                    atomicAdd(&probe[3 + cls], (unsigned long long)cnt * ch.count);
+                if (A.counters) {
+                    if (cls == 0) { cseg0 += 1; cpair0 += (uint64_t)cnt * ch.count; }
+                    else if (cls == 1) { cseg1 += 1; cpair1 += (uint64_t)cnt * ch.count; }
+                    else { cseg2 += 1; cpair2 += (uint64_t)cnt * ch.count; }
                 }
             }
         }
@@ -496,6 +561,14 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
             atomicAdd(&ppn[o], (unsigned long long)wtab(w, lane, QN));
         }
         for (int q = 0; q < kQ; ++q) wtab(w, lane, q) = 0.0;
+    }
+    if (A.counters && lane == 0) {
+        atomicAdd(&probe[0], (unsigned long long)cseg0);
+        atomicAdd(&probe[1], (unsigned long long)cseg1);
+        atomicAdd(&probe[2], (unsigned long long)cseg2);
+        atomicAdd(&probe[3], (unsigned long long)cpair0);
+        atomicAdd(&probe[4], (unsigned long long)cpair1);
+        atomicAdd(&probe[5], (unsigned long long)cpair2);
     }
 }
 

@@ -32,10 +32,8 @@ def sample_positions(rng: np.random.Generator, n: int, side: float, density: np.
     if density is None:
         return (rng.uniform(0.0, side, n).astype(np.float32), rng.uniform(0.0, side, n).astype(np.float32))
     npix = density.shape[0]
-    cdf = np.cumsum(density.ravel())
-    cdf /= cdf[-1]
-    idx = np.searchsorted(cdf, rng.random(n), side="right")
-    idx = np.minimum(idx, npix * npix - 1)
+    counts = rng.multinomial(n, density.ravel())
+    idx = rng.permutation(np.repeat(np.arange(npix * npix, dtype=np.int64), counts))
     iy, ix = np.divmod(idx, npix)
     pix = side / npix
     x = ((ix + rng.random(n)) * pix).astype(np.float32)
@@ -57,14 +55,12 @@ def kmeans_centroids(rng: np.random.Generator, x, y, k: int, iters: int = 10, sa
     return cen
 
 
-def assign(x, y, cen: np.ndarray, block: int = 1 << 20) -> np.ndarray:
-    out = np.empty(len(x), dtype=np.int32)
-    cx, cy = cen[:, 0][None, :], cen[:, 1][None, :]
-    for s in range(0, len(x), block):
-        xx = np.asarray(x[s:s + block], dtype=np.float64)[:, None]
-        yy = np.asarray(y[s:s + block], dtype=np.float64)[:, None]
-        out[s:s + block] = np.argmin((xx - cx) ** 2 + (yy - cy) ** 2, axis=1)
-    return out
+def assign(x, y, cen: np.ndarray) -> np.ndarray:
+    """Label of the nearest centroid."""
+    from scipy.spatial import cKDTree
+    _, lab = cKDTree(cen).query(np.stack([np.asarray(x, np.float64), np.asarray(y, np.float64)], axis=1),
+                                workers=-1)
+    return lab.astype(np.int32)
 
 
 def ellipticities(rng: np.random.Generator, n: int, sigma: float = 0.27):

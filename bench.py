@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell", type=float, default=0.0, help="source cell side (0: automatic)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed steps, copy 1 GiB device-to-device (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
@@ -150,6 +152,13 @@ def main() -> int:
         ppl = per_step_pairs / max(stats["n_lens_used"], 1)
         out["cpu_baseline"] = cpu_baseline(wl, ppl, args.cpu_seconds, threads)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    if args.calib_copy:
+        # a plain copy of known size: 1 GiB read + 1 GiB written, for scaling FETCH_SIZE / WRITE_SIZE
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = 2 * a.numel() * 4
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
