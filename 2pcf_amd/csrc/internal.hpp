@@ -19,6 +19,7 @@ constexpr int kChunk = 64;         // lenses per chunk = one lens per lane
 constexpr int kSlots = 8;          // per-XCD copies of the patch-pair accumulators
 constexpr int kNQ = 5;             // accumulated FP64 quantities: Theta, G1, G2, W, K
 constexpr uint32_t kSegMax = 1024; // longest source segment (bounds per-segment FP32 sums)
+constexpr int kRecPad = 4;         // readable records past the last source (prefetch slack)
 
 enum Kind { SHEAR = 0, KAPPA = 1 };
 
@@ -254,6 +255,7 @@ struct Prepared {
     const Item *items;           // work items of the pair kernel
     int64_t n_items;
     double margin;               // slack of the cell-range bounds
+    bool m_zero;                 // shear: every source has m == 0 (then K == W)
 };
 
 template <typename T>

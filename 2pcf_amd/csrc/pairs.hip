@@ -114,16 +114,17 @@ struct EdgeTest {
 // current two are processed: copying the prefetched records into `c0/c1`
 // makes the wait for them happen before the next scalar loads are issued
 // (scalar loads return out of order, so a later wait would cover both).
+// The record array is padded with kRecPad records, so reading up to three
+// records past a segment's end stays inside the allocation.
 template <typename T, typename F>
 __device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_t count, F &&f)
 {
     if (count == 0) return;
-    const uint32_t last = count - 1;
-    SrcRec<T> n0 = r[0], n1 = r[min(1u, last)];
+    SrcRec<T> n0 = r[0], n1 = r[1];
     for (uint32_t j = 0; j < count; j += 2) {
         const SrcRec<T> c0 = n0, c1 = n1;
-        n0 = r[min(j + 2, last)];
-        n1 = r[min(j + 3, last)];
+        n0 = r[j + 2];
+        n1 = r[j + 3];
         asm volatile("" ::: "memory");   // the next loads stay ahead of this iteration's work
         f(c0);
         if (j + 1 < count) f(c1);
@@ -132,10 +133,13 @@ __device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_
 
 // Per-wave window state: two adjacent bins (wk, wk+1) of one source patch.
 // The per-lane FP64 window sums live in LDS, lane-private: acc[q][lane] for
-// q = Theta, G1, G2, W, K, N of bin wk (q) and bin wk+1 (kQ + q), plus the
-// count of in-bin zero-separation pairs (shear with x_min == 0).
+// q = Theta, G1, G2, W, K of bin wk (q) and bin wk+1 (kWQ + q), plus the
+// count of in-bin zero-separation pairs (shear with x_min == 0).  Pair counts
+// are wave totals (ballot + popcount) and go straight to the per-bin table,
+// whose entries are q = Theta, G1, G2, W, K, N.
 constexpr int kQ = 6;
-constexpr int kWinQ = 2 * kQ + 1;
+constexpr int kWQ = 5;
+constexpr int kWinQ = 2 * kWQ + 1;
 enum { QT = 0, QG1 = 1, QG2 = 2, QW = 3, QK = 4, QN = 5 };
 
 struct Win {
@@ -143,7 +147,15 @@ struct Win {
     bool open, used0, used1, edge;
     double *acc;   // this wave's [kWinQ][kWave] block in LDS
     double *tab;   // this wave's [TPCF_MAX_BIN + 1][kQ] per-bin sums (lane b owns bin b)
+    uint64_t valid;   // lanes holding a lens
+    uint32_t n_valid;
 };
+
+// Adds a wave-total pair count to the table entry of bin `bin` (if real).
+__device__ __forceinline__ void tab_count(Win &w, int lane, int bin, int n_bin, double n)
+{
+    if (bin >= 0 && bin < n_bin && lane == bin) w.tab[bin * kQ + QN] += n;
+}
 
 __device__ __forceinline__ double &wacc(const Win &w, int q, int lane) { return w.acc[q * kWave + lane]; }
 __device__ __forceinline__ double &wtab(const Win &w, int bin, int q) { return w.tab[bin * kQ + q]; }
@@ -167,7 +179,7 @@ __device__ __forceinline__ void win_clear(Win &w, int lane)
 
 // Reduce the window's per-lane sums across the wave into the per-bin table
 // (and, with marks, add each lane's sums to its lens' marks).
-template <typename T, int KIND, bool MARKS>
+template <typename T, int KIND, bool MARKS, bool MZ>
 __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bin, double *__restrict__ marks)
 {
     if (!w.open) return;
@@ -179,10 +191,10 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
         const int bin = w.wk + s;
         const bool used = s ? w.used1 : w.used0;
         if (!used || bin < 0 || bin >= n_bin) continue;
-        const int o = s * kQ;
+        const int o = s * kWQ;
         const double aT = wacc(w, o + QT, lane), aG1 = wacc(w, o + QG1, lane), aG2 = wacc(w, o + QG2, lane);
         const double sT = wave_sum(dwa * aT);
-        double sG1 = 0, sG2 = 0, sW = 0, sK = 0, sN = 0;
+        double sG1 = 0, sG2 = 0, sW = 0, sK = 0;
         if (KIND == SHEAR || MARKS || w.edge) {
             sG1 = gs * wave_sum(daa * aG1);
             sG2 = gs * wave_sum(daa * aG2);
@@ -190,12 +202,16 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
         double aW = 0, aK = 0;
         if (MARKS || w.edge) {
             aW = wacc(w, o + QW, lane);
-            aK = wacc(w, o + QK, lane);
             sW = wave_sum(dwa * aW);
-            sK = wave_sum(dwa * aK);
+            if (MZ) {
+                aK = aW;   // m == 0: K == W
+                sK = sW;
+            } else {
+                aK = wacc(w, o + QK, lane);
+                sK = wave_sum(dwa * aK);
+            }
         }
-        if (w.edge) sN = wave_sum((double)L.vmask * wacc(w, o + QN, lane));
-        if (KIND == SHEAR && bin == 0 && w.edge && wave_sum((double)L.vmask * wacc(w, 2 * kQ, lane)) > 0.0) {
+        if (KIND == SHEAR && bin == 0 && w.edge && wave_sum((double)L.vmask * wacc(w, 2 * kWQ, lane)) > 0.0) {
             // the position angle of a zero-separation pair is 0/0: the reference's
             // G_t and G_x of that bin become NaN (src/tpcf.f90:343-349)
             sG1 = __builtin_nan("");
@@ -207,7 +223,6 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
             wtab(w, bin, QG2) += sG2;
             wtab(w, bin, QW) += sW;
             wtab(w, bin, QK) += sK;
-            wtab(w, bin, QN) += sN;
         }
         if (MARKS && L.valid) {
             // a lens' chunk may be split over several waves: add atomically
@@ -221,10 +236,10 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
     win_clear(w, lane);
 }
 
-template <typename T, int KIND, bool MARKS>
+template <typename T, int KIND, bool MARKS, bool MZ>
 __device__ __forceinline__ void win_open(Win &w, const LaneData<T> &L, int n_bin, double *marks, int pb, int k)
 {
-    if (w.open && (pb != w.pb || k != w.wk)) win_flush<T, KIND, MARKS>(w, L, n_bin, marks);
+    if (w.open && (pb != w.pb || k != w.wk)) win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, marks);
     w.pb = pb;
     w.wk = k;
     w.open = true;
@@ -232,13 +247,15 @@ __device__ __forceinline__ void win_open(Win &w, const LaneData<T> &L, int n_bin
 
 // Edge pass: the segment straddles one edge (between window bins wk and
 // wk+1).  Per pair only the share of bin wk+1 is accumulated next to the
-// segment totals; bin wk gets total - share (W, K, N and kappa's G from the
-// exact segment sums).  GUARD: the chunk and segment boxes overlap, so a pair
-// may sit at zero separation (its per-pair values are forced finite); TRACK0:
-// shear with x_min == 0, where such a pair is in bin 0 (reference: NaN).
-template <typename T, int KIND, bool GUARD, bool TRACK0>
+// segment totals; bin wk gets total - share (W, K and kappa's G from the exact
+// segment sums, pair counts from wave ballots).  GUARD: the chunk and segment
+// boxes overlap, so a pair may sit at zero separation (its per-pair values are
+// forced finite); TRACK0: shear with x_min == 0, where such a pair is in bin 0
+// (reference: NaN).  MZ: all m == 0, so K == W.
+template <typename T, int KIND, bool MZ, bool GUARD, bool TRACK0>
 __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
-                                          const SrcRec<T> *__restrict__ r, const Seg<T> &sg, uint32_t count)
+                                          const SrcRec<T> *__restrict__ r, const Seg<T> &sg, uint32_t count,
+                                          int n_bin)
 {
     const EdgeTest<T> et(A, w.wk + 1);
     T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, hN = 0, sZ = 0;
@@ -263,6 +280,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
             below = d2 < et.ed;
         }
         const T m2 = below ? (T)0 : (T)1;
+        hN += m2;
         T d, inv;
         Geo<T>::dinv(d2, d, inv);
         if (GUARD) {
@@ -273,8 +291,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         sT = fma(d, q.w, sT);
         hT = fma(d, wm, hT);
         hW += wm;
-        hK = fma(q.wk, m2, hK);
-        hN += m2;
+        if (!MZ) hK = fma(q.wk, m2, hK);
         if (KIND == SHEAR) {
             if (TRACK0) sZ += (d2 == (T)0) ? (T)1 - m2 : (T)0;
             const T u = fma(dx, dx, -dy2);
@@ -294,7 +311,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
     const int lane = L.lane;
     const double dhT = (double)hT, dhG1 = (double)hG1, dhG2 = (double)hG2, dhW = (double)hW, dhK = (double)hK;
     wacc(w, QT, lane) += (double)sT - dhT;
-    wacc(w, kQ + QT, lane) += dhT;
+    wacc(w, kWQ + QT, lane) += dhT;
     if (KIND == SHEAR) {
         wacc(w, QG1, lane) += (double)sG1 - dhG1;
         wacc(w, QG2, lane) += (double)sG2 - dhG2;
@@ -302,47 +319,52 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         wacc(w, QG1, lane) += sg.S1 - dhG1;
         wacc(w, QG2, lane) += sg.S2 - dhG2;
     }
-    wacc(w, kQ + QG1, lane) += dhG1;
-    wacc(w, kQ + QG2, lane) += dhG2;
+    wacc(w, kWQ + QG1, lane) += dhG1;
+    wacc(w, kWQ + QG2, lane) += dhG2;
     wacc(w, QW, lane) += sg.W - dhW;
-    wacc(w, kQ + QW, lane) += dhW;
-    wacc(w, QK, lane) += sg.K - dhK;
-    wacc(w, kQ + QK, lane) += dhK;
-    wacc(w, QN, lane) += (double)count - (double)hN;
-    wacc(w, kQ + QN, lane) += (double)hN;
-    if (KIND == SHEAR && TRACK0) wacc(w, 2 * kQ, lane) += (double)sZ;
+    wacc(w, kWQ + QW, lane) += dhW;
+    if (!MZ) {
+        wacc(w, QK, lane) += sg.K - dhK;
+        wacc(w, kWQ + QK, lane) += dhK;
+    }
+    if (KIND == SHEAR && TRACK0) wacc(w, 2 * kWQ, lane) += (double)sZ;
+    // per-lane counts are integers <= kSegMax: their wave sum is exact in FP32
+    const double nhi = (double)wave_sum((float)(L.vmask * hN));
+    tab_count(w, lane, w.wk, n_bin, (double)count * (double)w.n_valid - nhi);
+    tab_count(w, lane, w.wk + 1, n_bin, nhi);
     w.used0 = w.used1 = true;
     w.edge = true;
 }
 
 // General pass over a segment for the window bins wk (mask m) and wk+1 (m2),
-// with bounds on both sides of the window and the z test when needed.
-template <typename T, int KIND, bool BOUNDED, bool ZCHECK>
+// bounded on both sides of the window, with the z test when needed.
+template <typename T, int KIND, bool MZ, bool ZCHECK>
 __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                          const SrcRec<T> *__restrict__ r, const T *__restrict__ zr, uint32_t count,
                                          int n_bin)
 {
-    const int lo_e = w.wk, mid_e = w.wk + 1, hi_e = w.wk + 2;
+    const int lo_e = w.wk, hi_e = w.wk + 2;
+    const bool has_lo = lo_e >= 0, has_hi = hi_e <= n_bin;
+    const EdgeTest<T> emid(A, w.wk + 1), elo(A, has_lo ? lo_e : 0), ehi(A, has_hi ? hi_e : 0);
     const bool track0 = (KIND == SHEAR) && A.xmin_zero && (w.wk == 0);
     T sT0 = 0, sT1 = 0, sG10 = 0, sG11 = 0, sG20 = 0, sG21 = 0;
-    T sW0 = 0, sW1 = 0, sK0 = 0, sK1 = 0, sN0 = 0, sN1 = 0, sZ = 0;
+    T sW0 = 0, sW1 = 0, sK0 = 0, sK1 = 0, sZ = 0;
+    uint32_t n0 = 0, n1 = 0;
     for (uint32_t j = 0; j < count; ++j) {
         const SrcRec<T> q = r[j];
         const T dx = L.xa - q.x, dy = L.ya - q.y;
         const T dy2 = dy * dy;
         const T d2 = fma(dx, dx, dy2);
-        const bool plo = below_edge<T>(A, mid_e, d2, L.xa, L.ya, q.x, q.y);
-        T m = plo ? (T)1 : (T)0;
-        T m2 = (T)1 - m;
-        if (BOUNDED) {
-            const bool ge_lo = (lo_e < 0) ? true : !below_edge<T>(A, lo_e, d2, L.xa, L.ya, q.x, q.y);
-            const bool lt_hi = (hi_e > n_bin) ? true : below_edge<T>(A, hi_e, d2, L.xa, L.ya, q.x, q.y);
-            m = (plo && ge_lo) ? (T)1 : (T)0;
-            m2 = (!plo && lt_hi) ? (T)1 : (T)0;
-        }
+        const bool plo = emid.below(d2, L.xa, L.ya, q.x, q.y);
+        const bool ge_lo = has_lo ? !elo.below(d2, L.xa, L.ya, q.x, q.y) : true;
+        const bool lt_hi = has_hi ? ehi.below(d2, L.xa, L.ya, q.x, q.y) : true;
+        bool in0 = plo && ge_lo, in1 = !plo && lt_hi;
         if (KIND == SHEAR && ZCHECK) {
-            if (L.za > zr[j]) { m = 0; m2 = 0; }   // source in front of the lens, src/tpcf.f90:322-324
+            if (L.za > zr[j]) { in0 = false; in1 = false; }   // source in front of the lens, src/tpcf.f90:322-324
         }
+        const T m = in0 ? (T)1 : (T)0, m2 = in1 ? (T)1 : (T)0;
+        n0 += __popcll(__builtin_amdgcn_ballot_w64(m != (T)0) & w.valid);
+        n1 += __popcll(__builtin_amdgcn_ballot_w64(m2 != (T)0) & w.valid);
         T d, inv;
         Geo<T>::dinv(d2, d, inv);
         // keep every per-pair value finite so that 0/1 masks cannot leak a NaN
@@ -370,24 +392,28 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
         }
         sW0 = fma(q.w, m, sW0);
         sW1 = fma(q.w, m2, sW1);
-        sK0 = fma(q.wk, m, sK0);
-        sK1 = fma(q.wk, m2, sK1);
-        sN0 += m;
-        sN1 += m2;
+        if (!MZ) {
+            sK0 = fma(q.wk, m, sK0);
+            sK1 = fma(q.wk, m2, sK1);
+        }
     }
     const int lane = L.lane;
-    wacc(w, QT, lane) += (double)sT0; wacc(w, kQ + QT, lane) += (double)sT1;
-    wacc(w, QG1, lane) += (double)sG10; wacc(w, kQ + QG1, lane) += (double)sG11;
-    wacc(w, QG2, lane) += (double)sG20; wacc(w, kQ + QG2, lane) += (double)sG21;
-    wacc(w, QW, lane) += (double)sW0; wacc(w, kQ + QW, lane) += (double)sW1;
-    wacc(w, QK, lane) += (double)sK0; wacc(w, kQ + QK, lane) += (double)sK1;
-    wacc(w, QN, lane) += (double)sN0; wacc(w, kQ + QN, lane) += (double)sN1;
-    if (KIND == SHEAR) wacc(w, 2 * kQ, lane) += (double)sZ;
+    wacc(w, QT, lane) += (double)sT0; wacc(w, kWQ + QT, lane) += (double)sT1;
+    wacc(w, QG1, lane) += (double)sG10; wacc(w, kWQ + QG1, lane) += (double)sG11;
+    wacc(w, QG2, lane) += (double)sG20; wacc(w, kWQ + QG2, lane) += (double)sG21;
+    wacc(w, QW, lane) += (double)sW0; wacc(w, kWQ + QW, lane) += (double)sW1;
+    if (!MZ) {
+        wacc(w, QK, lane) += (double)sK0;
+        wacc(w, kWQ + QK, lane) += (double)sK1;
+    }
+    if (KIND == SHEAR) wacc(w, 2 * kWQ, lane) += (double)sZ;
+    tab_count(w, lane, w.wk, n_bin, (double)n0);
+    tab_count(w, lane, w.wk + 1, n_bin, (double)n1);
     w.used0 = w.used1 = true;
     w.edge = true;
 }
 
-template <typename T, int KIND, bool MARKS>
+template <typename T, int KIND, bool MARKS, bool MZ>
 __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
     const PairArgs<T> A, const Chunk<T> *__restrict__ chunks, const T *__restrict__ lx, const T *__restrict__ ly,
     const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
@@ -426,6 +452,8 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
     win_clear(w, lane);
     w.pb = -1;
     w.wk = -3;
+    w.valid = __ballot(L.valid);
+    w.n_valid = ch.count;
     for (int q = 0; q < kQ; ++q) wtab(w, lane, q) = 0.0;
 
     uint64_t cseg0 = 0, cseg1 = 0, cseg2 = 0, cpair0 = 0, cpair1 = 0, cpair2 = 0;   // class counters
@@ -471,7 +499,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                     cls = 0;
                     // ---- fast: the whole segment is in bin kmin ----
                     if (!(w.open && pb == w.pb && (kmin == w.wk || kmin == w.wk + 1)))
-                        win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, kmin);
+                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, marks, pb, kmin);
                     T sT = 0, sG1 = 0, sG2 = 0;
                     if (KIND == SHEAR) {
                         stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
@@ -493,7 +521,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                             sT = fma(Geo<T>::dist(d2), q.w, sT);
                         });
                     }
-                    const int o = (kmin - w.wk) * kQ;
+                    const int o = (kmin - w.wk) * kWQ;
                     if (o == 0) w.used0 = true; else w.used1 = true;
                     wacc(w, o + QT, lane) += (double)sT;
                     if (KIND == SHEAR) {
@@ -502,7 +530,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                     }
                     if (MARKS) {
                         wacc(w, o + QW, lane) += sg.W;
-                        wacc(w, o + QK, lane) += sg.K;
+                        if (!MZ) wacc(w, o + QK, lane) += sg.K;
                         if (KIND == KAPPA) {
                             wacc(w, o + QG1, lane) += sg.S1;
                             wacc(w, o + QG2, lane) += sg.S2;
@@ -523,23 +551,23 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                 } else if (kmax == kmin + 1 && !zc) {
                     cls = 1;
                     // ---- edge: one bin edge inside the distance range ----
-                    win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, kmin);
+                    win_open<T, KIND, MARKS, MZ>(w, L, n_bin, marks, pb, kmin);
                     if (lo2 > (T)0) {
-                        edge_pass<T, KIND, false, false>(w, L, A, r, sg, cnt);
+                        edge_pass<T, KIND, MZ, false, false>(w, L, A, r, sg, cnt, n_bin);
                     } else if (KIND == SHEAR && A.xmin_zero && kmin == 0) {
-                        edge_pass<T, KIND, true, true>(w, L, A, r, sg, cnt);
+                        edge_pass<T, KIND, MZ, true, true>(w, L, A, r, sg, cnt, n_bin);
                     } else {
-                        edge_pass<T, KIND, true, false>(w, L, A, r, sg, cnt);
+                        edge_pass<T, KIND, MZ, true, false>(w, L, A, r, sg, cnt, n_bin);
                     }
                 } else {
                     cls = 2;
                     // ---- multi: two-bin windows over [kmin, kmax], bounded tests ----
                     const T *zr = (KIND == SHEAR) ? sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
                     for (int w0 = kmin; w0 <= kmax; w0 += 2) {
-                        win_open<T, KIND, MARKS>(w, L, n_bin, marks, pb, w0);
-                        if (zc) win_pass<T, KIND, true, true>(w, L, A, r, zr, cnt, n_bin);
-                        else win_pass<T, KIND, true, false>(w, L, A, r, zr, cnt, n_bin);
-                        win_flush<T, KIND, MARKS>(w, L, n_bin, marks);
+                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, marks, pb, w0);
+                        if (zc) win_pass<T, KIND, MZ, true>(w, L, A, r, zr, cnt, n_bin);
+                        else win_pass<T, KIND, MZ, false>(w, L, A, r, zr, cnt, n_bin);
+                        win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, marks);
                     }
                 }
                 if (A.counters) {
@@ -549,7 +577,7 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                 }
             }
         }
-        win_flush<T, KIND, MARKS>(w, L, n_bin, marks);
+        win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, marks);
         // hand the patch's per-bin sums to the patch-pair accumulators
         if (lane < n_bin && wtab(w, lane, QN) > 0.0) {
             const size_t o = (((size_t)slot * A.n_pa + ch.pa) * A.n_pb + pb) * n_bin + lane;
@@ -626,13 +654,17 @@ int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double 
     if (p.n_items == 0) return TPCF_OK;
     const dim3 grid((unsigned)(A.per_xcd * kSlots)), block(kWave * kWavesPerBlock);
     TPCF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-#define TPCF_LAUNCH(K, M)                                                                                         \
-    hipLaunchKernelGGL((k_pairs<T, K, M>), grid, block, 0, ctx->stream, A, p.chunks, p.lx, p.ly, p.lz, p.la, p.lw, \
-                       p.segs, p.seg_first, p.pbox, p.srec, p.sz, e2lane, p.items, pp, ppn, marks, probe)
+#define TPCF_LAUNCH(K, M, Z)                                                                                      \
+    hipLaunchKernelGGL((k_pairs<T, K, M, Z>), grid, block, 0, ctx->stream, A, p.chunks, p.lx, p.ly, p.lz, p.la,   \
+                       p.lw, p.segs, p.seg_first, p.pbox, p.srec, p.sz, e2lane, p.items, pp, ppn, marks, probe)
     if (a.kind == SHEAR) {
-        if (a.marks) TPCF_LAUNCH(SHEAR, true); else TPCF_LAUNCH(SHEAR, false);
+        if (p.m_zero) {
+            if (a.marks) TPCF_LAUNCH(SHEAR, true, true); else TPCF_LAUNCH(SHEAR, false, true);
+        } else {
+            if (a.marks) TPCF_LAUNCH(SHEAR, true, false); else TPCF_LAUNCH(SHEAR, false, false);
+        }
     } else {
-        if (a.marks) TPCF_LAUNCH(KAPPA, true); else TPCF_LAUNCH(KAPPA, false);
+        if (a.marks) TPCF_LAUNCH(KAPPA, true, true); else TPCF_LAUNCH(KAPPA, false, true);
     }
 #undef TPCF_LAUNCH
     TPCF_HIP(ctx, hipGetLastError());

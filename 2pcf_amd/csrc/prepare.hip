@@ -116,11 +116,12 @@ __global__ __launch_bounds__(kBlk) void k_src_gather(int64_t n, int kind, const 
                                                      const T *__restrict__ z, const T *__restrict__ s1,
                                                      const T *__restrict__ s2, const T *__restrict__ w,
                                                      const T *__restrict__ m, SrcRec<T> *__restrict__ rec,
-                                                     T *__restrict__ zs)
+                                                     T *__restrict__ zs, int32_t *__restrict__ flags)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = order[i];
+    if (kind == SHEAR && m && m[j] != (T)0) atomicOr(flags, 4);
     SrcRec<T> r;
     r.x = x[j];
     r.y = y[j];
@@ -428,13 +429,14 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     T *sz = nullptr;
     Seg<T> *segs;
     int4 *pbox;
-    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SREC, sizeof(SrcRec<T>) * std::max<int64_t>(p.n_src, 1), (void **)&srec));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SREC, sizeof(SrcRec<T>) * (p.n_src + kRecPad), (void **)&srec));
+    launch_fill<T>(s, reinterpret_cast<T *>(srec + p.n_src), (T)0, 8 * kRecPad);
     if (shear) TPCF_TRY(ensure(ctx, tpcf_ctx::B_SZ, sizeof(T) * std::max<int64_t>(p.n_src, 1), (void **)&sz));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_SEGS, sizeof(Seg<T>) * std::max<int64_t>(p.n_segs, 1), (void **)&segs));
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_PBOX, sizeof(int4) * a.n_pb, (void **)&pbox));
     if (p.n_src > 0)
         hipLaunchKernelGGL(k_src_gather<T>, dim3(blocks_for(p.n_src)), dim3(kBlk), 0, s, p.n_src, a.kind, sidx2, a.xb,
-                           a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb, srec, sz);
+                           a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb, srec, sz, err);
     launch_fill<int4>(s, pbox, make_int4(INT_MAX, -1, INT_MAX, -1), a.n_pb);
     hipLaunchKernelGGL(k_segs<T>, dim3(blocks_for(key_space)), dim3(kBlk), 0, s, key_space, g, counts, cstart,
                        seg_first, srec, sz, segs, pbox);
@@ -501,6 +503,7 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
         hipLaunchKernelGGL((k_chunk_blocks<T, false>), dim3(blocks_for(nblk)), dim3(kBlk), 0, s, nblk, dblk, dmax, lx,
                            ly, lz, lw, la, dnch, nullptr, nullptr);
         TPCF_HIP(ctx, hipMemcpyAsync(hnch.data(), dnch, sizeof(uint32_t) * nblk, hipMemcpyDeviceToHost, s));
+        TPCF_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         TPCF_HIP(ctx, hipStreamSynchronize(s));
         for (int64_t bi = 0; bi < nblk; ++bi) {
             hchoff[bi] = (uint32_t)nch;
@@ -509,6 +512,7 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
         TPCF_HIP(ctx, hipMemcpyAsync(dnch + nblk, hchoff.data(), sizeof(uint32_t) * nblk, hipMemcpyHostToDevice, s));
     }
     p.n_chunks = nch;
+    p.m_zero = !(herr & 4);
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_CHUNKS, sizeof(Chunk<T>) * std::max<int64_t>(nch, 1), (void **)&chunks));
     p.margin = 1e-9 * (std::fabs(g.x0) + std::fabs(g.y0) + g.cell * (g.nx + g.ny) + x_max) + 1e-12;
     Item *items;
