@@ -1,0 +1,123 @@
+"""The C-ABI library on a host without a GPU (CPU only): it loads, exports
+every entry point include/tpcf_gpu.h declares, its binning matches the oracle
+and the reference, and every correlation entry point refuses to run without a
+device instead of falling back to the CPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tpcf_gpu.h")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tpcf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding(tp):
+    from importlib import import_module
+    N = import_module("2pcf_amd._native")
+    assert declared_functions() == sorted(N.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol(tp):
+    from importlib import import_module
+    N = import_module("2pcf_amd._native")
+    L = C.CDLL(N.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    # the library is C-ABI: no C++ symbol is part of the interface
+    out = os.popen(f"nm -D --defined-only {N.LIB_PATH}").read()
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(declared_functions()) <= exported
+
+
+def test_struct_layout_matches_header(tp):
+    from importlib import import_module
+    N = import_module("2pcf_amd._native")
+    # tpcf_bins: 6 scalars (24 B) + 64 doubles + 63 floats, 8-byte aligned
+    assert C.sizeof(N.Bins) == 24 + 64 * 8 + 63 * 4 + 4
+    assert N.Bins.edges.offset == 24
+    assert C.sizeof(N.Out) == 8 + 22 * 8
+    assert C.sizeof(N.Stats) == 5 * 8 + 5 * 8 + 8 + 3 * 8 + 3 * 8 + 8
+
+
+@pytest.mark.parametrize("case", range(9))
+def test_product_edges_equal_oracle_edges(tp, orc, case):
+    g = np.load(os.path.join(GOLD, "ref_bins.npz"))
+    sp, lo, hi, nb, p = g[f"case{case}_params"]
+    b = tp.create_bins(float(lo), float(hi), int(nb), int(sp), float(p))
+    np.testing.assert_array_equal(b.bin_edges, orc.edges(int(sp), float(lo), float(hi), int(nb), float(p)))
+    # bin_centers follow create_bins (FP32), src/utils.f90:139-230
+    np.testing.assert_allclose(b.bin_centers, g[f"case{case}_centers"], rtol=2e-6, atol=1e-7)
+    # find_bin_index through the C-ABI equals the oracle's membership
+    x = g[f"case{case}_x"].astype(np.float64)
+    x = x[x >= 0][:500]
+    want = orc.bin_of_d2(x * x, b.bin_edges)
+    got = np.array([tp.find_bin_index(v, b) for v in x])
+    np.testing.assert_array_equal(np.where(want >= 0, want + 1, -1), got)
+
+
+def test_bin_size_follows_reference(tp):
+    # bin_size = (T(x_max) - T(x_min)) / n_bin in the spacing's coordinate, FP32
+    b = tp.create_bins(1.0, 100.0, 20, tp.LOG)
+    assert abs(b.bin_size - np.float32(np.log10(100.0) / 20)) < 1e-6
+    b = tp.create_bins(0.0, 10.0, 10, tp.LIN)
+    assert b.bin_size == np.float32(1.0)
+
+
+@pytest.mark.parametrize("args", [
+    (1.0, 100.0, 0, 1, 1.0),      # no bins
+    (1.0, 100.0, 64, 1, 1.0),     # more than TPCF_MAX_BIN
+    (0.0, 100.0, 10, 1, 1.0),     # log spacing needs x_min > 0
+    (5.0, 1.0, 10, 0, 1.0),       # x_max <= x_min
+    (1.0, 10.0, 10, 7, 1.0),      # unknown spacing
+    (1.0, 10.0, 10, 2, 0.0),      # ihs needs p > 0
+])
+def test_create_bins_rejects(tp, args):
+    with pytest.raises(tp.TpcfError):
+        tp.create_bins(*args)
+
+
+def test_no_cpu_fallback(tp):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(tp.TpcfError) as e:
+        tp.Context(0)
+    assert e.value.code == 7      # TPCF_ERR_NODEVICE
+    b = tp.create_bins(1.0, 10.0, 5, tp.LOG)
+    lens = dict(x=np.zeros(3, np.float32), y=np.zeros(3, np.float32), val=np.ones(3, np.float32),
+                w=np.ones(3, np.float32))
+    src = dict(x=np.ones(4, np.float32), y=np.ones(4, np.float32), kappa_E=np.ones(4, np.float32),
+               kappa_B=np.ones(4, np.float32), w=np.ones(4, np.float32))
+    with pytest.raises(tp.TpcfError):
+        tp.correlate(tp.KAPPA, lens, src, b)
+    # a null context at the C level: an error code, never a crash or a CPU result
+    from importlib import import_module
+    N = import_module("2pcf_amd._native")
+    out = N.Out()
+    rc = N.lib().tpcf_kappa_f32(None, 0, None, None, None, None, 0, None, None, None, None, None,
+                                C.byref(b.c), 0, None, None, 1, 1, C.byref(out))
+    assert rc != 0
+
+
+def test_stack_fields_matches_reference_formula(tp):
+    # src/main.f90:678-695: weight-averaged g_t, g_x, mean_theta; summed W, N
+    from types import SimpleNamespace as R
+    r1 = R(theta=np.array([1.0, 2.0]), g_t=np.array([0.1, 0.2]), g_x=np.array([0.0, 0.1]),
+           weights=np.array([2.0, 1.0]), n_pairs=np.array([4, 2]), mean_theta=np.array([1.1, 2.1]))
+    r2 = R(theta=np.array([1.0, 2.0]), g_t=np.array([0.3, 0.0]), g_x=np.array([0.2, 0.1]),
+           weights=np.array([2.0, 3.0]), n_pairs=np.array([5, 7]), mean_theta=np.array([0.9, 1.9]))
+    s = tp.stack_fields([r1, r2])
+    np.testing.assert_allclose(s["g_t"], [0.2, 0.05])
+    np.testing.assert_allclose(s["g_x"], [0.1, 0.1])
+    np.testing.assert_allclose(s["mean_theta"], [1.0, 1.95])
+    np.testing.assert_array_equal(s["n_pairs"], [9, 9])
+    np.testing.assert_allclose(s["weights"], [4.0, 4.0])
