@@ -211,9 +211,11 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
                 sK = wave_sum(dwa * aK);
             }
         }
-        if (KIND == SHEAR && bin == 0 && w.edge && wave_sum((double)L.vmask * wacc(w, 2 * kWQ, lane)) > 0.0) {
-            // the position angle of a zero-separation pair is 0/0: the reference's
-            // G_t and G_x of that bin become NaN (src/tpcf.f90:343-349)
+        // the position angle of a zero-separation pair is 0/0: the reference's
+        // G_t and G_x of that bin, and the xi_E / xi_B marks of that lens,
+        // become NaN (src/tpcf.f90:343-355)
+        const bool zero_sep = KIND == SHEAR && bin == 0 && w.edge && (double)L.vmask * wacc(w, 2 * kWQ, lane) > 0.0;
+        if (KIND == SHEAR && bin == 0 && w.edge && __ballot(zero_sep)) {
             sG1 = __builtin_nan("");
             sG2 = __builtin_nan("");
         }
@@ -227,8 +229,9 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
         if (MARKS && L.valid) {
             // a lens' chunk may be split over several waves: add atomically
             double *mk = marks + ((size_t)L.li * n_bin + bin) * 4;
-            unsafeAtomicAdd(&mk[0], gs * daa * aG1);
-            unsafeAtomicAdd(&mk[1], gs * daa * aG2);
+            const double nan_or_0 = zero_sep ? __builtin_nan("") : 0.0;
+            unsafeAtomicAdd(&mk[0], gs * daa * aG1 + nan_or_0);
+            unsafeAtomicAdd(&mk[1], gs * daa * aG2 + nan_or_0);
             unsafeAtomicAdd(&mk[2], dwa * aK);
             unsafeAtomicAdd(&mk[3], dwa * aW);
         }

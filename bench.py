@@ -13,8 +13,9 @@ with FP64 re-binning of pairs within 1e-5 of an edge and FP64 sums.
 The roofline line is for the dominant kernel (the pair kernel): achieved
 TFLOP/s = in-bin pairs x FLOPs/pair / pair-kernel time (HIP events on the
 library stream), against the MI355X FP32 vector peak.  The CPU baseline is
-the host-core port of the reference algorithm (kd-tree, FP32, OpenMP) timed on
-a lens subsample against all sources.
+the reference's own correlation code (kd-tree, FP32, OpenMP; built from its
+sources into oracle/_ref/ by oracle/build_ref.sh), or where that is absent a C
+port of the same algorithm, timed on a lens subsample against all sources.
 """
 from __future__ import annotations
 
@@ -40,33 +41,66 @@ def log(*a):
 
 
 def cpu_baseline(wl, pairs_per_lens: float, budget_s: float, threads: int) -> dict:
+    """Host-core rate of the reference algorithm on a random lens subsample
+    against all sources.  Uses the reference's own tangential_shear_tpcf /
+    scalar_kappa_tpcf (kd-tree, FP32, OpenMP; oracle/_ref/libref2pcf.so) when
+    it has been built, else the C port of the same algorithm.  The kd-tree
+    build over the sources is timed by a call with one lens and reported
+    apart, so ``value`` is the pair-search-and-accumulate rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle   # the host-core port of the reference algorithm (checker family)
+    import oracle   # checker family: the CPU baseline only
 
     rng = np.random.default_rng(12345)
     n_a = len(wl.lens["x"])
     spacing = 1
-    # calibrate on a few lenses, then size the sample for ~budget_s of pair work
-    probe = min(n_a, max(8, int(2e7 / max(pairs_per_lens, 1.0))))
-    sel = rng.choice(n_a, size=probe, replace=False)
-    sub = {k: v[sel] for k, v in wl.lens.items()}
-    t0 = time.perf_counter()
-    r = oracle.ref_port(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
-    dt = time.perf_counter() - t0 - r["tree_seconds"]
-    rate = max(r["n_pairs"].sum(), 1) / max(dt, 1e-6)
+    use_ref = oracle.have_reference()
+
+    def run(sub):
+        t0 = time.perf_counter()
+        if use_ref:
+            r = oracle.reference(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
+            r["tree_seconds"] = 0.0
+        else:
+            r = oracle.ref_port(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
+        return r, time.perf_counter() - t0
+
+    def sample(n):
+        sel = rng.choice(n_a, size=n, replace=False)
+        return {k: v[sel] for k, v in wl.lens.items()}
+
+    if use_ref:
+        _, t_fixed = run(sample(1))          # kd-tree build over all sources + call overhead
+    # calibrate on ~2e8 pairs, then size the sample for ~budget_s of pair work
+    probe = min(n_a, max(8, int(2e8 / max(pairs_per_lens, 1.0))))
+    r, dt = run(sample(probe))
+    fixed = t_fixed if use_ref else r["tree_seconds"]
+    rate = max(r["n_pairs"].sum(), 1) / max(dt - fixed, 1e-6)
     n_s = int(min(n_a, max(probe, budget_s * rate / max(pairs_per_lens, 1.0))))
-    sel = rng.choice(n_a, size=n_s, replace=False)
-    sub = {k: v[sel] for k, v in wl.lens.items()}
-    t0 = time.perf_counter()
-    r = oracle.ref_port(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
-    total = time.perf_counter() - t0
-    pair_time = total - r["tree_seconds"]
+    r, total = run(sample(n_s))
+    fixed = t_fixed if use_ref else r["tree_seconds"]
+    pair_time = total - fixed
     pairs = int(r["n_pairs"].sum())
-    return {"value": pairs / pair_time, "unit": "pairs/s", "cores": threads, "kind": "port",
+    return {"value": pairs / pair_time, "unit": "pairs/s", "cores": threads,
+            "kind": "reference" if use_ref else "port",
             "sample": f"{n_s} random lenses of {n_a} against all {len(wl.src['x'])} sources; "
                       f"{pairs} in-bin pairs in {pair_time:.2f} s; kd-tree build over the sources "
-                      f"{r['tree_seconds']:.2f} s reported apart",
-            "tree_build_s": r["tree_seconds"]}
+                      f"{fixed:.2f} s reported apart",
+            "tree_build_s": fixed,
+            "code": ("reference tangential_shear_tpcf/scalar_kappa_tpcf (flang -O2 -fopenmp)" if use_ref
+                     else "C port of the reference algorithm (gcc -O2 -fopenmp)")}
+
+
+def profiled_traffic(workload: str):
+    """HBM bytes per pair-kernel launch measured for this workload by
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (copy-calibrated), as recorded in
+    profiles/k_pairs_<workload>.md; None when no such profile exists."""
+    path = os.path.join(ROOT, "profiles", f"k_pairs_{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def main() -> int:
@@ -147,6 +181,11 @@ def main() -> int:
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": FLOPS_PER_PAIR[wl.kind]},
     }
+    if args.scale == 1.0 and args.cell == 0.0:
+        traffic, src = profiled_traffic(wl.name)
+        if traffic is not None:
+            out["roofline"]["traffic"] = traffic
+            out["roofline"]["traffic_source"] = f"{src} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         ppl = per_step_pairs / max(stats["n_lens_used"], 1)

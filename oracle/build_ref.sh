@@ -1,9 +1,23 @@
 #!/bin/sh
-# Builds oracle/_ref/ref_harness from the reference's unmodified sources
-# (src/constants.f90, src/utils.f90, src/kdtree.f90 under /root/reference) and
-# the harness oracle/ref_harness.f90, with the flang of the ROCm toolchain.
-# Outputs go to oracle/_ref/ only.  Exits 0 without building when the
-# reference tree or flang is absent (the GPU host has neither need nor tree).
+# Builds the reference's own code into oracle/_ref/ with the flang of the ROCm
+# toolchain, from the sources where they lie under the reference tree:
+#
+#   ref_harness      src/constants.f90, src/utils.f90, src/kdtree.f90 (unmodified)
+#                    + oracle/ref_harness.f90 (binning and kd-tree vectors)
+#   libref2pcf.so    the same modules + the correlation subroutines
+#                    tangential_shear_tpcf and scalar_kappa_tpcf of src/tpcf.f90
+#                    + oracle/ref_binding.f90 (bind(C) entry points), OpenMP on
+#
+# src/tpcf.f90 as a whole does not parse with flang (line 1160: the I/O list
+# "((g_t(j), g_x(j), block_weight(i), 0), j=1,bins%n_bin)" in the bootstrap's
+# per-block file output is not standard Fortran).  The two correlation
+# subroutines are therefore taken verbatim, by their subroutine/end subroutine
+# lines, into oracle/_ref/ref_tpcf_pairs.f90 inside a module that uses the same
+# modules as the reference's module tpcf (src/tpcf.f90:1-8).  Nothing in them
+# is changed.
+#
+# Outputs go to oracle/_ref/ only (git-ignored).  Exits 0 without building when
+# the reference tree or flang is absent.
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 REF=${TPCF_REFERENCE:-/root/reference}
@@ -15,9 +29,24 @@ if [ ! -f "$REF/src/utils.f90" ] || [ ! -x "$FC" ]; then
 fi
 mkdir -p "$OUT"
 cd "$OUT"
-"$FC" -O2 -c "$REF/src/constants.f90" -o constants.o
-"$FC" -O2 -c "$REF/src/utils.f90" -o utils.o
-"$FC" -O2 -c "$REF/src/kdtree.f90" -o kdtree.o
-"$FC" -O2 -c "$HERE/ref_harness.f90" -o ref_harness.o
-"$FC" -O2 -o ref_harness ref_harness.o kdtree.o utils.o constants.o
-echo "build_ref: $OUT/ref_harness"
+FFLAGS="-O2 -fPIC -fopenmp"
+"$FC" $FFLAGS -c "$REF/src/constants.f90" -o constants.o
+"$FC" $FFLAGS -c "$REF/src/utils.f90" -o utils.o
+"$FC" $FFLAGS -c "$REF/src/kdtree.f90" -o kdtree.o
+"$FC" $FFLAGS -c "$HERE/ref_harness.f90" -o ref_harness.o
+"$FC" $FFLAGS -o ref_harness ref_harness.o kdtree.o utils.o constants.o
+
+{
+    echo "module ref_tpcf_pairs"
+    echo "use, intrinsic :: ISO_FORTRAN_ENV"
+    echo "use kdtree_module"
+    echo "use utils"
+    echo "implicit none"
+    echo "contains"
+    sed -n '/^ *subroutine tangential_shear_tpcf(/,/^ *end subroutine scalar_kappa_tpcf/p' "$REF/src/tpcf.f90"
+    echo "end module ref_tpcf_pairs"
+} > ref_tpcf_pairs.f90
+"$FC" $FFLAGS -c ref_tpcf_pairs.f90 -o ref_tpcf_pairs.o
+"$FC" $FFLAGS -c "$HERE/ref_binding.f90" -o ref_binding.o
+"$FC" $FFLAGS -shared -o libref2pcf.so ref_binding.o ref_tpcf_pairs.o kdtree.o utils.o constants.o
+echo "build_ref: $OUT/ref_harness $OUT/libref2pcf.so"

@@ -14,6 +14,10 @@ Reference behaviour restated (tilmantroester/2pcf):
                       visited, FP64 geometry, long-double sums
   * ``finalize``   -- src/tpcf.f90:369-376 / :536-543
   * ``ref_port``   -- the reference algorithm (kd-tree, FP32, OpenMP) for timing
+  * ``reference``  -- the reference's own tangential_shear_tpcf /
+                      scalar_kappa_tpcf, compiled by ``build_ref.sh`` into
+                      ``_ref/libref2pcf.so`` (present only where the reference
+                      tree and flang were available when it was built)
 """
 from __future__ import annotations
 
@@ -73,6 +77,10 @@ class _RpOut(C.Structure):
     _fields_ = [("mean_theta", _fp), ("g_t", _fp), ("g_x", _fp), ("weights", _fp), ("n_pairs", _i64p),
                 ("xi_e", _fp), ("xi_b", _fp), ("k_marks", _fp), ("w_marks", _fp), ("marks_weight", _fp),
                 ("tree_seconds", C.c_double)]
+
+
+REF_LIB_PATH = os.path.join(_HERE, "_ref", "libref2pcf.so")
+_ref_lib = None
 
 
 def build() -> str:
@@ -263,4 +271,62 @@ def ref_port(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ihs
     if rc:
         raise RuntimeError(f"ref_port failed with code {rc}")
     res["tree_seconds"] = o.tree_seconds
+    return res
+
+
+def have_reference() -> bool:
+    return os.path.exists(REF_LIB_PATH)
+
+
+def ref_lib():
+    """The reference's own correlation code (``_ref/libref2pcf.so``)."""
+    global _ref_lib
+    if _ref_lib is None:
+        if not have_reference():
+            raise FileNotFoundError(f"{REF_LIB_PATH} is missing: run oracle/build_ref.sh where the reference is")
+        L = C.CDLL(REF_LIB_PATH)
+        ci, cf = C.c_int, C.c_float
+        L.ref_shear_c.argtypes = [ci] + [_fp] * 5 + [ci] + [_fp] * 7 + [cf, cf, ci, ci, cf, ci, ci] + \
+            [_fp] * 4 + [_i64p] + [_fp] * 5
+        L.ref_kappa_c.argtypes = [ci] + [_fp] * 4 + [ci] + [_fp] * 5 + [cf, cf, ci, ci, cf, ci, ci] + \
+            [_fp] * 4 + [_i64p] + [_fp] * 5
+        L.ref_bins_c.argtypes = [cf, cf, ci, ci, cf, ci, _fp, _fp, _fp, _i32p]
+        _ref_lib = L
+    return _ref_lib
+
+
+def reference(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ihs_theta=1.0,
+              n_threads: int = 1, use_tree: bool = True) -> dict:
+    """Runs the reference's tangential_shear_tpcf (kind 0) or scalar_kappa_tpcf
+    (kind 1), src/tpcf.f90:190-545, on FP32 copies of the inputs.  Returns its
+    outputs: mean_theta, g_t, g_x, weights, n_pairs and the per-lens marks
+    (shape (n_A, n_bin)), plus the wall time of the call in ``seconds``."""
+    import time
+    L = ref_lib()
+    keep = []
+
+    def f(dct, key, n):
+        v = dct.get(key)
+        a = np.ascontiguousarray(v if v is not None else np.zeros(n), dtype=np.float32)
+        keep.append(a)
+        return _ptr(a, _fp)
+
+    n_a, n_b = len(lens["x"]), len(src["x"])
+    res = {k: np.zeros(n_bin, dtype=np.float32) for k in ("mean_theta", "g_t", "g_x", "weights")}
+    res["n_pairs"] = np.zeros(n_bin, dtype=np.int64)
+    for k in ("xi_e", "xi_b", "k_marks", "w_marks"):
+        res[k] = np.zeros((max(n_a, 1), n_bin), dtype=np.float32)
+    res["marks_weight"] = np.zeros(max(n_a, 1), dtype=np.float32)
+    outs = [_ptr(res[k], _fp) for k in ("mean_theta", "g_t", "g_x", "weights")] + [_ptr(res["n_pairs"], _i64p)] + \
+        [_ptr(res[k], _fp) for k in ("xi_e", "xi_b", "k_marks", "w_marks", "marks_weight")]
+    common = [float(x_min), float(x_max), int(n_bin), int(spacing), float(ihs_theta), int(n_threads),
+              1 if use_tree else 0]
+    t0 = time.perf_counter()
+    if kind == SHEAR:
+        L.ref_shear_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "z", "val", "w")), n_b,
+                      *(f(src, k, n_b) for k in ("x", "y", "z", "e1", "e2", "w", "m")), *common, *outs)
+    else:
+        L.ref_kappa_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "val", "w")), n_b,
+                      *(f(src, k, n_b) for k in ("x", "y", "kappa_E", "kappa_B", "w")), *common, *outs)
+    res["seconds"] = time.perf_counter() - t0
     return res

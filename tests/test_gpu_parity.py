@@ -222,8 +222,11 @@ def test_linearity_and_weight_scaling(tp, ctx):
     lens2 = dict(lens, val=lens["val"] * 2)
     b = tp.correlate(SHEAR, lens2, src, bins, ctx=ctx, real_bytes=8)
     np.testing.assert_array_equal(a.n_pairs, b.n_pairs)
-    np.testing.assert_allclose(b.sum_gt, 2 * a.sum_gt, rtol=1e-12)
-    np.testing.assert_allclose(b.sum_gx, 2 * a.sum_gx, rtol=1e-12)
+    # FP64 sums reached through atomics in no fixed order: equal to FP64
+    # rounding, measured against the largest bin (a bin's sum can cancel)
+    for k in ("sum_gt", "sum_gx"):
+        ga, gb = getattr(a, k), getattr(b, k)
+        np.testing.assert_allclose(gb, 2 * ga, rtol=1e-12, atol=1e-13 * np.abs(ga).max())
     src3 = dict(src, w=src["w"] * 4)
     c = tp.correlate(SHEAR, lens, src3, bins, ctx=ctx, real_bytes=8)
     np.testing.assert_allclose(c.g_t, a.g_t, rtol=1e-9)
