@@ -12,11 +12,11 @@
 # "((g_t(j), g_x(j), block_weight(i), 0), j=1,bins%n_bin)" in the bootstrap's
 # per-block file output is not standard Fortran).  The two correlation
 # subroutines are therefore taken verbatim, by their subroutine/end subroutine
-# lines, into oracle/_ref/ref_tpcf_pairs.f90 inside a module that uses the same
-# modules as the reference's module tpcf (src/tpcf.f90:1-8).  Nothing in them
-# is changed.
+# lines, into a temporary file inside a module that uses the same modules as
+# the reference's module tpcf (src/tpcf.f90:1-8).  Nothing in them is changed.
 #
-# Outputs go to oracle/_ref/ only (git-ignored).  Exits 0 without building when
+# Outputs go to oracle/_ref/ only (git-ignored): the binary and the shared
+# library, no source and no intermediate.  Exits 0 without building when
 # the reference tree or flang is absent.
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -28,13 +28,17 @@ if [ ! -f "$REF/src/utils.f90" ] || [ ! -x "$FC" ]; then
     exit 0
 fi
 mkdir -p "$OUT"
-cd "$OUT"
+# intermediates (objects, module files and the extract of src/tpcf.f90) live in
+# a temporary directory outside the repository; only the two products are kept
+WORK=$(mktemp -d)
+trap 'rm -rf "$WORK"' EXIT
+cd "$WORK"
 FFLAGS="-O2 -fPIC -fopenmp"
 "$FC" $FFLAGS -c "$REF/src/constants.f90" -o constants.o
 "$FC" $FFLAGS -c "$REF/src/utils.f90" -o utils.o
 "$FC" $FFLAGS -c "$REF/src/kdtree.f90" -o kdtree.o
 "$FC" $FFLAGS -c "$HERE/ref_harness.f90" -o ref_harness.o
-"$FC" $FFLAGS -o ref_harness ref_harness.o kdtree.o utils.o constants.o
+"$FC" $FFLAGS -o "$OUT/ref_harness" ref_harness.o kdtree.o utils.o constants.o
 
 {
     echo "module ref_tpcf_pairs"
@@ -48,5 +52,5 @@ FFLAGS="-O2 -fPIC -fopenmp"
 } > ref_tpcf_pairs.f90
 "$FC" $FFLAGS -c ref_tpcf_pairs.f90 -o ref_tpcf_pairs.o
 "$FC" $FFLAGS -c "$HERE/ref_binding.f90" -o ref_binding.o
-"$FC" $FFLAGS -shared -o libref2pcf.so ref_binding.o ref_tpcf_pairs.o kdtree.o utils.o constants.o
+"$FC" $FFLAGS -shared -o "$OUT/libref2pcf.so" ref_binding.o ref_tpcf_pairs.o kdtree.o utils.o constants.o
 echo "build_ref: $OUT/ref_harness $OUT/libref2pcf.so"
