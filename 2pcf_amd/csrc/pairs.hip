@@ -135,8 +135,10 @@ __device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_
 // The per-lane FP64 window sums live in LDS, lane-private: acc[q][lane] for
 // q = Theta, G1, G2, W, K of bin wk (q) and bin wk+1 (kWQ + q), plus the
 // count of in-bin zero-separation pairs (shear with x_min == 0).  Pair counts
-// are wave totals (ballot + popcount) and go straight to the per-bin table,
-// whose entries are q = Theta, G1, G2, W, K, N.
+// are wave totals (ballot + popcount) and go straight to the per-bin sums,
+// q = Theta, G1, G2, W, K, N, kept in registers by the lane of the bin.  Only
+// the window sums use LDS (5.6 KB per wave), which leaves room for 7
+// workgroups of 4 waves per CU.
 constexpr int kQ = 6;
 constexpr int kWQ = 5;
 constexpr int kWinQ = 2 * kWQ + 1;
@@ -146,7 +148,7 @@ struct Win {
     int pb, wk;
     bool open, used0, used1, edge;
     double *acc;   // this wave's [kWinQ][kWave] block in LDS
-    double *tab;   // this wave's [TPCF_MAX_BIN + 1][kQ] per-bin sums (lane b owns bin b)
+    double t[6];   // per-bin sums Theta, G1, G2, W, K, N of bin `lane`, in registers (lane b owns bin b)
     uint64_t valid;   // lanes holding a lens
     uint32_t n_valid;
 };
@@ -154,11 +156,10 @@ struct Win {
 // Adds a wave-total pair count to the table entry of bin `bin` (if real).
 __device__ __forceinline__ void tab_count(Win &w, int lane, int bin, int n_bin, double n)
 {
-    if (bin >= 0 && bin < n_bin && lane == bin) w.tab[bin * kQ + QN] += n;
+    if (bin >= 0 && bin < n_bin && lane == bin) w.t[QN] += n;
 }
 
 __device__ __forceinline__ double &wacc(const Win &w, int q, int lane) { return w.acc[q * kWave + lane]; }
-__device__ __forceinline__ double &wtab(const Win &w, int bin, int q) { return w.tab[bin * kQ + q]; }
 
 template <typename T>
 struct LaneData {
@@ -220,11 +221,11 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
             sG2 = __builtin_nan("");
         }
         if (lane == bin) {
-            wtab(w, bin, QT) += sT;
-            wtab(w, bin, QG1) += sG1;
-            wtab(w, bin, QG2) += sG2;
-            wtab(w, bin, QW) += sW;
-            wtab(w, bin, QK) += sK;
+            w.t[QT] += sT;
+            w.t[QG1] += sG1;
+            w.t[QG2] += sG2;
+            w.t[QW] += sW;
+            w.t[QK] += sK;
         }
         if (MARKS && L.valid) {
             // a lens' chunk may be split over several waves: add atomically
@@ -448,16 +449,14 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
     const T e2_lane = e2lane[lane];   // squared edge `lane` (+inf past n_bin)
 
     __shared__ double s_acc[kWavesPerBlock][kWinQ * kWave];
-    __shared__ double s_tab[kWavesPerBlock][(TPCF_MAX_BIN + 1) * kQ];
     Win w;
     w.acc = s_acc[wv];
-    w.tab = s_tab[wv];
     win_clear(w, lane);
     w.pb = -1;
     w.wk = -3;
     w.valid = __ballot(L.valid);
     w.n_valid = ch.count;
-    for (int q = 0; q < kQ; ++q) wtab(w, lane, q) = 0.0;
+    for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
 
     uint64_t cseg0 = 0, cseg1 = 0, cseg2 = 0, cpair0 = 0, cpair1 = 0, cpair2 = 0;   // class counters
     const double xm = A.x_max, mg = A.margin;
@@ -541,13 +540,13 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
                     }
                     // pair count (and, without marks, W, K and kappa's G) from the segment sums
                     if (lane == kmin) {
-                        wtab(w, kmin, QN) += (double)cnt * (double)ch.count;
+                        w.t[QN] += (double)cnt * (double)ch.count;
                         if (!MARKS) {
-                            wtab(w, kmin, QW) += ch.wsum * sg.W;
-                            wtab(w, kmin, QK) += ch.wsum * sg.K;
+                            w.t[QW] += ch.wsum * sg.W;
+                            w.t[QK] += ch.wsum * sg.K;
                             if (KIND == KAPPA) {
-                                wtab(w, kmin, QG1) += ch.asum * sg.S1;
-                                wtab(w, kmin, QG2) += ch.asum * sg.S2;
+                                w.t[QG1] += ch.asum * sg.S1;
+                                w.t[QG2] += ch.asum * sg.S2;
                             }
                         }
                     }
@@ -582,16 +581,16 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
         }
         win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, marks);
         // hand the patch's per-bin sums to the patch-pair accumulators
-        if (lane < n_bin && wtab(w, lane, QN) > 0.0) {
+        if (lane < n_bin && w.t[QN] > 0.0) {
             const size_t o = (((size_t)slot * A.n_pa + ch.pa) * A.n_pb + pb) * n_bin + lane;
-            unsafeAtomicAdd(&pp[o * kNQ + 0], wtab(w, lane, QT));
-            unsafeAtomicAdd(&pp[o * kNQ + 1], wtab(w, lane, QG1));
-            unsafeAtomicAdd(&pp[o * kNQ + 2], wtab(w, lane, QG2));
-            unsafeAtomicAdd(&pp[o * kNQ + 3], wtab(w, lane, QW));
-            unsafeAtomicAdd(&pp[o * kNQ + 4], wtab(w, lane, QK));
-            atomicAdd(&ppn[o], (unsigned long long)wtab(w, lane, QN));
+            unsafeAtomicAdd(&pp[o * kNQ + 0], w.t[QT]);
+            unsafeAtomicAdd(&pp[o * kNQ + 1], w.t[QG1]);
+            unsafeAtomicAdd(&pp[o * kNQ + 2], w.t[QG2]);
+            unsafeAtomicAdd(&pp[o * kNQ + 3], w.t[QW]);
+            unsafeAtomicAdd(&pp[o * kNQ + 4], w.t[QK]);
+            atomicAdd(&ppn[o], (unsigned long long)w.t[QN]);
         }
-        for (int q = 0; q < kQ; ++q) wtab(w, lane, q) = 0.0;
+        for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
     }
     if (A.counters && lane == 0) {
         atomicAdd(&probe[0], (unsigned long long)cseg0);
