@@ -100,6 +100,7 @@ struct EdgeTest {
         if constexpr (sizeof(T) == 4) {
             bool p = d2 < ef;
             if (fabsf(d2 - ef) <= band_ef) {
+                asm volatile("" : "+v"(xa), "+v"(ya));
                 const double dx = (double)xa - (double)xs, dy = (double)ya - (double)ys;
                 p = fma(dx, dx, dy * dy) < ed;
             }
@@ -276,7 +277,11 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
                 // keeps this rare path a branch (not evaluated for every pair)
                 asm volatile("" ::: "memory");
                 if (amb) {
-                    const double ddx = (double)L.xa - (double)q.x, ddy = (double)L.ya - (double)q.y;
+                    // opaque copies: the FP64 lens coordinates are formed here,
+                    // not hoisted out of the loop into four long-lived registers
+                    float xa = L.xa, ya = L.ya;
+                    asm volatile("" : "+v"(xa), "+v"(ya));
+                    const double ddx = (double)xa - (double)q.x, ddy = (double)ya - (double)q.y;
                     below = fma(ddx, ddx, ddy * ddy) < et.ed;
                 }
             }
@@ -418,7 +423,15 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
 }
 
 template <typename T, int KIND, bool MARKS, bool MZ>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
+#ifndef TPCF_PAIR_WAVES_PER_SIMD
+#define TPCF_PAIR_WAVES_PER_SIMD 7
+#endif
+// Register budget for TPCF_PAIR_WAVES_PER_SIMD waves per SIMD (7: at most 72
+// VGPRs, a few cold-path spills; 6: 80 VGPRs, none); the 22.5 KB of LDS per
+// workgroup allows 7.  The pair loops are latency-bound below that: 5 waves
+// run the 20M x 20M workload in 6.73 s, 6 in 5.92 s, 7 in 5.60 s.
+__global__ __launch_bounds__(kWave *kWavesPerBlock)
+__attribute__((amdgpu_waves_per_eu(TPCF_PAIR_WAVES_PER_SIMD, 8))) void k_pairs(
     const PairArgs<T> A, const Chunk<T> *__restrict__ chunks, const T *__restrict__ lx, const T *__restrict__ ly,
     const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
     const uint32_t *__restrict__ seg_first, const int4 *__restrict__ pbox, const SrcRec<T> *__restrict__ srec,
@@ -473,8 +486,11 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void k_pairs(
         const int first = cy0 + ((split - (cy0 - ry0)) % n_split + n_split) % n_split;
         for (int cy = first; cy <= cy1; cy += n_split) {
             int cx0, cx1;
-            if (!col_span(g, (double)ch.bx0, (double)ch.bx1, (double)ch.by0, (double)ch.by1, cy, xm, mg, cx0, cx1))
-                continue;
+            // opaque copies: the FP64 box is formed per row instead of being
+            // hoisted into eight registers live across the whole loop nest
+            T bx0 = ch.bx0, bx1 = ch.bx1, by0 = ch.by0, by1 = ch.by1;
+            asm volatile("" : "+s"(bx0), "+s"(bx1), "+s"(by0), "+s"(by1));
+            if (!col_span(g, (double)bx0, (double)bx1, (double)by0, (double)by1, cy, xm, mg, cx0, cx1)) continue;
             cx0 = max(cx0, pbx.x);
             cx1 = min(cx1, pbx.y);
             if (cx0 > cx1) continue;
