@@ -111,44 +111,68 @@ struct EdgeTest {
     }
 };
 
-// Visits r[0..count) with the next two records already requested while the
-// current two are processed: copying the prefetched records into `c0/c1`
-// makes the wait for them happen before the next scalar loads are issued
-// (scalar loads return out of order, so a later wait would cover both).
-// The record array is padded with kRecPad records, so reading up to three
-// records past a segment's end stays inside the allocation.
+// Makes the record's values inputs of an (empty) instruction: the wait for
+// its scalar loads is placed here, before the loads issued next.
+template <typename T>
+__device__ __forceinline__ void arrived(const SrcRec<T> &q)
+{
+    if constexpr (sizeof(T) == 4)
+        asm volatile("" ::"s"(q.x), "s"(q.y), "s"(q.w), "s"(q.wk), "s"(q.s1w), "s"(q.s2w), "s"(q.s1w2), "s"(q.s2w2));
+    else
+        asm volatile("" ::"s"(q.x), "s"(q.y), "s"(q.w), "s"(q.wk), "s"(q.s1w), "s"(q.s2w), "s"(q.s1w2), "s"(q.s2w2));
+}
+
+// Visits r[0..count) two records at a time while the next two are already
+// requested, alternating between two register sets (no copies).  Scalar loads
+// return out of order, so a wait covers every load in flight: each set is
+// waited for (`arrived`) before the loads of the other set are issued.  The
+// record array is padded with kRecPad records, so reading up to three records
+// past a segment's end stays inside the allocation.
 template <typename T, typename F>
 __device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_t count, F &&f)
 {
     if (count == 0) return;
-    SrcRec<T> n0 = r[0], n1 = r[1];
-    for (uint32_t j = 0; j < count; j += 2) {
-        const SrcRec<T> c0 = n0, c1 = n1;
-        n0 = r[j + 2];
-        n1 = r[j + 3];
-        asm volatile("" ::: "memory");   // the next loads stay ahead of this iteration's work
-        f(c0);
-        if (j + 1 < count) f(c1);
+    SrcRec<T> a0 = r[0], a1 = r[1];
+    uint32_t j = 0;
+    for (;;) {
+        arrived(a0);
+        arrived(a1);
+        const SrcRec<T> b0 = r[j + 2], b1 = r[j + 3];
+        asm volatile("" ::: "memory");   // the loads stay ahead of this half's work
+        f(a0);
+        if (j + 1 < count) f(a1);
+        j += 2;
+        if (j >= count) break;
+        arrived(b0);
+        arrived(b1);
+        a0 = r[j + 2];
+        a1 = r[j + 3];
+        asm volatile("" ::: "memory");
+        f(b0);
+        if (j + 1 < count) f(b1);
+        j += 2;
+        if (j >= count) break;
     }
 }
 
 // Per-wave window state: two adjacent bins (wk, wk+1) of one source patch.
 // The per-lane FP64 window sums live in LDS, lane-private: acc[q][lane] for
-// q = Theta, G1, G2, W, K of bin wk (q) and bin wk+1 (kWQ + q), plus the
+// q = Theta, G1, G2, W, K of bin wk (q) and bin wk+1 (WQ + q), plus the
 // count of in-bin zero-separation pairs (shear with x_min == 0).  Pair counts
 // are wave totals (ballot + popcount) and go straight to the per-bin sums,
 // q = Theta, G1, G2, W, K, N, kept in registers by the lane of the bin.  Only
 // the window sums use LDS (5.6 KB per wave), which leaves room for 7
 // workgroups of 4 waves per CU.
 constexpr int kQ = 6;
-constexpr int kWQ = 5;
-constexpr int kWinQ = 2 * kWQ + 1;
+// window quantities per bin: Theta, G1, G2, W (+ K unless all m == 0)
+template <bool MZ>
+constexpr int win_q() { return MZ ? 4 : 5; }
 enum { QT = 0, QG1 = 1, QG2 = 2, QW = 3, QK = 4, QN = 5 };
 
 struct Win {
     int pb, wk;
     bool open, used0, used1, edge;
-    double *acc;   // this wave's [kWinQ][kWave] block in LDS
+    double *acc;   // this wave's [2 WQ + 1][kWave] block in LDS
     double t[6];   // per-bin sums Theta, G1, G2, W, K, N of bin `lane`, in registers (lane b owns bin b)
     uint64_t valid;   // lanes holding a lens
     uint32_t n_valid;
@@ -170,10 +194,11 @@ struct LaneData {
     int lane;
 };
 
+template <bool MZ>
 __device__ __forceinline__ void win_clear(Win &w, int lane)
 {
 #pragma unroll
-    for (int q = 0; q < kWinQ; ++q) wacc(w, q, lane) = 0.0;
+    for (int q = 0; q < 2 * win_q<MZ>() + 1; ++q) wacc(w, q, lane) = 0.0;
     w.open = false;
     w.used0 = w.used1 = false;
     w.edge = false;
@@ -184,6 +209,7 @@ __device__ __forceinline__ void win_clear(Win &w, int lane)
 template <typename T, int KIND, bool MARKS, bool MZ>
 __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bin, double *__restrict__ marks)
 {
+    constexpr int WQ = win_q<MZ>();
     if (!w.open) return;
     const int lane = L.lane;
     const double dwa = (double)L.wa, daa = (double)L.aa;
@@ -193,7 +219,7 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
         const int bin = w.wk + s;
         const bool used = s ? w.used1 : w.used0;
         if (!used || bin < 0 || bin >= n_bin) continue;
-        const int o = s * kWQ;
+        const int o = s * WQ;
         const double aT = wacc(w, o + QT, lane), aG1 = wacc(w, o + QG1, lane), aG2 = wacc(w, o + QG2, lane);
         const double sT = wave_sum(dwa * aT);
         double sG1 = 0, sG2 = 0, sW = 0, sK = 0;
@@ -216,7 +242,7 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
         // the position angle of a zero-separation pair is 0/0: the reference's
         // G_t and G_x of that bin, and the xi_E / xi_B marks of that lens,
         // become NaN (src/tpcf.f90:343-355)
-        const bool zero_sep = KIND == SHEAR && bin == 0 && w.edge && (double)L.vmask * wacc(w, 2 * kWQ, lane) > 0.0;
+        const bool zero_sep = KIND == SHEAR && bin == 0 && w.edge && (double)L.vmask * wacc(w, 2 * WQ, lane) > 0.0;
         if (KIND == SHEAR && bin == 0 && w.edge && __ballot(zero_sep)) {
             sG1 = __builtin_nan("");
             sG2 = __builtin_nan("");
@@ -238,7 +264,7 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
             unsafeAtomicAdd(&mk[3], dwa * aW);
         }
     }
-    win_clear(w, lane);
+    win_clear<MZ>(w, lane);
 }
 
 template <typename T, int KIND, bool MARKS, bool MZ>
@@ -262,6 +288,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
                                           const SrcRec<T> *__restrict__ r, const Seg<T> &sg, uint32_t count,
                                           int n_bin)
 {
+    constexpr int WQ = win_q<MZ>();
     const EdgeTest<T> et(A, w.wk + 1);
     T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, hN = 0, sZ = 0;
     stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
@@ -320,7 +347,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
     const int lane = L.lane;
     const double dhT = (double)hT, dhG1 = (double)hG1, dhG2 = (double)hG2, dhW = (double)hW, dhK = (double)hK;
     wacc(w, QT, lane) += (double)sT - dhT;
-    wacc(w, kWQ + QT, lane) += dhT;
+    wacc(w, WQ + QT, lane) += dhT;
     if (KIND == SHEAR) {
         wacc(w, QG1, lane) += (double)sG1 - dhG1;
         wacc(w, QG2, lane) += (double)sG2 - dhG2;
@@ -328,15 +355,15 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         wacc(w, QG1, lane) += sg.S1 - dhG1;
         wacc(w, QG2, lane) += sg.S2 - dhG2;
     }
-    wacc(w, kWQ + QG1, lane) += dhG1;
-    wacc(w, kWQ + QG2, lane) += dhG2;
+    wacc(w, WQ + QG1, lane) += dhG1;
+    wacc(w, WQ + QG2, lane) += dhG2;
     wacc(w, QW, lane) += sg.W - dhW;
-    wacc(w, kWQ + QW, lane) += dhW;
+    wacc(w, WQ + QW, lane) += dhW;
     if (!MZ) {
         wacc(w, QK, lane) += sg.K - dhK;
-        wacc(w, kWQ + QK, lane) += dhK;
+        wacc(w, WQ + QK, lane) += dhK;
     }
-    if (KIND == SHEAR && TRACK0) wacc(w, 2 * kWQ, lane) += (double)sZ;
+    if (KIND == SHEAR && TRACK0) wacc(w, 2 * WQ, lane) += (double)sZ;
     // per-lane counts are integers <= kSegMax: their wave sum is exact in FP32
     const double nhi = (double)wave_sum((float)(L.vmask * hN));
     tab_count(w, lane, w.wk, n_bin, (double)count * (double)w.n_valid - nhi);
@@ -352,6 +379,7 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
                                          const SrcRec<T> *__restrict__ r, const T *__restrict__ zr, uint32_t count,
                                          int n_bin)
 {
+    constexpr int WQ = win_q<MZ>();
     const int lo_e = w.wk, hi_e = w.wk + 2;
     const bool has_lo = lo_e >= 0, has_hi = hi_e <= n_bin;
     const EdgeTest<T> emid(A, w.wk + 1), elo(A, has_lo ? lo_e : 0), ehi(A, has_hi ? hi_e : 0);
@@ -407,31 +435,33 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
         }
     }
     const int lane = L.lane;
-    wacc(w, QT, lane) += (double)sT0; wacc(w, kWQ + QT, lane) += (double)sT1;
-    wacc(w, QG1, lane) += (double)sG10; wacc(w, kWQ + QG1, lane) += (double)sG11;
-    wacc(w, QG2, lane) += (double)sG20; wacc(w, kWQ + QG2, lane) += (double)sG21;
-    wacc(w, QW, lane) += (double)sW0; wacc(w, kWQ + QW, lane) += (double)sW1;
+    wacc(w, QT, lane) += (double)sT0; wacc(w, WQ + QT, lane) += (double)sT1;
+    wacc(w, QG1, lane) += (double)sG10; wacc(w, WQ + QG1, lane) += (double)sG11;
+    wacc(w, QG2, lane) += (double)sG20; wacc(w, WQ + QG2, lane) += (double)sG21;
+    wacc(w, QW, lane) += (double)sW0; wacc(w, WQ + QW, lane) += (double)sW1;
     if (!MZ) {
         wacc(w, QK, lane) += (double)sK0;
-        wacc(w, kWQ + QK, lane) += (double)sK1;
+        wacc(w, WQ + QK, lane) += (double)sK1;
     }
-    if (KIND == SHEAR) wacc(w, 2 * kWQ, lane) += (double)sZ;
+    if (KIND == SHEAR) wacc(w, 2 * WQ, lane) += (double)sZ;
     tab_count(w, lane, w.wk, n_bin, (double)n0);
     tab_count(w, lane, w.wk + 1, n_bin, (double)n1);
     w.used0 = w.used1 = true;
     w.edge = true;
 }
 
+// Waves per SIMD the register budget is set for.  The pair loops are
+// latency-bound: on the 20M x 20M shear workload 5 waves take 6.73 s, 6 take
+// 5.92 s, 7 take 5.60 s and 8 take 5.35 s.  FP32 with all m == 0 (MZ) keeps
+// 18 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
+// few cold-path spills); otherwise 22.5 KB allows 7 (72 VGPRs).  FP64 keeps 4
+// waves (no spills).
+template <typename T, bool MZ>
+constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : 4; }
+
 template <typename T, int KIND, bool MARKS, bool MZ>
-#ifndef TPCF_PAIR_WAVES_PER_SIMD
-#define TPCF_PAIR_WAVES_PER_SIMD 7
-#endif
-// Register budget for TPCF_PAIR_WAVES_PER_SIMD waves per SIMD (7: at most 72
-// VGPRs, a few cold-path spills; 6: 80 VGPRs, none); the 22.5 KB of LDS per
-// workgroup allows 7.  The pair loops are latency-bound below that: 5 waves
-// run the 20M x 20M workload in 6.73 s, 6 in 5.92 s, 7 in 5.60 s.
 __global__ __launch_bounds__(kWave *kWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(TPCF_PAIR_WAVES_PER_SIMD, 8))) void k_pairs(
+__attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pairs(
     const PairArgs<T> A, const Chunk<T> *__restrict__ chunks, const T *__restrict__ lx, const T *__restrict__ ly,
     const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
     const uint32_t *__restrict__ seg_first, const int4 *__restrict__ pbox, const SrcRec<T> *__restrict__ srec,
@@ -461,10 +491,11 @@ __attribute__((amdgpu_waves_per_eu(TPCF_PAIR_WAVES_PER_SIMD, 8))) void k_pairs(
     const int n_bin = A.n_bin;
     const T e2_lane = e2lane[lane];   // squared edge `lane` (+inf past n_bin)
 
-    __shared__ double s_acc[kWavesPerBlock][kWinQ * kWave];
+    constexpr int WQ = win_q<MZ>();
+    __shared__ double s_acc[kWavesPerBlock][(2 * WQ + 1) * kWave];
     Win w;
     w.acc = s_acc[wv];
-    win_clear(w, lane);
+    win_clear<MZ>(w, lane);
     w.pb = -1;
     w.wk = -3;
     w.valid = __ballot(L.valid);
@@ -539,7 +570,7 @@ __attribute__((amdgpu_waves_per_eu(TPCF_PAIR_WAVES_PER_SIMD, 8))) void k_pairs(
                             sT = fma(Geo<T>::dist(d2), q.w, sT);
                         });
                     }
-                    const int o = (kmin - w.wk) * kWQ;
+                    const int o = (kmin - w.wk) * WQ;
                     if (o == 0) w.used0 = true; else w.used1 = true;
                     wacc(w, o + QT, lane) += (double)sT;
                     if (KIND == SHEAR) {
