@@ -35,6 +35,7 @@ EXPORTS = (
     "tpcf_create_bins", "tpcf_find_bin_index", "tpcf_ctx_create", "tpcf_ctx_destroy", "tpcf_set_option",
     "tpcf_set_stream", "tpcf_get_stats", "tpcf_last_error",
     "tpcf_shear_f32", "tpcf_kappa_f32", "tpcf_shear_f64", "tpcf_kappa_f64",
+    "tpcf_adjoint_shear_f32", "tpcf_adjoint_shear_f64", "tpcf_adjoint_kappa_f32", "tpcf_adjoint_kappa_f64",
 )
 
 
@@ -119,6 +120,13 @@ def lib():
                                                                  C.POINTER(Out)]
     for name, args in (("tpcf_shear_f32", shear_args), ("tpcf_shear_f64", shear_args),
                        ("tpcf_kappa_f32", kappa_args), ("tpcf_kappa_f64", kappa_args)):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    adj_shear = shear_args[:-7] + [C.POINTER(Bins), C.c_uint32, _vp, _vp] + [_vp] * 6
+    adj_kappa = kappa_args[:-7] + [C.POINTER(Bins), C.c_uint32, _vp, _vp] + [_vp] * 5
+    for name, args in (("tpcf_adjoint_shear_f32", adj_shear), ("tpcf_adjoint_shear_f64", adj_shear),
+                       ("tpcf_adjoint_kappa_f32", adj_kappa), ("tpcf_adjoint_kappa_f64", adj_kappa)):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int

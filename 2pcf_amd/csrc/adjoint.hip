@@ -1,0 +1,342 @@
+// adjoint.hip -- reverse mode of the per-bin outputs: given the gradient of a
+// scalar loss with respect to g_t / g_x (shear) or xi_E / xi_B (kappa) per bin,
+// the gradient with respect to every object's value and weight.  The
+// reference has no derivative; the forward quantities are those of
+// tangential_shear_tpcf (src/tpcf.f90:340-352, :369-376) and scalar_kappa_tpcf
+// (src/tpcf.f90:512-519, :536-543).
+//
+// With a_i = val_i w_i and per bin b (sums over the pairs (i, j) in bin b):
+//   shear  G_b = -sum a_i w_j (e1_j c_ij + e2_j s_ij),  X_b = -sum a_i w_j (-e1_j s_ij + e2_j c_ij),
+//          K_b = sum w_i w_j (1 + m_j),  g_t = G/K, g_x = X/K;
+//   kappa  E_b = sum a_i w_j kE_j,  B_b = sum a_i w_j kB_j,  W_b = sum w_i w_j,  xi = E/W, B/W.
+// With upstream gradients lt_b, lx_b, alpha_b = lt_b / K_b, alpha'_b = lx_b / K_b and
+// beta_b = (lt_b g_t,b + lx_b g_x,b) / K_b, the gradient is a per-object
+// contraction over bins of per-object sums over its partners:
+//   lens i:    P_ib = -sum_j w_j (e1_j c + e2_j s), Q_ib = -sum_j w_j (-e1_j s + e2_j c), kap_ib = sum_j w_j (1 + m_j)
+//              dval_i = w_i sum_b (alpha P + alpha' Q),  dw_i = val_i sum_b (alpha P + alpha' Q) - sum_b beta kap
+//   source j:  C_jb = -sum_i a_i c_ij, S'_jb = sum_i a_i s_ij, Om_jb = sum_i w_i
+//              de1_j = w_j sum_b (alpha C + alpha' S'),  de2_j = w_j sum_b (-alpha S' + alpha' C),
+//              dw_j = sum_b (alpha (e1 C - e2 S') + alpha' (e1 S' + e2 C)) - (1 + m_j) sum_b beta Om,
+//              dm_j = -w_j sum_b beta Om
+// and for kappa (S^E_ib = sum_j w_j kE_j, S^B_ib, Om_ib = sum_j w_j; A_jb = sum_i a_i, Om_jb = sum_i w_i):
+//              dval_i = w_i sum_b (alpha S^E + alpha' S^B),  dw_i = val_i sum_b (...) - sum_b beta Om_ib,
+//              dkE_j = w_j sum_b alpha A,  dkB_j = w_j sum_b alpha' A,  dw_j = sum_b (alpha kE_j + alpha' kB_j) A - sum_b beta Om_jb.
+// The per-object sums are the pair kernel's marks: once with unit lens values
+// and weights (lens side), once with the roles of the catalogues swapped
+// (source side: the sources become the lanes, the lenses are streamed with
+// their a_i as the first component; the z test is reversed by storing -z).
+// Lenses with w = 0 (skipped by the reference) get the gradient of the
+// smooth extension; objects masked by z < 0 get zero.
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace tpcf {
+
+void fill_f64(hipStream_t s, double *p, int64_t n);
+
+namespace {
+
+struct Coef {
+    double at[TPCF_MAX_BIN], ax[TPCF_MAX_BIN], be[TPCF_MAX_BIN];
+};
+
+template <typename T>
+__global__ void k_fill_value(T *p, int64_t n, T v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Lens side: marks[li][b] = (P or S^E, Q or S^B, kap, Om) of sorted lens li.
+template <typename T>
+__global__ void k_adj_lens(int64_t n_lens, int32_t n_bin, const uint32_t *__restrict__ lidx,
+                           const double *__restrict__ mk, const T *__restrict__ val, const T *__restrict__ w,
+                           const Coef c, double *__restrict__ dval, double *__restrict__ dw)
+{
+    const int64_t li = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (li >= n_lens) return;
+    const uint32_t o = lidx[li];
+    double s = 0.0, sk = 0.0;
+    for (int b = 0; b < n_bin; ++b) {
+        const double *m4 = mk + ((size_t)li * n_bin + b) * 4;
+        s += c.at[b] * m4[0] + c.ax[b] * m4[1];
+        sk += c.be[b] * m4[2];
+    }
+    if (dval) dval[o] = (double)w[o] * s;
+    if (dw) dw[o] = (double)val[o] * s - sk;
+}
+
+// Source side (roles swapped): marks[li][b] = (C or A, S', ., Om) of the
+// sorted source li.
+template <typename T, int KIND>
+__global__ void k_adj_src(int64_t n_src, int32_t n_bin, const uint32_t *__restrict__ lidx,
+                          const double *__restrict__ mk, const T *__restrict__ s1, const T *__restrict__ s2,
+                          const T *__restrict__ w, const T *__restrict__ m, const Coef c, double *__restrict__ d1,
+                          double *__restrict__ d2, double *__restrict__ dw, double *__restrict__ dm)
+{
+    const int64_t li = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (li >= n_src) return;
+    const uint32_t o = lidx[li];
+    const double e1 = (double)s1[o], e2 = (double)s2[o], wj = (double)w[o];
+    const double opm = (KIND == SHEAR && m) ? 1.0 + (double)m[o] : 1.0;
+    double a1 = 0.0, a2 = 0.0, aw = 0.0, bo = 0.0;
+    for (int b = 0; b < n_bin; ++b) {
+        const double *m4 = mk + ((size_t)li * n_bin + b) * 4;
+        const double P = m4[0], Q = m4[1], Om = m4[3];
+        if (KIND == SHEAR) {
+            a1 += c.at[b] * P + c.ax[b] * Q;
+            a2 += -c.at[b] * Q + c.ax[b] * P;
+            aw += c.at[b] * (e1 * P - e2 * Q) + c.ax[b] * (e1 * Q + e2 * P);
+        } else {
+            a1 += c.at[b] * P;
+            a2 += c.ax[b] * P;
+            aw += (c.at[b] * e1 + c.ax[b] * e2) * P;
+        }
+        bo += c.be[b] * Om;
+    }
+    if (d1) d1[o] = wj * a1;
+    if (d2) d2[o] = wj * a2;
+    if (dw) dw[o] = aw - opm * bo;
+    if (dm) dm[o] = -wj * bo;
+}
+
+template <typename T>
+int upload_in(tpcf_ctx *ctx, int which, const T *src, int64_t n, const T **dst)
+{
+    if (!src) {
+        *dst = nullptr;
+        return TPCF_OK;
+    }
+    void *p;
+    TPCF_TRY(ensure(ctx, which, sizeof(T) * std::max<int64_t>(n, 1), &p));
+    if (n > 0) TPCF_HIP(ctx, hipMemcpyAsync(p, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream));
+    *dst = static_cast<const T *>(p);
+    return TPCF_OK;
+}
+
+template <typename T>
+int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala, const T *wa,
+            int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b, const T *wb,
+            const T *mb, const tpcf_bins *bins, uint32_t flags, const double *gbar1, const double *gbar2,
+            double *dval_a, double *dw_a, double *d1_b, double *d2_b, double *dw_b, double *dm_b)
+{
+    if (!ctx) return TPCF_ERR_ARG;
+    ctx->err.clear();
+    if (!bins || bins->n_bin < 1 || bins->n_bin > TPCF_MAX_BIN) return fail(ctx, TPCF_ERR_BINS, "bad bins");
+    if (flags & TPCF_SPHERICAL)
+        return fail(ctx, TPCF_ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build");
+    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE))
+        return fail(ctx, TPCF_ERR_ARG, "unknown flag");
+    if (!gbar1 || !gbar2) return fail(ctx, TPCF_ERR_ARG, "upstream gradients are NULL");
+    if (n_a < 0 || n_b < 0 || n_a >= ((int64_t)1 << 31) || n_b >= ((int64_t)1 << 31))
+        return fail(ctx, TPCF_ERR_ARG, "catalogue sizes must be in [0, 2^31)");
+    const bool shear = (kind == SHEAR);
+    if (n_a > 0 && (!xa || !ya || !vala || !wa || (shear && !za)))
+        return fail(ctx, TPCF_ERR_ARG, "a required lens array is NULL");
+    if (n_b > 0 && (!xb || !yb || !s1b || !s2b || !wb || (shear && (!zb || !mb))))
+        return fail(ctx, TPCF_ERR_ARG, "a required source array is NULL");
+    TPCF_HIP(ctx, hipSetDevice(ctx->device));
+    const double t0 = now_ms();
+    hipStream_t s = ctx->stream;
+    const int n_bin = bins->n_bin;
+    const bool dev = (flags & TPCF_DEVICE_INPUTS) != 0;
+    const uint32_t run_flags = flags | TPCF_DEVICE_INPUTS;
+
+    CallArgs<T> base;
+    base.kind = kind;
+    base.n_a = n_a;
+    base.n_b = n_b;
+    base.n_pa = base.n_pb = 1;
+    base.patch_a = base.patch_b = nullptr;
+    base.bins = bins;
+    base.marks = false;
+    base.verbose = (flags & TPCF_VERBOSE) != 0;
+    if (dev) {
+        base.xa = xa; base.ya = ya; base.za = za; base.vala = vala; base.wa = wa;
+        base.xb = xb; base.yb = yb; base.zb = zb; base.s1b = s1b; base.s2b = s2b; base.wb = wb; base.mb = mb;
+    } else {
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN0, xa, n_a, &base.xa));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN1, ya, n_a, &base.ya));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN2, shear ? za : (const T *)nullptr, n_a, &base.za));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN3, vala, n_a, &base.vala));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN4, wa, n_a, &base.wa));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN5, xb, n_b, &base.xb));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN6, yb, n_b, &base.yb));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN7, shear ? zb : (const T *)nullptr, n_b, &base.zb));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN8, s1b, n_b, &base.s1b));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN9, s2b, n_b, &base.s2b));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN10, wb, n_b, &base.wb));
+        TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN11, shear ? mb : (const T *)nullptr, n_b, &base.mb));
+    }
+    if (!shear) { base.za = nullptr; base.zb = nullptr; base.mb = nullptr; }
+
+    // ---- forward: per-bin raw sums ----
+    double G[TPCF_MAX_BIN], X[TPCF_MAX_BIN], K[TPCF_MAX_BIN];
+    {
+        tpcf_out o;
+        std::memset(&o, 0, sizeof o);
+        o.real_bytes = 8;
+        o.sum_gt = G;
+        o.sum_gx = X;
+        o.sum_k = K;
+        CallArgs<T> f = base;
+        TPCF_TRY(run_call<T>(ctx, f, run_flags, &o, t0, nullptr));
+    }
+    Coef c;
+    std::memset(&c, 0, sizeof c);
+    for (int b = 0; b < n_bin; ++b) {
+        if (!(K[b] > 0.0)) continue;   // g = 0 there: no dependence
+        const double g1 = G[b] / K[b], g2 = X[b] / K[b];
+        c.at[b] = gbar1[b] / K[b];
+        c.ax[b] = gbar2[b] / K[b];
+        c.be[b] = (gbar1[b] * g1 + gbar2[b] * g2) / K[b];
+    }
+
+    // ---- gradient buffers (device), zero for masked objects ----
+    double *gA, *gB;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_GRAD_A, sizeof(double) * 2 * std::max<int64_t>(n_a, 1), (void **)&gA));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_GRAD_B, sizeof(double) * 4 * std::max<int64_t>(n_b, 1), (void **)&gB));
+    double *dvA = dev ? dval_a : gA, *dwA = dev ? dw_a : gA + n_a;
+    double *d1B = dev ? d1_b : gB, *d2B = dev ? d2_b : gB + n_b, *dwB = dev ? dw_b : gB + 2 * n_b,
+           *dmB = dev ? dm_b : gB + 3 * n_b;
+    if (!dev) {
+        // host outputs: every device slot is computed and only the requested ones copied
+    } else {
+        // device outputs: only the requested ones are computed
+        if (!dval_a) dvA = nullptr;
+        if (!dw_a) dwA = nullptr;
+        if (!d1_b) d1B = nullptr;
+        if (!d2_b) d2B = nullptr;
+        if (!dw_b) dwB = nullptr;
+        if (!dm_b || !shear) dmB = nullptr;
+    }
+    if (!shear && !dev) dmB = nullptr;
+    double *zero_list[6] = {dvA, dwA, d1B, d2B, dwB, dmB};
+    const int64_t zero_n[6] = {n_a, n_a, n_b, n_b, n_b, n_b};
+    for (int q = 0; q < 6; ++q)
+        if (zero_list[q] && zero_n[q] > 0) fill_f64(s, zero_list[q], zero_n[q]);
+
+    T *ones, *zeros;
+    const int64_t n_max = std::max<int64_t>(std::max(n_a, n_b), 1);
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_ONES, sizeof(T) * n_max, (void **)&ones));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_ZEROS, sizeof(T) * n_max, (void **)&zeros));
+    const unsigned fb = (unsigned)std::min<int64_t>(4096, (n_max + 255) / 256);
+    hipLaunchKernelGGL(k_fill_value<T>, dim3(fb), dim3(256), 0, s, ones, n_max, (T)1);
+    hipLaunchKernelGGL(k_fill_value<T>, dim3(fb), dim3(256), 0, s, zeros, n_max, (T)0);
+    TPCF_HIP(ctx, hipGetLastError());
+
+    tpcf_out none;
+    std::memset(&none, 0, sizeof none);
+    none.real_bytes = 8;
+    // ---- lens side: unit lens values and weights, real sources ----
+    if (n_a > 0 && n_b > 0 && (dvA || dwA)) {
+        CallArgs<T> l = base;
+        l.vala = ones;
+        l.wa = ones;
+        l.marks = true;
+        l.marks_device = true;
+        Prepared<T> p;
+        TPCF_TRY(run_call<T>(ctx, l, run_flags, &none, t0, &p));
+        const double *mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
+        if (p.n_lens > 0) {
+            hipLaunchKernelGGL(k_adj_lens<T>, dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s, p.n_lens, n_bin,
+                               p.lidx, mk, base.vala, base.wa, c, dvA, dwA);
+            TPCF_HIP(ctx, hipGetLastError());
+        }
+    }
+    // ---- source side: the sources as lanes, the lenses streamed (a_i, 0) ----
+    if (n_a > 0 && n_b > 0 && (d1B || d2B || dwB || dmB)) {
+        CallArgs<T> r = base;
+        r.n_a = n_b;
+        r.xa = base.xb; r.ya = base.yb; r.za = base.zb;
+        r.vala = ones;
+        r.wa = ones;
+        r.n_b = n_a;
+        r.xb = base.xa; r.yb = base.ya; r.zb = base.za;
+        r.s1b = base.vala;
+        r.s2b = zeros;
+        r.wb = base.wa;
+        r.mb = shear ? zeros : nullptr;
+        r.zsign = -1;   // the z test z_A > z_B of the original roles
+        r.marks = true;
+        r.marks_device = true;
+        Prepared<T> p;
+        TPCF_TRY(run_call<T>(ctx, r, run_flags, &none, t0, &p));
+        const double *mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
+        if (p.n_lens > 0) {
+            if (shear)
+                hipLaunchKernelGGL((k_adj_src<T, SHEAR>), dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
+                                   p.n_lens, n_bin, p.lidx, mk, base.s1b, base.s2b, base.wb, base.mb, c, d1B, d2B, dwB,
+                                   dmB);
+            else
+                hipLaunchKernelGGL((k_adj_src<T, KAPPA>), dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
+                                   p.n_lens, n_bin, p.lidx, mk, base.s1b, base.s2b, base.wb, (const T *)nullptr, c, d1B,
+                                   d2B, dwB, (double *)nullptr);
+            TPCF_HIP(ctx, hipGetLastError());
+        }
+    }
+    if (!dev) {
+        double *dst[6] = {dval_a, dw_a, d1_b, d2_b, dw_b, shear ? dm_b : nullptr};
+        for (int q = 0; q < 6; ++q)
+            if (dst[q] && zero_n[q] > 0)
+                TPCF_HIP(ctx, hipMemcpyAsync(dst[q], zero_list[q], sizeof(double) * zero_n[q], hipMemcpyDeviceToHost, s));
+        if (!shear && dm_b && n_b > 0) std::memset(dm_b, 0, sizeof(double) * n_b);
+    }
+    TPCF_HIP(ctx, hipStreamSynchronize(s));
+    ctx->stats.ms_total = now_ms() - t0;
+    return TPCF_OK;
+}
+
+}  // namespace
+}  // namespace tpcf
+
+using namespace tpcf;
+
+extern "C" {
+
+int tpcf_adjoint_shear_f32(tpcf_ctx *ctx, int64_t n_A, const float *x_A, const float *y_A, const float *z_A,
+                           const float *val_A, const float *w_A, int64_t n_B, const float *x_B, const float *y_B,
+                           const float *z_B, const float *e1_B, const float *e2_B, const float *w_B,
+                           const float *m_B, const tpcf_bins *bins, uint32_t flags, const double *gbar_t,
+                           const double *gbar_x, double *val_A_bar, double *w_A_bar, double *e1_B_bar,
+                           double *e2_B_bar, double *w_B_bar, double *m_B_bar)
+{
+    return adjoint<float>(ctx, SHEAR, n_A, x_A, y_A, z_A, val_A, w_A, n_B, x_B, y_B, z_B, e1_B, e2_B, w_B, m_B, bins,
+                          flags, gbar_t, gbar_x, val_A_bar, w_A_bar, e1_B_bar, e2_B_bar, w_B_bar, m_B_bar);
+}
+
+int tpcf_adjoint_shear_f64(tpcf_ctx *ctx, int64_t n_A, const double *x_A, const double *y_A, const double *z_A,
+                           const double *val_A, const double *w_A, int64_t n_B, const double *x_B, const double *y_B,
+                           const double *z_B, const double *e1_B, const double *e2_B, const double *w_B,
+                           const double *m_B, const tpcf_bins *bins, uint32_t flags, const double *gbar_t,
+                           const double *gbar_x, double *val_A_bar, double *w_A_bar, double *e1_B_bar,
+                           double *e2_B_bar, double *w_B_bar, double *m_B_bar)
+{
+    return adjoint<double>(ctx, SHEAR, n_A, x_A, y_A, z_A, val_A, w_A, n_B, x_B, y_B, z_B, e1_B, e2_B, w_B, m_B, bins,
+                           flags, gbar_t, gbar_x, val_A_bar, w_A_bar, e1_B_bar, e2_B_bar, w_B_bar, m_B_bar);
+}
+
+int tpcf_adjoint_kappa_f32(tpcf_ctx *ctx, int64_t n_A, const float *x_A, const float *y_A, const float *val_A,
+                           const float *w_A, int64_t n_B, const float *x_B, const float *y_B, const float *kappa_E,
+                           const float *kappa_B, const float *w_B, const tpcf_bins *bins, uint32_t flags,
+                           const double *gbar_E, const double *gbar_B, double *val_A_bar, double *w_A_bar,
+                           double *kE_B_bar, double *kB_B_bar, double *w_B_bar)
+{
+    return adjoint<float>(ctx, KAPPA, n_A, x_A, y_A, nullptr, val_A, w_A, n_B, x_B, y_B, nullptr, kappa_E, kappa_B,
+                          w_B, nullptr, bins, flags, gbar_E, gbar_B, val_A_bar, w_A_bar, kE_B_bar, kB_B_bar, w_B_bar,
+                          nullptr);
+}
+
+int tpcf_adjoint_kappa_f64(tpcf_ctx *ctx, int64_t n_A, const double *x_A, const double *y_A, const double *val_A,
+                           const double *w_A, int64_t n_B, const double *x_B, const double *y_B,
+                           const double *kappa_E, const double *kappa_B, const double *w_B, const tpcf_bins *bins,
+                           uint32_t flags, const double *gbar_E, const double *gbar_B, double *val_A_bar,
+                           double *w_A_bar, double *kE_B_bar, double *kB_B_bar, double *w_B_bar)
+{
+    return adjoint<double>(ctx, KAPPA, n_A, x_A, y_A, nullptr, val_A, w_A, n_B, x_B, y_B, nullptr, kappa_E, kappa_B,
+                           w_B, nullptr, bins, flags, gbar_E, gbar_B, val_A_bar, w_A_bar, kE_B_bar, kB_B_bar, w_B_bar,
+                           nullptr);
+}
+
+}  // extern "C"

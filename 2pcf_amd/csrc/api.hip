@@ -51,6 +51,12 @@ int ensure(tpcf_ctx *ctx, int which, size_t bytes, void **ptr)
     return TPCF_OK;
 }
 
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
 namespace {
 
 // Sum of the per-XCD accumulator copies into the output layout
@@ -161,12 +167,6 @@ int upload(tpcf_ctx *ctx, int which, const T *src, int64_t n, const T **dst)
     return TPCF_OK;
 }
 
-double now_ms()
-{
-    using namespace std::chrono;
-    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
-}
-
 template <typename T>
 int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala,
               const T *wa, int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b,
@@ -197,10 +197,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
     TPCF_HIP(ctx, hipSetDevice(ctx->device));
 
     const double t0 = now_ms();
-    const int n_bin = bins->n_bin;
-    const int rb = out->real_bytes;
     hipStream_t s = ctx->stream;
-    tpcf_stats st{};
 
     CallArgs<T> a;
     a.kind = kind;
@@ -233,7 +230,27 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
         TPCF_TRY(upload(ctx, tpcf_ctx::B_PATCH_B, pat_b, n_b, &a.patch_b));
     }
     if (!shear) { a.za = nullptr; a.zb = nullptr; a.mb = nullptr; }
-    TPCF_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    return run_call<T>(ctx, a, flags, out, t0, nullptr);
+}
+
+}  // namespace
+
+// Everything after the inputs are on the device: preparation, the pair
+// kernel, reductions and outputs.  With a.marks_device the marks stay in the
+// context's FP64 buffer (sorted lens order) for the adjoint; *pout receives the
+// preparation (lens order) when given.
+template <typename T>
+int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, double t0, Prepared<T> *pout)
+{
+    const int64_t n_a = a.n_a, n_b = a.n_b;
+    const int32_t n_pa = a.n_pa, n_pb = a.n_pb;
+    const bool shear = (a.kind == SHEAR);
+    const tpcf_bins *bins = a.bins;
+    const int n_bin = bins->n_bin;
+    const int rb = out->real_bytes;
+    hipStream_t s = ctx->stream;
+    tpcf_stats st{};
+    TPCF_HIP(ctx, hipEventRecord(ctx->ev[1], s));   // inputs on the device (ev[0]: before the uploads)
 
     std::vector<double> tot((size_t)n_bin * (kNQ + 1), 0.0);
     const int64_t n_pp = (int64_t)n_pa * n_pb;
@@ -282,8 +299,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
             TPCF_HIP(ctx, hipMemcpyAsync(hpp.data(), ppo, sizeof(double) * hpp.size(), hipMemcpyDeviceToHost, s));
             TPCF_HIP(ctx, hipMemcpyAsync(hppn.data(), ppno, sizeof(long long) * hppn.size(), hipMemcpyDeviceToHost, s));
         }
-        if (a.marks) {
-            // marks share B_MARKS' neighbour buffer B_IN0 only when inputs were uploaded; use a dedicated buffer
+        if (a.marks && !a.marks_device) {
             void *mo;
             const size_t plane = (size_t)n_a * n_bin;
             TPCF_TRY(ensure(ctx, tpcf_ctx::B_TMP, (size_t)rb * (4 * plane + n_a), &mo));
@@ -319,7 +335,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
         }
         const size_t plane = (size_t)n_a * n_bin;
         void *dsts[5] = {out->xi_E_marks, out->xi_B_marks, out->K_marks, out->w_marks, out->marks_weight};
-        for (int q = 0; q < 5; ++q)
+        for (int q = 0; q < 5 && !a.marks_device; ++q)
             if (dsts[q]) std::memset(dsts[q], 0, (size_t)rb * ((q < 4) ? plane : (size_t)n_a));
     }
     TPCF_HIP(ctx, hipEventRecord(ctx->ev[4], s));
@@ -388,10 +404,13 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
                     p.grid.cell, (long long)p.n_chunks, (long long)p.n_segs, (long long)npairs, st.ms_pairs, st.ms_total);
         std::fflush(stdout);
     }
+    if (pout) *pout = p;
     return TPCF_OK;
 }
 
-}  // namespace
+template int run_call<float>(tpcf_ctx *, CallArgs<float> &, uint32_t, tpcf_out *, double, Prepared<float> *);
+template int run_call<double>(tpcf_ctx *, CallArgs<double> &, uint32_t, tpcf_out *, double, Prepared<double> *);
+
 }  // namespace tpcf
 
 extern "C" {

@@ -197,6 +197,7 @@ struct tpcf_ctx {
         B_LX, B_LY, B_LZ, B_LA, B_LW, B_CHUNKS,
         B_PP, B_PPN, B_MARKS, B_OUT, B_TMP, B_PROBE, B_E2,
         B_BLK, B_BLKCH, B_WORK, B_ITEMS,
+        B_GRAD_A, B_GRAD_B, B_ONES, B_ZEROS,
         B_COUNT
     };
     tpcf::DevBuf buf[B_COUNT];
@@ -234,6 +235,8 @@ struct CallArgs {
     const tpcf_bins *bins;
     bool marks;
     bool verbose;
+    int zsign = 1;              // z as stored for the z test: +z, or -z to reverse z_A > z_B
+    bool marks_device = false;  // keep the marks on the device (adjoint) instead of copying them out
 };
 
 // Device-side result of the preparation stage.
@@ -262,6 +265,12 @@ struct Prepared {
 
 template <typename T>
 int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p);
+
+// The correlation call once the inputs are on the device (api.hip).
+template <typename T>
+int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, double t0, Prepared<T> *pout);
+
+double now_ms();
 
 template <typename T>
 int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double *pp, unsigned long long *ppn,

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlk) void k_src_gather(int64_t n, int kind, const 
                                                      const T *__restrict__ z, const T *__restrict__ s1,
                                                      const T *__restrict__ s2, const T *__restrict__ w,
                                                      const T *__restrict__ m, SrcRec<T> *__restrict__ rec,
-                                                     T *__restrict__ zs, int32_t *__restrict__ flags)
+                                                     T *__restrict__ zs, int32_t *__restrict__ flags, T zsign)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlk) void k_src_gather(int64_t n, int kind, const 
     r.s1w2 = (T)2 * r.s1w;
     r.s2w2 = (T)2 * r.s2w;
     rec[i] = r;
-    if (zs) zs[i] = z[j];
+    if (zs) zs[i] = zsign * z[j];
 }
 
 template <typename T>
@@ -213,14 +213,15 @@ __global__ __launch_bounds__(kBlk) void k_lens_gather(int64_t n, const uint32_t 
                                                       const T *__restrict__ x, const T *__restrict__ y,
                                                       const T *__restrict__ z, const T *__restrict__ val,
                                                       const T *__restrict__ w, T *__restrict__ lx, T *__restrict__ ly,
-                                                      T *__restrict__ lz, T *__restrict__ la, T *__restrict__ lw)
+                                                      T *__restrict__ lz, T *__restrict__ la, T *__restrict__ lw,
+                                                      T zsign)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = order[i];
     lx[i] = x[j];
     ly[i] = y[j];
-    lz[i] = z ? z[j] : (T)0;
+    lz[i] = z ? zsign * z[j] : (T)0;
     la[i] = val[j] * w[j];
     lw[i] = w[j];
 }
@@ -436,7 +437,7 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_PBOX, sizeof(int4) * a.n_pb, (void **)&pbox));
     if (p.n_src > 0)
         hipLaunchKernelGGL(k_src_gather<T>, dim3(blocks_for(p.n_src)), dim3(kBlk), 0, s, p.n_src, a.kind, sidx2, a.xb,
-                           a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb, srec, sz, err);
+                           a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb, srec, sz, err, (T)a.zsign);
     launch_fill<int4>(s, pbox, make_int4(INT_MAX, -1, INT_MAX, -1), a.n_pb);
     hipLaunchKernelGGL(k_segs<T>, dim3(blocks_for(key_space)), dim3(kBlk), 0, s, key_space, g, counts, cstart,
                        seg_first, srec, sz, segs, pbox);
@@ -499,7 +500,7 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     std::vector<uint32_t> hnch(nblk), hchoff(nblk);
     if (nl > 0) {
         hipLaunchKernelGGL(k_lens_gather<T>, dim3(blocks_for(nl)), dim3(kBlk), 0, s, nl, lidx2, a.xa, a.ya,
-                           shear ? a.za : nullptr, a.vala, a.wa, lx, ly, lz, la, lw);
+                           shear ? a.za : nullptr, a.vala, a.wa, lx, ly, lz, la, lw, (T)a.zsign);
         hipLaunchKernelGGL((k_chunk_blocks<T, false>), dim3(blocks_for(nblk)), dim3(kBlk), 0, s, nblk, dblk, dmax, lx,
                            ly, lz, lw, la, dnch, nullptr, nullptr);
         TPCF_HIP(ctx, hipMemcpyAsync(hnch.data(), dnch, sizeof(uint32_t) * nblk, hipMemcpyDeviceToHost, s));

@@ -324,3 +324,66 @@ def stack_fields(results: list[Shear2pcfResults]) -> dict:
         mt = sum(np.asarray(r.weights, np.float64) * np.asarray(r.mean_theta, np.float64) for r in results) / W
     return dict(theta=results[0].theta, g_t=g_t, g_x=g_x, weights=W,
                 n_pairs=sum(np.asarray(r.n_pairs) for r in results), mean_theta=mt)
+
+
+# ---------------------------------------------------------------- adjoint ----
+_GRAD_KEYS = {SHEAR: (("val", "w"), ("e1", "e2", "w", "m")), KAPPA: (("val", "w"), ("kappa_E", "kappa_B", "w"))}
+
+
+def adjoint(kind: int, lens: dict, src: dict, bins: Binning, gbar1, gbar2, *, precision: str = "f32",
+            device_inputs: bool = False, grads: Optional[dict] = None, ctx: Optional[Context] = None,
+            n_A: Optional[int] = None, n_B: Optional[int] = None) -> dict:
+    """Gradient of sum_b (gbar1_b g_t,b + gbar2_b g_x,b) (shear; xi_E / xi_B
+    for kappa) with respect to the lens values and weights and the source
+    components and weights: {"lens": {"val", "w"}, "src": {"e1", "e2", "w",
+    "m"} or {"kappa_E", "kappa_B", "w"}} as FP64 arrays.  With
+    ``device_inputs`` the inputs are device addresses and ``grads`` maps the
+    same keys to device addresses of FP64 arrays to fill (missing keys are not
+    computed); the return value is then empty."""
+    ctx = ctx or default_context()
+    L = N.lib()
+    dt = np.float32 if precision == "f32" else np.float64
+    keep = []
+
+    def ptr(v, dtype=dt):
+        if v is None:
+            return None
+        if device_inputs:
+            return C.c_void_p(int(v))
+        a = _host(v, dtype)
+        keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+    if not device_inputs:
+        n_A, n_B = len(lens["x"]), len(src["x"])
+    g1 = np.ascontiguousarray(gbar1, dtype=np.float64)
+    g2 = np.ascontiguousarray(gbar2, dtype=np.float64)
+    if len(g1) != bins.n_bin or len(g2) != bins.n_bin:
+        raise ValueError("upstream gradients must have n_bin entries")
+    lk, sk = _GRAD_KEYS[kind]
+    out = {"lens": {}, "src": {}}
+    gptr = []
+    for side, keys, n in (("lens", lk, n_A), ("src", sk, n_B)):
+        for k in keys:
+            if device_inputs:
+                addr = (grads or {}).get(side, {}).get(k)
+                gptr.append(C.c_void_p(int(addr)) if addr is not None else None)
+            else:
+                a = np.zeros(n, dtype=np.float64)
+                out[side][k] = a
+                gptr.append(a.ctypes.data_as(C.c_void_p))
+    flags = N.FLAG_DEVICE_INPUTS if device_inputs else 0
+    pc = lambda a: g1.ctypes.data_as(C.c_void_p) if a is g1 else g2.ctypes.data_as(C.c_void_p)  # noqa: E731
+    if kind == SHEAR:
+        fn = L.tpcf_adjoint_shear_f32 if precision == "f32" else L.tpcf_adjoint_shear_f64
+        rc = fn(ctx.handle, n_A, ptr(lens["x"]), ptr(lens["y"]), ptr(lens["z"]), ptr(lens["val"]), ptr(lens["w"]),
+                n_B, ptr(src["x"]), ptr(src["y"]), ptr(src["z"]), ptr(src["e1"]), ptr(src["e2"]), ptr(src["w"]),
+                ptr(src["m"]), C.byref(bins.c), flags, pc(g1), pc(g2), *gptr)
+    else:
+        fn = L.tpcf_adjoint_kappa_f32 if precision == "f32" else L.tpcf_adjoint_kappa_f64
+        rc = fn(ctx.handle, n_A, ptr(lens["x"]), ptr(lens["y"]), ptr(lens["val"]), ptr(lens["w"]),
+                n_B, ptr(src["x"]), ptr(src["y"]), ptr(src["kappa_E"]), ptr(src["kappa_B"]), ptr(src["w"]),
+                C.byref(bins.c), flags, pc(g1), pc(g2), *gptr)
+    if rc:
+        raise TpcfError(rc, ctx.last_error())
+    return out if not device_inputs else {}

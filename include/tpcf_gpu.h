@@ -181,6 +181,49 @@ int tpcf_kappa_f64(tpcf_ctx *ctx,
                    const int32_t *patch_A, const int32_t *patch_B,
                    int32_t n_patch_A, int32_t n_patch_B, tpcf_out *out);
 
+/* Reverse mode (no counterpart in the reference, which has no derivative):
+ * given the gradient of a scalar loss with respect to the per-bin outputs
+ * g_t, g_x of tpcf_shear_* (gbar_t, gbar_x: host arrays [n_bin]), the
+ * gradient with respect to each lens' val and w [n_A] and each source's e1,
+ * e2, w and m [n_B].  Inputs are those of the forward call; with
+ * TPCF_DEVICE_INPUTS the inputs and the gradient arrays are device pointers.
+ * Any gradient pointer may be NULL.  Lenses with w == 0 (skipped by the
+ * reference, src/tpcf.f90:285) receive the gradient of the smooth extension;
+ * objects masked by z < 0 receive zero.  Gradients are FP64. */
+int tpcf_adjoint_shear_f32(tpcf_ctx *ctx,
+                           int64_t n_A, const float *x_A, const float *y_A, const float *z_A,
+                           const float *val_A, const float *w_A,
+                           int64_t n_B, const float *x_B, const float *y_B, const float *z_B,
+                           const float *e1_B, const float *e2_B, const float *w_B, const float *m_B,
+                           const tpcf_bins *bins, uint32_t flags, const double *gbar_t, const double *gbar_x,
+                           double *val_A_bar, double *w_A_bar,
+                           double *e1_B_bar, double *e2_B_bar, double *w_B_bar, double *m_B_bar);
+int tpcf_adjoint_shear_f64(tpcf_ctx *ctx,
+                           int64_t n_A, const double *x_A, const double *y_A, const double *z_A,
+                           const double *val_A, const double *w_A,
+                           int64_t n_B, const double *x_B, const double *y_B, const double *z_B,
+                           const double *e1_B, const double *e2_B, const double *w_B, const double *m_B,
+                           const tpcf_bins *bins, uint32_t flags, const double *gbar_t, const double *gbar_x,
+                           double *val_A_bar, double *w_A_bar,
+                           double *e1_B_bar, double *e2_B_bar, double *w_B_bar, double *m_B_bar);
+/* The same for tpcf_kappa_*: gradients of xi_E (gbar_E) and xi_B (gbar_B)
+ * with respect to val, w of A and kappa_E, kappa_B, w of B. */
+int tpcf_adjoint_kappa_f32(tpcf_ctx *ctx,
+                           int64_t n_A, const float *x_A, const float *y_A, const float *val_A, const float *w_A,
+                           int64_t n_B, const float *x_B, const float *y_B, const float *kappa_E,
+                           const float *kappa_B, const float *w_B,
+                           const tpcf_bins *bins, uint32_t flags, const double *gbar_E, const double *gbar_B,
+                           double *val_A_bar, double *w_A_bar,
+                           double *kE_B_bar, double *kB_B_bar, double *w_B_bar);
+int tpcf_adjoint_kappa_f64(tpcf_ctx *ctx,
+                           int64_t n_A, const double *x_A, const double *y_A, const double *val_A,
+                           const double *w_A,
+                           int64_t n_B, const double *x_B, const double *y_B, const double *kappa_E,
+                           const double *kappa_B, const double *w_B,
+                           const tpcf_bins *bins, uint32_t flags, const double *gbar_E, const double *gbar_B,
+                           double *val_A_bar, double *w_A_bar,
+                           double *kE_B_bar, double *kB_B_bar, double *w_B_bar);
+
 #ifdef __cplusplus
 }
 #endif
