@@ -288,10 +288,7 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
         ppo = static_cast<double *>(vb);
         ppno = reinterpret_cast<long long *>(ppo + (size_t)kNQ * pp_total);
         dtot = reinterpret_cast<double *>(ppno + pp_total);
-        hipLaunchKernelGGL(k_reduce_pp, dim3((unsigned)((pp_total + 255) / 256)), dim3(256), 0, s, n_pa, n_pb, n_bin, pp,
-                           ppn, ppo, ppno);
-        hipLaunchKernelGGL(k_bin_totals, dim3(n_bin), dim3(256), 0, s, n_pp, n_bin, ppo, ppno, dtot);
-        TPCF_HIP(ctx, hipGetLastError());
+        TPCF_TRY(launch_reduce(ctx, s, n_pa, n_pb, n_bin, pp, ppn, ppo, ppno, dtot));
         TPCF_HIP(ctx, hipMemcpyAsync(tot.data(), dtot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, s));
         if (want_pp) {
             hpp.resize((size_t)kNQ * pp_total);
@@ -405,6 +402,17 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
         std::fflush(stdout);
     }
     if (pout) *pout = p;
+    return TPCF_OK;
+}
+
+int launch_reduce(tpcf_ctx *ctx, hipStream_t s, int32_t n_pa, int32_t n_pb, int32_t n_bin, const double *pp,
+                  const unsigned long long *ppn, double *ppo, long long *ppno, double *tot)
+{
+    const int64_t n_pp = (int64_t)n_pa * n_pb, pp_total = n_pp * n_bin;
+    hipLaunchKernelGGL(k_reduce_pp, dim3((unsigned)((pp_total + 255) / 256)), dim3(256), 0, s, n_pa, n_pb, n_bin, pp,
+                       ppn, ppo, ppno);
+    hipLaunchKernelGGL(k_bin_totals, dim3(n_bin), dim3(256), 0, s, n_pp, n_bin, ppo, ppno, tot);
+    TPCF_HIP(ctx, hipGetLastError());
     return TPCF_OK;
 }
 

@@ -261,10 +261,74 @@ struct Prepared {
     int64_t n_items;
     double margin;               // slack of the cell-range bounds
     bool m_zero;                 // shear: every source has m == 0 (then K == W)
+    // what refresh_values needs to re-read the values on the same structure
+    const uint32_t *sidx;        // sorted source -> original index
+    const uint32_t *counts;      // [key_space + 1] sources per key
+    const uint4 *blk;            // lens blocks (start, count, patch)
+    int64_t n_blk;
+    const uint32_t *choff;       // first chunk of each lens block
+    T dmax;                      // largest chunk box side
+    int32_t *err;                // device flags
 };
 
 template <typename T>
 int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p);
+
+// Re-reads the values (val, w, z, s1, s2, m) of the same catalogues into the
+// prepared structure on stream s: source records and segment sums, sorted
+// lens values and chunk sums.  Kernel launches only (graph-capturable).  The
+// positions and the masks (w_A == 0, z < 0) are those of the preparation.
+template <typename T>
+int refresh_values(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, hipStream_t s);
+
+// Arguments of the pair kernels shared by every catalogue (binning, layout).
+template <typename T>
+struct PairArgs {
+    int64_t n_items;
+    int64_t per_xcd;       // blocks per XCD of the remapped grid
+    int32_t n_bin, counters;
+    double x_max;          // outer edge (FP64)
+    int32_t xmin_zero;     // x_min == 0: pairs at zero separation fall into bin 0
+    T cls_lo, cls_hi;      // classification margins on squared box distances
+    float band;            // relative ambiguity band on d^2 (FP32 path)
+    float e2f[TPCF_MAX_BIN + 1];   // squared edges rounded to FP32 (+inf past n_bin)
+    double e2d[TPCF_MAX_BIN + 1];  // squared edges in FP64; e2d[n_bin] = next double above x_max^2
+};
+
+// One catalogue's prepared state, as the pair kernels read it.
+template <typename T>
+struct CatArgs {
+    Grid g;
+    int32_t n_pa, n_pb;
+    double margin;                  // absolute slack for cell-range bounds
+    const Chunk<T> *chunks;
+    const T *lx, *ly, *lz, *la, *lw;
+    const Seg<T> *segs;
+    const uint32_t *seg_first;
+    const int4 *pbox;
+    const SrcRec<T> *srec;
+    const T *sz;
+    double *pp;                     // [slot][P_A][P_B][n_bin][5]
+    unsigned long long *ppn;        // [slot][P_A][P_B][n_bin]
+    double *marks;                  // [n_lens][n_bin][4] or NULL
+};
+
+
+
+template <typename T>
+int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, PairArgs<T> &A, const T **e2lane_out);
+template <typename T>
+CatArgs<T> cat_args(const Prepared<T> &p, int32_t n_pa, int32_t n_pb, double *pp, unsigned long long *ppn,
+                    double *marks);
+template <typename T>
+int launch_pairs(tpcf_ctx *ctx, hipStream_t s, int kind, bool m_zero, bool marks, const PairArgs<T> &A,
+                 const CatArgs<T> &C, const T *e2lane, const Item *items, unsigned long long *probe);
+// Per-patch-pair and per-bin reduction of the accumulators (kernels on s):
+// ppo [5][P_B][P_A][n_bin], ppno [P_B][P_A][n_bin], tot [n_bin][6].
+int launch_reduce(tpcf_ctx *ctx, hipStream_t s, int32_t n_pa, int32_t n_pb, int32_t n_bin, const double *pp,
+                  const unsigned long long *ppn, double *ppo, long long *ppno, double *tot);
+void fill_f64(hipStream_t s, double *p, int64_t n);
+void fill_u64(hipStream_t s, unsigned long long *p, int64_t n);
 
 // The correlation call once the inputs are on the device (api.hip).
 template <typename T>

@@ -29,20 +29,6 @@ namespace tpcf {
 void fill_u64(hipStream_t s, unsigned long long *p, int64_t n);
 void fill_f64(hipStream_t s, double *p, int64_t n);
 
-template <typename T>
-struct PairArgs {
-    Grid g;
-    int64_t n_items;
-    int64_t per_xcd;       // blocks per XCD of the remapped grid
-    int32_t n_pa, n_pb, n_bin, counters;
-    double x_max;          // outer edge (FP64)
-    int32_t xmin_zero;     // x_min == 0: pairs at zero separation fall into bin 0
-    double margin;         // absolute slack for cell-range bounds
-    T cls_lo, cls_hi;      // classification margins on squared box distances
-    float band;            // relative ambiguity band on d^2 (FP32 path)
-    float e2f[TPCF_MAX_BIN + 1];   // squared edges rounded to FP32 (+inf past n_bin)
-    double e2d[TPCF_MAX_BIN + 1];  // squared edges in FP64; e2d[n_bin] = next double above x_max^2
-};
 
 template <typename T>
 struct Geo;
@@ -450,51 +436,32 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
     w.edge = true;
 }
 
-// Waves per SIMD the register budget is set for.  The pair loops are
-// latency-bound: on the 20M x 20M shear workload 5 waves take 6.73 s, 6 take
-// 5.92 s, 7 take 5.60 s and 8 take 5.35 s.  FP32 with all m == 0 (MZ) keeps
-// 18 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
-// few cold-path spills); otherwise 22.5 KB allows 7 (72 VGPRs).  FP64 keeps 4
-// waves (no spills).
-template <typename T, bool MZ>
-constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : 4; }
 
+
+// The work of one item: lens chunk item.chunk against the source segments in
+// its share of cell rows, into the per-XCD accumulator copy `slot`.
 template <typename T, int KIND, bool MARKS, bool MZ>
-__global__ __launch_bounds__(kWave *kWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pairs(
-    const PairArgs<T> A, const Chunk<T> *__restrict__ chunks, const T *__restrict__ lx, const T *__restrict__ ly,
-    const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
-    const uint32_t *__restrict__ seg_first, const int4 *__restrict__ pbox, const SrcRec<T> *__restrict__ srec,
-    const T *__restrict__ sz, const T *__restrict__ e2lane, const Item *__restrict__ items, double *__restrict__ pp,
-    unsigned long long *__restrict__ ppn, double *__restrict__ marks, unsigned long long *__restrict__ probe)
+__device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T> &C, const Item item, const int slot,
+                                          const int lane, double *acc, const T e2_lane,
+                                          unsigned long long *__restrict__ probe)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int64_t b = blockIdx.x;
-    const int slot = (int)(b % kSlots);
-    // consecutive items (neighbouring chunks) stay on one XCD and share its L2
-    const int64_t it = ((b % kSlots) * A.per_xcd + b / kSlots) * kWavesPerBlock + wv;
-    if (it >= A.n_items) return;
-    const Item item = items[it];
-    const Chunk<T> ch = chunks[item.chunk];
+    constexpr int WQ = win_q<MZ>();
+    const Chunk<T> ch = C.chunks[item.chunk];
     const int split = item.split, n_split = item.n_split;
     LaneData<T> L;
     L.lane = lane;
     L.valid = lane < (int)ch.count;
     L.li = ch.start + (L.valid ? lane : 0);
-    L.xa = lx[L.li];
-    L.ya = ly[L.li];
-    L.za = (KIND == SHEAR) ? lz[L.li] : (T)0;
-    L.wa = L.valid ? lw[L.li] : (T)0;
-    L.aa = L.valid ? la[L.li] : (T)0;
+    L.xa = C.lx[L.li];
+    L.ya = C.ly[L.li];
+    L.za = (KIND == SHEAR) ? C.lz[L.li] : (T)0;
+    L.wa = L.valid ? C.lw[L.li] : (T)0;
+    L.aa = L.valid ? C.la[L.li] : (T)0;
     L.vmask = L.valid ? (T)1 : (T)0;
     const int n_bin = A.n_bin;
-    const T e2_lane = e2lane[lane];   // squared edge `lane` (+inf past n_bin)
 
-    constexpr int WQ = win_q<MZ>();
-    __shared__ double s_acc[kWavesPerBlock][(2 * WQ + 1) * kWave];
     Win w;
-    w.acc = s_acc[wv];
+    w.acc = acc;
     win_clear<MZ>(w, lane);
     w.pb = -1;
     w.wk = -3;
@@ -503,12 +470,12 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
     for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
 
     uint64_t cseg0 = 0, cseg1 = 0, cseg2 = 0, cpair0 = 0, cpair1 = 0, cpair2 = 0;   // class counters
-    const double xm = A.x_max, mg = A.margin;
-    const Grid g = A.g;
+    const double xm = A.x_max, mg = C.margin;
+    const Grid g = C.g;
     int ry0, ry1;
     row_span(g, (double)ch.by0, (double)ch.by1, xm, mg, ry0, ry1);
-    for (int pb = 0; pb < A.n_pb; ++pb) {
-        const int4 pbx = pbox[pb];
+    for (int pb = 0; pb < C.n_pb; ++pb) {
+        const int4 pbx = C.pbox[pb];
         if (pbx.x > pbx.y) continue;   // no sources in this patch
         const int cy0 = max(ry0, pbx.z);
         const int cy1 = min(ry1, pbx.w);
@@ -526,12 +493,12 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
             cx1 = min(cx1, pbx.y);
             if (cx0 > cx1) continue;
             const int64_t key0 = (int64_t)pb * g.ncells + (int64_t)cy * g.nx + cx0;
-            const uint32_t s_begin = __builtin_amdgcn_readfirstlane(seg_first[key0]);
-            const uint32_t s_end = __builtin_amdgcn_readfirstlane(seg_first[key0 + (cx1 - cx0) + 1]);
+            const uint32_t s_begin = __builtin_amdgcn_readfirstlane(C.seg_first[key0]);
+            const uint32_t s_end = __builtin_amdgcn_readfirstlane(C.seg_first[key0 + (cx1 - cx0) + 1]);
             for (uint32_t si = s_begin; si < s_end; ++si) {
-                const Seg<T> sg = segs[si];
+                const Seg<T> sg = C.segs[si];
                 const uint32_t cnt = __builtin_amdgcn_readfirstlane(sg.count);
-                const SrcRec<T> *__restrict__ r = srec + __builtin_amdgcn_readfirstlane(sg.start);
+                const SrcRec<T> *__restrict__ r = C.srec + __builtin_amdgcn_readfirstlane(sg.start);
                 // squared distance range between the chunk box and the segment box
                 const T dxl = fmax((T)0, fmax(sg.x0 - ch.bx1, ch.bx0 - sg.x1));
                 const T dyl2 = fmax((T)0, fmax(sg.y0 - ch.by1, ch.by0 - sg.y1));
@@ -548,7 +515,7 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
                     cls = 0;
                     // ---- fast: the whole segment is in bin kmin ----
                     if (!(w.open && pb == w.pb && (kmin == w.wk || kmin == w.wk + 1)))
-                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, marks, pb, kmin);
+                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, kmin);
                     T sT = 0, sG1 = 0, sG2 = 0;
                     if (KIND == SHEAR) {
                         stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
@@ -600,7 +567,7 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
                 } else if (kmax == kmin + 1 && !zc) {
                     cls = 1;
                     // ---- edge: one bin edge inside the distance range ----
-                    win_open<T, KIND, MARKS, MZ>(w, L, n_bin, marks, pb, kmin);
+                    win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, kmin);
                     if (lo2 > (T)0) {
                         edge_pass<T, KIND, MZ, false, false>(w, L, A, r, sg, cnt, n_bin);
                     } else if (KIND == SHEAR && A.xmin_zero && kmin == 0) {
@@ -611,12 +578,12 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
                 } else {
                     cls = 2;
                     // ---- multi: two-bin windows over [kmin, kmax], bounded tests ----
-                    const T *zr = (KIND == SHEAR) ? sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
+                    const T *zr = (KIND == SHEAR) ? C.sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
                     for (int w0 = kmin; w0 <= kmax; w0 += 2) {
-                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, marks, pb, w0);
+                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, w0);
                         if (zc) win_pass<T, KIND, MZ, true>(w, L, A, r, zr, cnt, n_bin);
                         else win_pass<T, KIND, MZ, false>(w, L, A, r, zr, cnt, n_bin);
-                        win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, marks);
+                        win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
                     }
                 }
                 if (A.counters) {
@@ -626,16 +593,16 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
                 }
             }
         }
-        win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, marks);
+        win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
         // hand the patch's per-bin sums to the patch-pair accumulators
         if (lane < n_bin && w.t[QN] > 0.0) {
-            const size_t o = (((size_t)slot * A.n_pa + ch.pa) * A.n_pb + pb) * n_bin + lane;
-            unsafeAtomicAdd(&pp[o * kNQ + 0], w.t[QT]);
-            unsafeAtomicAdd(&pp[o * kNQ + 1], w.t[QG1]);
-            unsafeAtomicAdd(&pp[o * kNQ + 2], w.t[QG2]);
-            unsafeAtomicAdd(&pp[o * kNQ + 3], w.t[QW]);
-            unsafeAtomicAdd(&pp[o * kNQ + 4], w.t[QK]);
-            atomicAdd(&ppn[o], (unsigned long long)w.t[QN]);
+            const size_t o = (((size_t)slot * C.n_pa + ch.pa) * C.n_pb + pb) * n_bin + lane;
+            unsafeAtomicAdd(&C.pp[o * kNQ + 0], w.t[QT]);
+            unsafeAtomicAdd(&C.pp[o * kNQ + 1], w.t[QG1]);
+            unsafeAtomicAdd(&C.pp[o * kNQ + 2], w.t[QG2]);
+            unsafeAtomicAdd(&C.pp[o * kNQ + 3], w.t[QW]);
+            unsafeAtomicAdd(&C.pp[o * kNQ + 4], w.t[QK]);
+            atomicAdd(&C.ppn[o], (unsigned long long)w.t[QN]);
         }
         for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
     }
@@ -649,24 +616,63 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
     }
 }
 
-template <typename T>
-int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double *pp, unsigned long long *ppn,
-              double *marks)
+// Waves per SIMD the register budget is set for.  The pair loops are
+// latency-bound: on the 20M x 20M shear workload 5 waves take 6.73 s, 6 take
+// 5.92 s, 7 take 5.60 s and 8 take 5.35 s.  FP32 with all m == 0 (MZ) keeps
+// 18 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
+// few cold-path spills); otherwise 22.5 KB allows 7 (72 VGPRs).  FP64 keeps 4
+// waves (no spills).
+template <typename T, bool MZ>
+constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : 4; }
+
+// One catalogue: one item per wave, items dealt so that consecutive items
+// (neighbouring chunks) run on one XCD and share its L2.  The catalogue's
+// pointers arrive as separate arguments (scalar loads on demand).
+template <typename T, int KIND, bool MARKS, bool MZ>
+__global__ __launch_bounds__(kWave *kWavesPerBlock)
+__attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pairs(
+    const PairArgs<T> A, const Grid g, const int32_t n_pa, const int32_t n_pb, const double margin,
+    const Chunk<T> *__restrict__ chunks, const T *__restrict__ lx, const T *__restrict__ ly,
+    const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
+    const uint32_t *__restrict__ seg_first, const int4 *__restrict__ pbox, const SrcRec<T> *__restrict__ srec,
+    const T *__restrict__ sz, const T *__restrict__ e2lane, const Item *__restrict__ items, double *__restrict__ pp,
+    unsigned long long *__restrict__ ppn, double *__restrict__ marks, unsigned long long *__restrict__ probe)
 {
-    PairArgs<T> A;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t b = blockIdx.x;
+    const int slot = (int)(b % kSlots);
+    const int64_t it = ((b % kSlots) * A.per_xcd + b / kSlots) * kWavesPerBlock + wv;
+    if (it >= A.n_items) return;
+    constexpr int WQ = win_q<MZ>();
+    __shared__ double s_acc[kWavesPerBlock][(2 * WQ + 1) * kWave];
+    CatArgs<T> C;
+    C.g = g;
+    C.n_pa = n_pa;
+    C.n_pb = n_pb;
+    C.margin = margin;
+    C.chunks = chunks;
+    C.lx = lx; C.ly = ly; C.lz = lz; C.la = la; C.lw = lw;
+    C.segs = segs;
+    C.seg_first = seg_first;
+    C.pbox = pbox;
+    C.srec = srec;
+    C.sz = sz;
+    C.pp = pp;
+    C.ppn = ppn;
+    C.marks = marks;
+    pair_item<T, KIND, MARKS, MZ>(A, C, items[it], slot, lane, s_acc[wv], e2lane[lane], probe);
+}
+
+template <typename T>
+int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, PairArgs<T> &A, const T **e2lane_out)
+{
     std::memset(&A, 0, sizeof A);
-    A.g = p.grid;
-    A.n_items = p.n_items;
-    const int64_t nblk = (p.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
-    A.per_xcd = (nblk + kSlots - 1) / kSlots;
-    A.n_pa = a.n_pa;
-    A.n_pb = a.n_pb;
-    A.n_bin = a.bins->n_bin;
+    A.n_bin = bins->n_bin;
     A.counters = ctx->opt_counters ? 1 : 0;
     const int n = A.n_bin;
-    A.x_max = a.bins->edges[n];
-    A.xmin_zero = (a.bins->edges[0] == 0.0) ? 1 : 0;
-    A.margin = p.margin;
+    A.x_max = bins->edges[n];
+    A.xmin_zero = (bins->edges[0] == 0.0) ? 1 : 0;
     if (sizeof(T) == 4) {
         A.cls_lo = (T)(1.0 - 2e-6);
         A.cls_hi = (T)(1.0 + 2e-6);
@@ -677,7 +683,7 @@ int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double 
     A.band = 1e-5f;
     for (int i = 0; i <= TPCF_MAX_BIN; ++i) {
         if (i <= n) {
-            const double e = a.bins->edges[i];
+            const double e = bins->edges[i];
             A.e2d[i] = e * e;
             A.e2f[i] = (float)(e * e);
         } else {
@@ -686,41 +692,98 @@ int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double 
         }
     }
     A.e2d[n] = std::nextafter(A.e2d[n], INFINITY);   // d == x_max belongs to the last bin
-
     T *e2lane;
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_E2, sizeof(T) * kWave, (void **)&e2lane));
-    {
-        T h[kWave];
-        for (int i = 0; i < kWave; ++i) h[i] = (i <= n) ? (sizeof(T) == 4 ? (T)A.e2f[i] : (T)A.e2d[i]) : (T)INFINITY;
-        TPCF_HIP(ctx, hipMemcpyAsync(e2lane, h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
-        TPCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    }
+    T h[kWave];
+    for (int i = 0; i < kWave; ++i) h[i] = (i <= n) ? (sizeof(T) == 4 ? (T)A.e2f[i] : (T)A.e2d[i]) : (T)INFINITY;
+    TPCF_HIP(ctx, hipMemcpyAsync(e2lane, h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+    TPCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *e2lane_out = e2lane;
+    return TPCF_OK;
+}
+
+template <typename T>
+CatArgs<T> cat_args(const Prepared<T> &p, int32_t n_pa, int32_t n_pb, double *pp, unsigned long long *ppn,
+                    double *marks)
+{
+    CatArgs<T> C;
+    std::memset(&C, 0, sizeof C);
+    C.g = p.grid;
+    C.n_pa = n_pa;
+    C.n_pb = n_pb;
+    C.margin = p.margin;
+    C.chunks = p.chunks;
+    C.lx = p.lx; C.ly = p.ly; C.lz = p.lz; C.la = p.la; C.lw = p.lw;
+    C.segs = p.segs;
+    C.seg_first = p.seg_first;
+    C.pbox = p.pbox;
+    C.srec = p.srec;
+    C.sz = p.sz;
+    C.pp = pp;
+    C.ppn = ppn;
+    C.marks = marks;
+    return C;
+}
+
+template <typename T>
+int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double *pp, unsigned long long *ppn,
+              double *marks)
+{
+    PairArgs<T> A;
+    const T *e2lane;
+    TPCF_TRY(make_pair_args<T>(ctx, a.bins, A, &e2lane));
+    A.n_items = p.n_items;
+    const int64_t nblk = (p.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
+    A.per_xcd = (nblk + kSlots - 1) / kSlots;
+    const CatArgs<T> C = cat_args<T>(p, a.n_pa, a.n_pb, pp, ppn, marks);
     unsigned long long *probe = nullptr;
     if (A.counters) {
         TPCF_TRY(ensure(ctx, tpcf_ctx::B_PROBE, sizeof(unsigned long long) * 8, (void **)&probe));
         fill_u64(ctx->stream, probe, 8);
     }
     if (p.n_items == 0) return TPCF_OK;
-    const dim3 grid((unsigned)(A.per_xcd * kSlots)), block(kWave * kWavesPerBlock);
     TPCF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-#define TPCF_LAUNCH(K, M, Z)                                                                                      \
-    hipLaunchKernelGGL((k_pairs<T, K, M, Z>), grid, block, 0, ctx->stream, A, p.chunks, p.lx, p.ly, p.lz, p.la,   \
-                       p.lw, p.segs, p.seg_first, p.pbox, p.srec, p.sz, e2lane, p.items, pp, ppn, marks, probe)
-    if (a.kind == SHEAR) {
-        if (p.m_zero) {
-            if (a.marks) TPCF_LAUNCH(SHEAR, true, true); else TPCF_LAUNCH(SHEAR, false, true);
-        } else {
-            if (a.marks) TPCF_LAUNCH(SHEAR, true, false); else TPCF_LAUNCH(SHEAR, false, false);
-        }
-    } else {
-        if (a.marks) TPCF_LAUNCH(KAPPA, true, true); else TPCF_LAUNCH(KAPPA, false, true);
-    }
-#undef TPCF_LAUNCH
-    TPCF_HIP(ctx, hipGetLastError());
+    TPCF_TRY(launch_pairs<T>(ctx, ctx->stream, a.kind, p.m_zero, a.marks, A, C, e2lane, p.items, probe));
     TPCF_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     return TPCF_OK;
 }
 
+// The pair kernel for one catalogue's items: only a kernel launch on `s`
+// (graph-capturable).  A.n_items and A.per_xcd describe the items.
+template <typename T>
+int launch_pairs(tpcf_ctx *ctx, hipStream_t s, int kind, bool m_zero, bool marks, const PairArgs<T> &A,
+                 const CatArgs<T> &C, const T *e2lane, const Item *items, unsigned long long *probe)
+{
+    if (A.n_items == 0) return TPCF_OK;
+    const dim3 grid((unsigned)(A.per_xcd * kSlots)), block(kWave * kWavesPerBlock);
+#define TPCF_LAUNCH(K, M, Z)                                                                                    \
+    hipLaunchKernelGGL((k_pairs<T, K, M, Z>), grid, block, 0, s, A, C.g, C.n_pa, C.n_pb, C.margin, C.chunks,      \
+                       C.lx, C.ly, C.lz, C.la, C.lw, C.segs, C.seg_first, C.pbox, C.srec, C.sz, e2lane, items, C.pp, \
+                       C.ppn, C.marks, probe)
+    if (kind == SHEAR) {
+        if (m_zero) {
+            if (marks) TPCF_LAUNCH(SHEAR, true, true); else TPCF_LAUNCH(SHEAR, false, true);
+        } else {
+            if (marks) TPCF_LAUNCH(SHEAR, true, false); else TPCF_LAUNCH(SHEAR, false, false);
+        }
+    } else {
+        if (marks) TPCF_LAUNCH(KAPPA, true, true); else TPCF_LAUNCH(KAPPA, false, true);
+    }
+#undef TPCF_LAUNCH
+    TPCF_HIP(ctx, hipGetLastError());
+    return TPCF_OK;
+}
+
+template int make_pair_args<float>(tpcf_ctx *, const tpcf_bins *, PairArgs<float> &, const float **);
+template int make_pair_args<double>(tpcf_ctx *, const tpcf_bins *, PairArgs<double> &, const double **);
+template CatArgs<float> cat_args<float>(const Prepared<float> &, int32_t, int32_t, double *, unsigned long long *,
+                                        double *);
+template CatArgs<double> cat_args<double>(const Prepared<double> &, int32_t, int32_t, double *, unsigned long long *,
+                                          double *);
+template int launch_pairs<float>(tpcf_ctx *, hipStream_t, int, bool, bool, const PairArgs<float> &,
+                                 const CatArgs<float> &, const float *, const Item *, unsigned long long *);
+template int launch_pairs<double>(tpcf_ctx *, hipStream_t, int, bool, bool, const PairArgs<double> &,
+                                  const CatArgs<double> &, const double *, const Item *, unsigned long long *);
 template int run_pairs<float>(tpcf_ctx *, const CallArgs<float> &, const Prepared<float> &, double *,
                               unsigned long long *, double *);
 template int run_pairs<double>(tpcf_ctx *, const CallArgs<double> &, const Prepared<double> &, double *,

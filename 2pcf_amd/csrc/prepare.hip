@@ -547,12 +547,46 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
         p.items = nullptr;
     }
     p.cstart = cstart;
+    p.sidx = sidx2;
+    p.counts = counts;
+    p.blk = dblk;
+    p.n_blk = nblk;
+    p.choff = dnch + nblk;
+    p.dmax = dmax;
+    p.err = err;
     TPCF_HIP(ctx, hipGetLastError());
     p.chunks = chunks;
     p.lx = lx; p.ly = ly; p.lz = lz; p.la = la; p.lw = lw;
     p.lidx = lidx2;
     return TPCF_OK;
 }
+
+template <typename T>
+int refresh_values(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, hipStream_t s)
+{
+    const bool shear = (a.kind == SHEAR);
+    if (p.n_src > 0)
+        hipLaunchKernelGGL(k_src_gather<T>, dim3(blocks_for(p.n_src)), dim3(kBlk), 0, s, p.n_src, a.kind, p.sidx, a.xb,
+                           a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb, const_cast<SrcRec<T> *>(p.srec),
+                           const_cast<T *>(p.sz), p.err, (T)a.zsign);
+    launch_fill<int4>(s, const_cast<int4 *>(p.pbox), make_int4(INT_MAX, -1, INT_MAX, -1), a.n_pb);
+    hipLaunchKernelGGL(k_segs<T>, dim3(blocks_for(p.key_space)), dim3(kBlk), 0, s, p.key_space, p.grid, p.counts,
+                       p.cstart, p.seg_first, p.srec, p.sz, const_cast<Seg<T> *>(p.segs), const_cast<int4 *>(p.pbox));
+    if (p.n_lens > 0) {
+        hipLaunchKernelGGL(k_lens_gather<T>, dim3(blocks_for(p.n_lens)), dim3(kBlk), 0, s, p.n_lens, p.lidx, a.xa,
+                           a.ya, shear ? a.za : nullptr, a.vala, a.wa, const_cast<T *>(p.lx), const_cast<T *>(p.ly),
+                           const_cast<T *>(p.lz), const_cast<T *>(p.la), const_cast<T *>(p.lw), (T)a.zsign);
+        if (p.n_chunks > 0)
+            hipLaunchKernelGGL((k_chunk_blocks<T, true>), dim3(blocks_for(p.n_blk)), dim3(kBlk), 0, s, p.n_blk, p.blk,
+                               p.dmax, p.lx, p.ly, p.lz, p.lw, p.la, nullptr, p.choff,
+                               const_cast<Chunk<T> *>(p.chunks));
+    }
+    TPCF_HIP(ctx, hipGetLastError());
+    return TPCF_OK;
+}
+
+template int refresh_values<float>(tpcf_ctx *, const CallArgs<float> &, const Prepared<float> &, hipStream_t);
+template int refresh_values<double>(tpcf_ctx *, const CallArgs<double> &, const Prepared<double> &, hipStream_t);
 
 template int prepare<float>(tpcf_ctx *, const CallArgs<float> &, Prepared<float> &);
 template int prepare<double>(tpcf_ctx *, const CallArgs<double> &, Prepared<double> &);
