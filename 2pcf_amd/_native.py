@@ -27,6 +27,8 @@ FLAG_DEVICE_INPUTS = 1 << 0
 FLAG_SPHERICAL = 1 << 1
 FLAG_VERBOSE = 1 << 2
 FLAG_NO_KD_TREE = 1 << 3
+FLAG_BATCH_AUTOTUNE = 1 << 4
+FLAG_BATCH_NO_GRAPH = 1 << 5
 
 OPT_CELL_SIZE, OPT_COUNTERS, OPT_CELL_TARGET, OPT_SPLIT = 1, 2, 3, 4
 
@@ -36,6 +38,7 @@ EXPORTS = (
     "tpcf_set_stream", "tpcf_get_stats", "tpcf_last_error",
     "tpcf_shear_f32", "tpcf_kappa_f32", "tpcf_shear_f64", "tpcf_kappa_f64",
     "tpcf_adjoint_shear_f32", "tpcf_adjoint_shear_f64", "tpcf_adjoint_kappa_f32", "tpcf_adjoint_kappa_f64",
+    "tpcf_batch_create_f32", "tpcf_batch_create_f64", "tpcf_batch_run", "tpcf_batch_get_stats", "tpcf_batch_destroy",
 )
 
 
@@ -81,6 +84,32 @@ class Stats(C.Structure):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["seg_visits"] = list(self.seg_visits)
         d["pair_visits"] = list(self.pair_visits)
+        return d
+
+
+class Catalog(C.Structure):
+    """``tpcf_catalog``: one catalogue of a batch (device pointers)."""
+    _fields_ = [
+        ("kind", C.c_int32), ("n_patch_A", C.c_int32), ("n_patch_B", C.c_int32),
+        ("n_A", C.c_int64), ("n_B", C.c_int64),
+        ("x_A", C.c_void_p), ("y_A", C.c_void_p), ("z_A", C.c_void_p), ("val_A", C.c_void_p), ("w_A", C.c_void_p),
+        ("x_B", C.c_void_p), ("y_B", C.c_void_p), ("z_B", C.c_void_p), ("s1_B", C.c_void_p), ("s2_B", C.c_void_p),
+        ("w_B", C.c_void_p), ("m_B", C.c_void_p),
+        ("patch_A", C.c_void_p), ("patch_B", C.c_void_p),
+        ("out", C.POINTER(Out)),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("ms_plan", C.c_double), ("ms_run", C.c_double), ("cell_target", C.c_double), ("split", C.c_double),
+        ("tune_ms", C.c_double * 5), ("n_pairs", C.c_int64), ("n_items", C.c_int64), ("graph_nodes", C.c_int64),
+        ("runs", C.c_int64), ("n_catalogs", C.c_int32), ("autotuned", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["tune_ms"] = list(self.tune_ms)
         return d
 
 
@@ -130,5 +159,15 @@ def lib():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int
+    for name in ("tpcf_batch_create_f32", "tpcf_batch_create_f64"):
+        f = getattr(L, name)
+        f.argtypes = [_vp, _i32, C.POINTER(Catalog), C.POINTER(Bins), C.c_uint32, C.POINTER(_vp)]
+        f.restype = C.c_int
+    L.tpcf_batch_run.argtypes = [_vp]
+    L.tpcf_batch_run.restype = C.c_int
+    L.tpcf_batch_get_stats.argtypes = [_vp, C.POINTER(BatchStats)]
+    L.tpcf_batch_get_stats.restype = C.c_int
+    L.tpcf_batch_destroy.argtypes = [_vp]
+    L.tpcf_batch_destroy.restype = C.c_int
     _lib = L
     return L

@@ -387,3 +387,88 @@ def adjoint(kind: int, lens: dict, src: dict, bins: Binning, gbar1, gbar2, *, pr
     if rc:
         raise TpcfError(rc, ctx.last_error())
     return out if not device_inputs else {}
+
+
+# ---------------------------------------------------------------- batch ----
+class Batch:
+    """Many catalogues run as one HIP graph (tpcf_batch_*).
+
+    ``catalogs``: list of dicts with ``kind`` (SHEAR / KAPPA), ``lens`` and
+    ``src`` (device addresses of arrays of the batch precision: lens x, y,
+    (z,) val, w; source x, y, (z,) e1/e2 or kappa_E/kappa_B, w, (m)), sizes
+    ``n_A``, ``n_B`` and optional ``patch_A``, ``patch_B`` (device int32
+    addresses) with ``n_patch_A``, ``n_patch_B``.  The arrays must stay valid
+    while the batch exists; ``run()`` re-reads their values each time (the
+    positions, w == 0 lenses and z < 0 masks are fixed at creation)."""
+
+    def __init__(self, catalogs: list, bins: Binning, *, precision: str = "f32", autotune: bool = False,
+                 graph: bool = True, patches: bool = False, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.bins = bins
+        self._h = C.c_void_p()
+        n = len(catalogs)
+        self._cats = (N.Catalog * max(n, 1))()
+        self._outs = []
+        nb = bins.n_bin
+        for i, cdesc in enumerate(catalogs):
+            kind = cdesc["kind"]
+            L, S = cdesc["lens"], cdesc["src"]
+            s1, s2 = ("e1", "e2") if kind == SHEAR else ("kappa_E", "kappa_B")
+            c = self._cats[i]
+            c.kind = kind
+            c.n_A, c.n_B = int(cdesc["n_A"]), int(cdesc["n_B"])
+            c.x_A, c.y_A, c.val_A, c.w_A = L["x"], L["y"], L["val"], L["w"]
+            c.z_A = L.get("z") if kind == SHEAR else None
+            c.x_B, c.y_B, c.s1_B, c.s2_B, c.w_B = S["x"], S["y"], S[s1], S[s2], S["w"]
+            c.z_B = S.get("z") if kind == SHEAR else None
+            c.m_B = S.get("m") if kind == SHEAR else None
+            c.patch_A, c.patch_B = cdesc.get("patch_A"), cdesc.get("patch_B")
+            c.n_patch_A, c.n_patch_B = int(cdesc.get("n_patch_A", 1)), int(cdesc.get("n_patch_B", 1))
+            res = {k: np.zeros(nb, dtype=np.float64) for k in ("theta", "mean_theta", "g_t", "g_x", "weights",
+                                                                "sum_theta", "sum_gt", "sum_gx", "sum_w", "sum_k")}
+            res["n_pairs"] = np.zeros(nb, dtype=np.int64)
+            o = N.Out()
+            o.real_bytes = 8
+            for k, v in res.items():
+                setattr(o, k, v.ctypes.data_as(C.c_void_p))
+            if patches:
+                pa = c.n_patch_A if c.patch_A else 1
+                pb = c.n_patch_B if c.patch_B else 1
+                res["pp_n"] = np.zeros((pb, pa, nb), dtype=np.int64)
+                o.pp_n = res["pp_n"].ctypes.data_as(C.c_void_p)
+                for k in ("pp_theta", "pp_gt", "pp_gx", "pp_w", "pp_k"):
+                    res[k] = np.zeros((pb, pa, nb), dtype=np.float64)
+                    setattr(o, k, res[k].ctypes.data_as(C.c_void_p))
+            self._outs.append((o, res))
+            c.out = C.pointer(o)
+        flags = N.FLAG_DEVICE_INPUTS | (N.FLAG_BATCH_AUTOTUNE if autotune else 0) | \
+            (0 if graph else N.FLAG_BATCH_NO_GRAPH)
+        fn = N.lib().tpcf_batch_create_f32 if precision == "f32" else N.lib().tpcf_batch_create_f64
+        rc = fn(self.ctx.handle, n, self._cats, C.byref(bins.c), flags, C.byref(self._h))
+        if rc:
+            raise TpcfError(rc, self.ctx.last_error())
+
+    def run(self) -> list:
+        """One run of every catalogue: a list of result dicts (per-bin
+        outputs, raw sums, per-patch-pair sums when asked for)."""
+        rc = N.lib().tpcf_batch_run(self._h)
+        if rc:
+            raise TpcfError(rc, self.ctx.last_error())
+        return [{k: v.copy() for k, v in res.items()} for _, res in self._outs]
+
+    @property
+    def stats(self) -> dict:
+        st = N.BatchStats()
+        N.lib().tpcf_batch_get_stats(self._h, C.byref(st))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().tpcf_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -55,8 +55,13 @@ enum tpcf_flags {
     TPCF_DEVICE_INPUTS = 1u << 0,  /* input arrays (and patch labels) are device pointers */
     TPCF_SPHERICAL = 1u << 1,      /* spherical_coords = .true. (not implemented: TPCF_ERR_UNSUPPORTED) */
     TPCF_VERBOSE = 1u << 2,        /* verbose = .true.: print a summary to stdout */
-    TPCF_NO_KD_TREE = 1u << 3      /* use_kd_tree = .false.: accepted; the pair set is identical */
+    TPCF_NO_KD_TREE = 1u << 3,     /* use_kd_tree = .false.: accepted; the pair set is identical */
+    TPCF_BATCH_AUTOTUNE = 1u << 4, /* batch: time candidate cell / work-item sizes first */
+    TPCF_BATCH_NO_GRAPH = 1u << 5  /* batch: run eagerly instead of capturing a HIP graph */
 };
+
+/* Catalogue kinds. */
+enum tpcf_kind { TPCF_SHEAR = 0, TPCF_KAPPA = 1 };
 
 /* type(binning), src/utils.f90:18-24, plus the FP64 edges used for binning. */
 typedef struct tpcf_bins {
@@ -223,6 +228,51 @@ int tpcf_adjoint_kappa_f64(tpcf_ctx *ctx,
                            const tpcf_bins *bins, uint32_t flags, const double *gbar_E, const double *gbar_B,
                            double *val_A_bar, double *w_A_bar,
                            double *kE_B_bar, double *kB_B_bar, double *w_B_bar);
+
+/* ---- many catalogues (mock fields) as one HIP graph ----
+ * No counterpart in the reference beyond its field loop (src/main.f90:596-660,
+ * one shear_2pcf / kappa_2pcf call per field, results stacked as in
+ * src/main.f90:678-695).  tpcf_batch_create_* prepares every catalogue once
+ * (cells, segments, chunks: positions and masks w_A == 0, z < 0) and, with
+ * TPCF_BATCH_AUTOTUNE, first times candidate cell sizes and work-item sizes on
+ * the largest catalogues.  Each tpcf_batch_run re-reads the values (val, w,
+ * z, s1, s2, m) from the same device arrays, runs every catalogue and writes
+ * each catalogue's outputs (per-bin outputs, raw sums, per-patch-pair sums;
+ * marks are not supported).  The first run captures the work into one HIP
+ * graph (catalogues on parallel branches); later runs replay it.  Device
+ * arrays only (TPCF_DEVICE_INPUTS); they must stay valid while the batch
+ * exists.  m_B may be NULL (m == 0). */
+typedef struct tpcf_catalog {
+    int32_t kind;                  /* enum tpcf_kind */
+    int32_t n_patch_A, n_patch_B;
+    int64_t n_A, n_B;
+    const void *x_A, *y_A, *z_A, *val_A, *w_A;          /* z_A: shear only */
+    const void *x_B, *y_B, *z_B, *s1_B, *s2_B, *w_B, *m_B;   /* s1, s2: e1, e2 or kappa_E, kappa_B */
+    const int32_t *patch_A, *patch_B;                   /* nullable */
+    tpcf_out *out;                 /* host outputs of this catalogue (marks pointers must be NULL) */
+} tpcf_catalog;
+
+typedef struct tpcf_batch_stats {
+    double ms_plan;            /* preparation, autotuning included */
+    double ms_run;             /* last run (host wall clock, graph launch to outputs written) */
+    double cell_target, split; /* the settings used (TPCF_OPT_CELL_TARGET, TPCF_OPT_SPLIT) */
+    double tune_ms[5];         /* autotuning: time of each candidate on the sample */
+    int64_t n_pairs;           /* last run, all catalogues */
+    int64_t n_items;           /* pair-kernel work items, all catalogues */
+    int64_t graph_nodes;       /* nodes of the captured graph (0 before the first run or without a graph) */
+    int64_t runs;
+    int32_t n_catalogs, autotuned;
+} tpcf_batch_stats;
+
+typedef struct tpcf_batch tpcf_batch;
+
+int tpcf_batch_create_f32(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const tpcf_bins *bins,
+                          uint32_t flags, tpcf_batch **batch);
+int tpcf_batch_create_f64(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const tpcf_bins *bins,
+                          uint32_t flags, tpcf_batch **batch);
+int tpcf_batch_run(tpcf_batch *batch);
+int tpcf_batch_get_stats(const tpcf_batch *batch, tpcf_batch_stats *stats);
+int tpcf_batch_destroy(tpcf_batch *batch);
 
 #ifdef __cplusplus
 }
