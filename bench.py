@@ -103,6 +103,154 @@ def profiled_traffic(workload: str):
     return None, None
 
 
+def bench_mocks(args) -> int:
+    """Batched form: 256 mock fields, each a kappa catalogue and a shear
+    catalogue with sizes log-uniform in [5e4, 2e6] on a 600'x600' patch,
+    kappa-kappa (auto) and kappa x gamma_t per field, 20 log bins 1'-100',
+    16 patches (4 x 4 squares) per field; one tpcf_batch of 512 catalogues
+    (graph captured on the first run, replayed after).  value = in-bin pairs
+    per second of one batch run."""
+    import torch
+    tp = importlib.import_module("2pcf_amd")
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(1004)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1004)
+    side, n_f = 600.0, int(256 * args.scale) or 1
+    keep, descs = [], []
+
+    def uni(n, lo, hi):
+        return torch.rand(n, generator=gen, device=dev) * (hi - lo) + lo
+
+    t0 = time.perf_counter()
+    for f in range(n_f):
+        nk, ng = (int(np.exp(rng.uniform(np.log(5e4), np.log(2e6)))) for _ in range(2))
+        kx, ky = uni(nk, 0, side), uni(nk, 0, side)
+        kap = torch.randn(nk, generator=gen, device=dev) * 0.02
+        kw = uni(nk, 0.5, 1.5)
+        kz = torch.zeros(nk, device=dev)
+        gx, gy = uni(ng, 0, side), uni(ng, 0, side)
+        e1 = torch.randn(ng, generator=gen, device=dev) * 0.27
+        e2 = torch.randn(ng, generator=gen, device=dev) * 0.27
+        gw = uni(ng, 0.5, 1.5)
+        gz = torch.ones(ng, device=dev)
+        kb = torch.randn(nk, generator=gen, device=dev) * 0.02
+        pk = (torch.clamp((kx / 150).int(), 0, 3) + 4 * torch.clamp((ky / 150).int(), 0, 3)).int()
+        pg = (torch.clamp((gx / 150).int(), 0, 3) + 4 * torch.clamp((gy / 150).int(), 0, 3)).int()
+        keep += [kx, ky, kap, kw, kz, gx, gy, e1, e2, gw, gz, kb, pk, pg]
+        lens = dict(x=kx.data_ptr(), y=ky.data_ptr(), z=kz.data_ptr(), val=kap.data_ptr(), w=kw.data_ptr())
+        descs.append(dict(kind=tp.KAPPA, lens=lens, n_A=nk, n_B=nk, patch_A=pk.data_ptr(), patch_B=pk.data_ptr(),
+                          n_patch_A=16, n_patch_B=16,
+                          src=dict(x=kx.data_ptr(), y=ky.data_ptr(), kappa_E=kap.data_ptr(), kappa_B=kb.data_ptr(),
+                                   w=kw.data_ptr())))
+        descs.append(dict(kind=tp.SHEAR, lens=lens, n_A=nk, n_B=ng, patch_A=pk.data_ptr(), patch_B=pg.data_ptr(),
+                          n_patch_A=16, n_patch_B=16,
+                          src=dict(x=gx.data_ptr(), y=gy.data_ptr(), z=gz.data_ptr(), e1=e1.data_ptr(),
+                                   e2=e2.data_ptr(), w=gw.data_ptr())))
+    torch.cuda.synchronize()
+    log(f"{len(descs)} catalogues generated on the device in {time.perf_counter() - t0:.1f} s")
+    bins = tp.create_bins(1.0, 100.0, 20, tp.LOG)
+    ctx = tp.Context(0)
+    t0 = time.perf_counter()
+    b = tp.Batch(descs, bins, autotune=not args.no_autotune, ctx=ctx)
+    st = b.stats
+    log(f"batch planned in {time.perf_counter() - t0:.1f} s (autotune {st['autotuned']}: cell target "
+        f"{st['cell_target']}, split {st['split']}, candidates {[round(t, 1) for t in st['tune_ms']]} ms)")
+    for i in range(args.warmup):
+        out = b.run()
+        log(f"warmup {i}: {b.stats['ms_run']:.1f} ms ({b.stats['graph_nodes']} graph nodes)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pairs = 0
+    for i in range(args.steps):
+        out = b.run()
+        pairs += int(sum(int(r["n_pairs"].sum()) for r in out))
+        log(f"step {i}: {b.stats['ms_run']:.1f} ms")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    res = {"metric": "pairs/sec, batch of mock catalogues on 1 MI355X", "value": pairs / el, "unit": "pairs/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "mocks_256", "fields": n_f, "catalogues": len(descs),
+                      "pairs_per_step": pairs / args.steps, "cell_target": st["cell_target"], "split": st["split"],
+                      "graph_nodes": b.stats["graph_nodes"], "plan_ms": st["ms_plan"]}}
+    print(json.dumps(res), flush=True)
+    b.close()
+    ctx.close()
+    return 0
+
+
+def bench_adjoint(args) -> int:
+    """Differentiable form: 5M kappa points x 5M sources uniform on a
+    1200'x1200' patch, 20 log bins 1'-200': forward xi_kk and gamma_t plus the
+    adjoint of chi^2 (diagonal C) with respect to every kappa, e1, e2 and w.
+    value = in-bin pairs of the two forward correlations per second of
+    forward + backward."""
+    import torch
+    tp = importlib.import_module("2pcf_amd")
+    ag = importlib.import_module("2pcf_amd.autograd")
+    dev = torch.device("cuda:0")
+    n = int(5_000_000 * args.scale)
+    side = 1200.0 * np.sqrt(args.scale)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1005)
+
+    def uni(k, lo, hi):
+        return torch.rand(k, generator=gen, device=dev) * (hi - lo) + lo
+
+    kx, ky = uni(n, 0, side), uni(n, 0, side)
+    kap = (torch.randn(n, generator=gen, device=dev) * 0.02).requires_grad_()
+    kw = uni(n, 0.5, 1.5).requires_grad_()
+    kz = torch.zeros(n, device=dev)
+    gx, gy = uni(n, 0, side), uni(n, 0, side)
+    e1 = (torch.randn(n, generator=gen, device=dev) * 0.27).requires_grad_()
+    e2 = (torch.randn(n, generator=gen, device=dev) * 0.27).requires_grad_()
+    gw = uni(n, 0.5, 1.5).requires_grad_()
+    gz, gm = torch.ones(n, device=dev), torch.zeros(n, device=dev)
+    kb = torch.randn(n, generator=gen, device=dev) * 0.02
+    bins = tp.create_bins(1.0, 200.0, 20, tp.LOG)
+    ctx = tp.Context(0)
+
+    def step():
+        xi, _ = ag.kappa_kappa(dict(x=kx, y=ky, val=kap, w=kw), dict(x=kx, y=ky, kappa_E=kap, kappa_B=kb, w=kw),
+                               bins, ctx=ctx)
+        gt, _ = ag.shear_scalar(dict(x=kx, y=ky, z=kz, val=kap, w=kw),
+                                dict(x=gx, y=gy, z=gz, e1=e1, e2=e2, w=gw, m=gm), bins, ctx=ctx)
+        sig = torch.full_like(xi, 1e-4)
+        loss = ag.chi2(xi, torch.zeros_like(xi), sig) + ag.chi2(gt, torch.zeros_like(gt), sig)
+        for t in (kap, kw, e1, e2, gw):
+            t.grad = None
+        loss.backward()
+        return loss
+
+    for i in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    r = tp.correlate(tp.KAPPA, dict(x=kx.data_ptr(), y=ky.data_ptr(), val=kap.data_ptr(), w=kw.data_ptr()),
+                     dict(x=kx.data_ptr(), y=ky.data_ptr(), kappa_E=kap.data_ptr(), kappa_B=kb.data_ptr(),
+                          w=kw.data_ptr()), bins, device_inputs=True, ctx=ctx, n_A=n, n_B=n)
+    r2 = tp.correlate(tp.SHEAR, dict(x=kx.data_ptr(), y=ky.data_ptr(), z=kz.data_ptr(), val=kap.data_ptr(),
+                                     w=kw.data_ptr()),
+                      dict(x=gx.data_ptr(), y=gy.data_ptr(), z=gz.data_ptr(), e1=e1.data_ptr(), e2=e2.data_ptr(),
+                           w=gw.data_ptr(), m=gm.data_ptr()), bins, device_inputs=True, ctx=ctx, n_A=n, n_B=n)
+    fwd_pairs = int(r.n_pairs.sum()) + int(r2.n_pairs.sum())
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    res = {"metric": "pairs/sec of forward + adjoint on 1 MI355X", "value": fwd_pairs * args.steps / el,
+           "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "adjoint_5Mx5M", "n": n, "forward_pairs_per_step": fwd_pairs,
+                      "passes_per_step": "2 forward + per correlation 1 forward, 1 lens-side, 1 source-side"}}
+    print(json.dumps(res), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,13 +261,18 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell", type=float, default=0.0, help="source cell side (0: automatic)")
+    ap.add_argument("--no-autotune", action="store_true", help="mocks_256: skip the batch autotuner")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print("bench.py measures one GPU: the pair-counting path runs on a single MI355X; "
+              "sharding over several GPUs is not implemented", file=sys.stderr)
         return 2
+    if args.config == "mocks_256":
+        return bench_mocks(args)
+    if args.config == "adjoint_5Mx5M":
+        return bench_adjoint(args)
 
     import torch
 
