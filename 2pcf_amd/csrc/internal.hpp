@@ -264,10 +264,6 @@ struct Prepared {
     // what refresh_values needs to re-read the values on the same structure
     const uint32_t *sidx;        // sorted source -> original index
     const uint32_t *counts;      // [key_space + 1] sources per key
-    const uint4 *blk;            // lens blocks (start, count, patch)
-    int64_t n_blk;
-    const uint32_t *choff;       // first chunk of each lens block
-    T dmax;                      // largest chunk box side
     int32_t *err;                // device flags
 };
 

@@ -56,36 +56,22 @@ struct Geo<double> {
 };
 
 // d^2 < edge test for the squared edge with index e.  FP32: decided in FP32
-// unless d^2 lies within the band around the edge, then re-decided in FP64
-// from the FP32 inputs exactly as the FP64 oracle does.  FP64: direct.
-template <typename T>
-__device__ inline bool below_edge(const PairArgs<T> &A, int e, T d2, T xa, T ya, T xs, T ys)
-{
-    if constexpr (sizeof(T) == 4) {
-        const float ef = A.e2f[e];
-        bool p = d2 < ef;
-        if (fabsf(d2 - ef) <= A.band * ef) {
-            const double dx = (double)xa - (double)xs, dy = (double)ya - (double)ys;
-            p = fma(dx, dx, dy * dy) < A.e2d[e];
-        }
-        return p;
-    } else {
-        return d2 < A.e2d[e];
-    }
-}
-
-// The same test with the edge values already in registers.
+// unless d^2 lies within the band [lo, hi] = edge^2 (1 -/+ band) around the
+// edge, then re-decided in FP64 from the FP32 inputs exactly as the FP64
+// oracle does.  Outside the band d2 < lo and d2 < edge^2 agree, so two
+// compares (d2 < lo, d2 <= hi) give both the FP32 decision and the band.
+// FP64: direct.
 template <typename T>
 struct EdgeTest {
-    float ef, band_ef;
+    float lo, hi;
     double ed;
     __device__ __forceinline__ EdgeTest(const PairArgs<T> &A, int e)
-        : ef(A.e2f[e]), band_ef(A.band * A.e2f[e]), ed(A.e2d[e]) {}
+        : lo(A.e2f[e] - A.band * A.e2f[e]), hi(A.e2f[e] + A.band * A.e2f[e]), ed(A.e2d[e]) {}
     __device__ __forceinline__ bool below(T d2, T xa, T ya, T xs, T ys) const
     {
         if constexpr (sizeof(T) == 4) {
-            bool p = d2 < ef;
-            if (fabsf(d2 - ef) <= band_ef) {
+            bool p = d2 < lo;
+            if (!p && d2 <= hi) {
                 asm volatile("" : "+v"(xa), "+v"(ya));
                 const double dx = (double)xa - (double)xs, dy = (double)ya - (double)ys;
                 p = fma(dx, dx, dy * dy) < ed;
@@ -276,16 +262,18 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
 {
     constexpr int WQ = win_q<MZ>();
     const EdgeTest<T> et(A, w.wk + 1);
-    T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, hN = 0, sZ = 0;
+    T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, sZ = 0;
+    uint32_t n_hi = 0;   // pairs at or above the edge (wave total, scalar)
     stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
         const T dx = L.xa - q.x, dy = L.ya - q.y;
         const T dy2 = dy * dy;
         const T d2 = fma(dx, dx, dy2);
-        bool below = d2 < (T)et.ef;
+        bool below;
         if constexpr (sizeof(T) == 4) {
+            below = d2 < et.lo;
             // pairs within the band are re-decided in FP64; the branch is
             // taken by the whole wave only when some lane needs it
-            const bool amb = fabsf(d2 - et.ef) <= et.band_ef;
+            const bool amb = !below && d2 <= et.hi;
             if (__ballot(amb)) {
                 // keeps this rare path a branch (not evaluated for every pair)
                 asm volatile("" ::: "memory");
@@ -301,33 +289,34 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         } else {
             below = d2 < et.ed;
         }
-        const T m2 = below ? (T)0 : (T)1;
-        hN += m2;
+        // the count of the upper bin is a wave total from the lane mask
+        n_hi += __popcll(__builtin_amdgcn_ballot_w64(!below) & w.valid);
         T d, inv;
         Geo<T>::dinv(d2, d, inv);
         if (GUARD) {
             d = (d2 > (T)0) ? d : (T)0;
             inv = (d2 > (T)0) ? inv : (T)0;
         }
-        const T wm = q.w * m2;
+        // selects rather than products with a 0/1 mask
+        const T wm = below ? (T)0 : q.w;
         sT = fma(d, q.w, sT);
         hT = fma(d, wm, hT);
         hW += wm;
-        if (!MZ) hK = fma(q.wk, m2, hK);
+        if (!MZ) hK += below ? (T)0 : q.wk;
         if (KIND == SHEAR) {
-            if (TRACK0) sZ += (d2 == (T)0) ? (T)1 - m2 : (T)0;
+            if (TRACK0) sZ += (d2 == (T)0 && below) ? (T)1 : (T)0;
             const T u = fma(dx, dx, -dy2);
             const T v = dx * dy;
             const T t1 = fma(q.s1w, u, q.s2w2 * v);
             const T t2 = fma(q.s2w, u, -(q.s1w2 * v));
-            const T invm = inv * m2;
+            const T invm = below ? (T)0 : inv;
             sG1 = fma(t1, inv, sG1);
             sG2 = fma(t2, inv, sG2);
             hG1 = fma(t1, invm, hG1);
             hG2 = fma(t2, invm, hG2);
         } else {
-            hG1 = fma(q.s1w, m2, hG1);
-            hG2 = fma(q.s2w, m2, hG2);
+            hG1 += below ? (T)0 : q.s1w;
+            hG2 += below ? (T)0 : q.s2w;
         }
     });
     const int lane = L.lane;
@@ -350,8 +339,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         wacc(w, WQ + QK, lane) += dhK;
     }
     if (KIND == SHEAR && TRACK0) wacc(w, 2 * WQ, lane) += (double)sZ;
-    // per-lane counts are integers <= kSegMax: their wave sum is exact in FP32
-    const double nhi = (double)wave_sum((float)(L.vmask * hN));
+    const double nhi = (double)n_hi;
     tab_count(w, lane, w.wk, n_bin, (double)count * (double)w.n_valid - nhi);
     tab_count(w, lane, w.wk + 1, n_bin, nhi);
     w.used0 = w.used1 = true;
@@ -499,6 +487,15 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                 const Seg<T> sg = C.segs[si];
                 const uint32_t cnt = __builtin_amdgcn_readfirstlane(sg.count);
                 const SrcRec<T> *__restrict__ r = C.srec + __builtin_amdgcn_readfirstlane(sg.start);
+                // Warm L2 with the segment's records: lane l reads one value of
+                // record 4 l (one 128-B line per lane, 256 records; segments are
+                // never empty).  The scalar loads of the passes wait for every
+                // load in flight, so their latency is exposed once per two
+                // records; with the lines on their way to L2 it is the L2
+                // latency rather than HBM's (2 % on the 20M x 20M workload).
+                // The value is only consumed after the pass.
+                const uint32_t pfi = min((uint32_t)(4 * lane), cnt - 1);
+                const float pfv = reinterpret_cast<const float *>(r + pfi)[0];
                 // squared distance range between the chunk box and the segment box
                 const T dxl = fmax((T)0, fmax(sg.x0 - ch.bx1, ch.bx0 - sg.x1));
                 const T dyl2 = fmax((T)0, fmax(sg.y0 - ch.by1, ch.by0 - sg.y1));
@@ -586,6 +583,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                         win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
                     }
                 }
+                asm volatile("" ::"v"(pfv));
                 if (A.counters) {
                     if (cls == 0) { cseg0 += 1; cpair0 += (uint64_t)cnt * ch.count; }
                     else if (cls == 1) { cseg1 += 1; cpair1 += (uint64_t)cnt * ch.count; }

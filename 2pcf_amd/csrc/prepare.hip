@@ -278,6 +278,29 @@ __global__ __launch_bounds__(kBlk) void k_chunk_blocks(int64_t n_blocks, const u
     if (!WRITE) nch[bi] = c;
 }
 
+// Value-dependent fields of existing chunks (z maximum, sums of w and val w),
+// one thread per chunk, in the same order as k_chunk_blocks<T, true>: used
+// when new values arrive on an unchanged structure.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_chunk_sums(int64_t n_chunks, const T *__restrict__ lz,
+                                                     const T *__restrict__ lw, const T *__restrict__ la,
+                                                     Chunk<T> *__restrict__ chunks)
+{
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= n_chunks) return;
+    const uint32_t s = chunks[c].start, e = s + chunks[c].count;
+    T zmax = lz[s];
+    double ws = 0, as = 0;
+    for (uint32_t q = s; q < e; ++q) {
+        zmax = fmax(zmax, lz[q]);
+        ws += (double)lw[q];
+        as += (double)la[q];
+    }
+    chunks[c].zmax = zmax;
+    chunks[c].wsum = ws;
+    chunks[c].asum = as;
+}
+
 // Sources a chunk can meet: the sum over its candidate row ranges.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_chunk_work(int64_t n_chunks, Grid g, int32_t n_pb, double xm, double mg,
@@ -549,10 +572,6 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     p.cstart = cstart;
     p.sidx = sidx2;
     p.counts = counts;
-    p.blk = dblk;
-    p.n_blk = nblk;
-    p.choff = dnch + nblk;
-    p.dmax = dmax;
     p.err = err;
     TPCF_HIP(ctx, hipGetLastError());
     p.chunks = chunks;
@@ -577,9 +596,8 @@ int refresh_values(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, hi
                            a.ya, shear ? a.za : nullptr, a.vala, a.wa, const_cast<T *>(p.lx), const_cast<T *>(p.ly),
                            const_cast<T *>(p.lz), const_cast<T *>(p.la), const_cast<T *>(p.lw), (T)a.zsign);
         if (p.n_chunks > 0)
-            hipLaunchKernelGGL((k_chunk_blocks<T, true>), dim3(blocks_for(p.n_blk)), dim3(kBlk), 0, s, p.n_blk, p.blk,
-                               p.dmax, p.lx, p.ly, p.lz, p.lw, p.la, nullptr, p.choff,
-                               const_cast<Chunk<T> *>(p.chunks));
+            hipLaunchKernelGGL(k_chunk_sums<T>, dim3(blocks_for(p.n_chunks)), dim3(kBlk), 0, s, p.n_chunks, p.lz,
+                               p.lw, p.la, const_cast<Chunk<T> *>(p.chunks));
     }
     TPCF_HIP(ctx, hipGetLastError());
     return TPCF_OK;

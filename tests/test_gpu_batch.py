@@ -123,7 +123,15 @@ def test_batch_autotune(tp, ctx):
     st = b.stats
     print("autotune:", st, flush=True)
     assert st["autotuned"] == 1 and all(t > 0 for t in st["tune_ms"])
+    assert st["tune_ms"][[(32, .35), (64, .35), (128, .35), (64, 1.), (128, 1.)].index(
+        (st["cell_target"], st["split"]))] == min(st["tune_ms"])
     out = b.run()
+    # single calls with the settings the autotuner chose (the same cells and
+    # segments, so the same FP32 partial sums)
+    c1 = tp.Context(0)
+    c1.set_option(3, st["cell_target"])
+    c1.set_option(4, st["split"])
     for c, res in zip(cats, out):
-        compare(res, single(tp, ctx, c, bins), False)
+        compare(res, single(tp, c1, c, bins), False)
+    c1.close()
     b.close()
