@@ -178,9 +178,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
     TPCF_TRY(check_bins(ctx, bins));
     if (!out) return fail(ctx, TPCF_ERR_ARG, "out is NULL");
     if (out->real_bytes != 4 && out->real_bytes != 8) return fail(ctx, TPCF_ERR_ARG, "out->real_bytes must be 4 or 8");
-    if (flags & TPCF_SPHERICAL)
-        return fail(ctx, TPCF_ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build");
-    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE))
+    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE | TPCF_SPHERICAL))
         return fail(ctx, TPCF_ERR_ARG, "unknown flag");
     if (n_a < 0 || n_b < 0 || n_a >= ((int64_t)1 << 31) || n_b >= ((int64_t)1 << 31))
         return fail(ctx, TPCF_ERR_ARG, "catalogue sizes must be in [0, 2^31)");
@@ -230,6 +228,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
         TPCF_TRY(upload(ctx, tpcf_ctx::B_PATCH_B, pat_b, n_b, &a.patch_b));
     }
     if (!shear) { a.za = nullptr; a.zb = nullptr; a.mb = nullptr; }
+    if (flags & TPCF_SPHERICAL) return sph_correlate<T>(ctx, a, flags, out, t0);
     return run_call<T>(ctx, a, flags, out, t0, nullptr);
 }
 
@@ -263,6 +262,9 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
     std::memset(&p, 0, sizeof p);
     if (!empty) {
         TPCF_TRY(prepare<T>(ctx, a, p));
+        if constexpr (sizeof(T) == 8) {
+            if (a.sph_pa) TPCF_TRY(sph_finish(ctx, a, p));
+        }
         double *pp, *marks = nullptr;
         unsigned long long *ppn;
         TPCF_TRY(ensure(ctx, tpcf_ctx::B_PP, sizeof(double) * kNQ * kSlots * pp_total, (void **)&pp));

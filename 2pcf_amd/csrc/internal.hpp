@@ -198,6 +198,8 @@ struct tpcf_ctx {
         B_PP, B_PPN, B_MARKS, B_OUT, B_TMP, B_PROBE, B_E2,
         B_BLK, B_BLKCH, B_WORK, B_ITEMS,
         B_GRAD_A, B_GRAD_B, B_ONES, B_ZEROS,
+        // spherical geometry: FP64 inputs, projected coordinates, lens frames, sorted extras
+        B_SPH_IN, B_SPH_XA, B_SPH_XB, B_SPH_LP, B_SPH_LF, B_SPH_SEGP, B_SPH_CHP, B_SPH_PART,
         B_COUNT
     };
     tpcf::DevBuf buf[B_COUNT];
@@ -237,6 +239,14 @@ struct CallArgs {
     bool verbose;
     int zsign = 1;              // z as stored for the z test: +z, or -z to reverse z_A > z_B
     bool marks_device = false;  // keep the marks on the device (adjoint) instead of copying them out
+    // Spherical geometry (T = double only): xa/ya and xb/yb then hold the
+    // projected grid coordinates (q, s) of the unit vectors, sph_pa/sph_pb the
+    // third rotated component p, sph_frame the lens' east and north unit
+    // vectors in the rotated basis ([n_a][6]); kbins has the chord edges
+    // 2 sin(e/2) that the neighbour search and the pair kernel use, while bins
+    // (the angles) give the outputs.  kbins == bins on the flat sky.
+    const tpcf_bins *kbins = nullptr;
+    const double *sph_pa = nullptr, *sph_pb = nullptr, *sph_frame = nullptr;
 };
 
 // Device-side result of the preparation stage.
@@ -265,6 +275,10 @@ struct Prepared {
     const uint32_t *sidx;        // sorted source -> original index
     const uint32_t *counts;      // [key_space + 1] sources per key
     int32_t *err;                // device flags
+    // spherical geometry (else NULL): sorted lens p and frames [n_lens][6]
+    // (east, north), per segment and per chunk (min, max) of p
+    const double *sph_lp, *sph_lf;
+    const double2 *sph_segp, *sph_chp;
 };
 
 template <typename T>
@@ -307,6 +321,8 @@ struct CatArgs {
     double *pp;                     // [slot][P_A][P_B][n_bin][5]
     unsigned long long *ppn;        // [slot][P_A][P_B][n_bin]
     double *marks;                  // [n_lens][n_bin][4] or NULL
+    const double *sph_lp, *sph_lf;  // spherical geometry (else NULL)
+    const double2 *sph_segp, *sph_chp;
 };
 
 
@@ -329,6 +345,14 @@ void fill_u64(hipStream_t s, unsigned long long *p, int64_t n);
 // The correlation call once the inputs are on the device (api.hip).
 template <typename T>
 int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, double t0, Prepared<T> *pout);
+
+// Spherical coordinates (sphere.hip): projects the device inputs, converts
+// them to FP64 and runs the FP64 path with the chord bins.
+template <typename T>
+int sph_correlate(tpcf_ctx *ctx, const CallArgs<T> &a, uint32_t flags, tpcf_out *out, double t0);
+// After prepare: the sorted lens p and frames, the records' p and the
+// per-segment / per-chunk p ranges.
+int sph_finish(tpcf_ctx *ctx, const CallArgs<double> &a, Prepared<double> &p);
 
 double now_ms();
 

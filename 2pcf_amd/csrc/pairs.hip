@@ -164,7 +164,93 @@ struct LaneData {
     bool valid;
     uint32_t li;
     int lane;
+    // spherical geometry: third rotated component and the east / north unit
+    // vectors at the lens in the rotated basis (p, q, s)
+    T pa, fe0, fe1, fe2, fn0, fn1, fn2;
 };
+
+// Pair geometry.  Flat sky: (a, b) = (x_A - x_B, y_A - y_B), c = b^2,
+// d2 = a^2 + b^2; the projection terms are u = a^2 - b^2 = d^2 cos 2phi,
+// v = a b = d^2 sin 2phi / 2 and inv = 1/d^2 (src/tpcf.f90:317-319,
+// :343-344).  Sphere: (a, b, c) = u_B - u_A in the rotated basis (the
+// source's p sits in record slot s1w2), d2 = |u_B - u_A|^2 (the squared
+// chord); E, N = components along the east and north unit vectors at the
+// lens, u = E^2 - N^2, v = E N, inv = 1/(E^2 + N^2) = 1/sin^2 d, which are the
+// reference's cos 2phi and sin 2phi from angles_sphere (src/utils.f90:618-639)
+// without trigonometry; the separation angle is 2 asin(chord / 2).
+template <typename T, bool SPH>
+struct PairGeo {
+    T a, b, c, d2;
+};
+
+template <typename T, bool SPH>
+__device__ __forceinline__ PairGeo<T, SPH> pair_geo(const LaneData<T> &L, const SrcRec<T> &q)
+{
+    PairGeo<T, SPH> g;
+    if constexpr (!SPH) {
+        g.a = L.xa - q.x;
+        g.b = L.ya - q.y;
+        g.c = g.b * g.b;
+        g.d2 = fma(g.a, g.a, g.c);
+    } else {
+        g.a = q.s1w2 - L.pa;
+        g.b = q.x - L.xa;
+        g.c = q.y - L.ya;
+        g.d2 = fma(g.a, g.a, fma(g.b, g.b, g.c * g.c));
+    }
+    return g;
+}
+
+// Separation angle from the squared chord: 2 asin(c / 2).  Below c = 0.07
+// (4 degrees) the series of asin(x)/x to x^12 is exact to rounding.
+__device__ __forceinline__ double sph_angle(double d2)
+{
+    const double x = 0.5 * sqrt(d2);
+    const double x2 = x * x;
+    if (x2 < 1.3e-3) {
+        double t = 231.0 / 13312.0;
+        t = fma(t, x2, 63.0 / 2816.0);
+        t = fma(t, x2, 35.0 / 1152.0);
+        t = fma(t, x2, 5.0 / 112.0);
+        t = fma(t, x2, 3.0 / 40.0);
+        t = fma(t, x2, 1.0 / 6.0);
+        t = fma(t, x2, 1.0);
+        return 2.0 * x * t;
+    }
+    return 2.0 * asin(fmin(x, 1.0));
+}
+
+// Separation, 1/d^2 (or 1/sin^2 d) and the projection terms of one pair; d
+// and inv are 0 at zero separation (and at an antipodal pair on the sphere).
+template <typename T, bool SPH, bool GUARD>
+__device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T, SPH> &g, T &d, T &inv, T &u, T &v)
+{
+    if constexpr (!SPH) {
+        Geo<T>::dinv(g.d2, d, inv);
+        if (GUARD) {
+            d = (g.d2 > (T)0) ? d : (T)0;
+            inv = (g.d2 > (T)0) ? inv : (T)0;
+        }
+        u = fma(g.a, g.a, -g.c);
+        v = g.a * g.b;
+    } else {
+        const T E = fma(L.fe0, g.a, fma(L.fe1, g.b, L.fe2 * g.c));
+        const T N = fma(L.fn0, g.a, fma(L.fn1, g.b, L.fn2 * g.c));
+        const T s2 = fma(E, E, N * N);
+        inv = (s2 > (T)0) ? (T)1 / s2 : (T)0;
+        u = fma(E, E, -(N * N));
+        v = E * N;
+        d = sph_angle(g.d2);
+    }
+}
+
+// 2 e1 w of a record (slot s1w2 holds p on the sphere)
+template <typename T, bool SPH>
+__device__ __forceinline__ T s1w2_of(const SrcRec<T> &q)
+{
+    if constexpr (SPH) return q.s1w + q.s1w;
+    else return q.s1w2;
+}
 
 template <bool MZ>
 __device__ __forceinline__ void win_clear(Win &w, int lane)
@@ -255,7 +341,7 @@ __device__ __forceinline__ void win_open(Win &w, const LaneData<T> &L, int n_bin
 // boxes overlap, so a pair may sit at zero separation (its per-pair values are
 // forced finite); TRACK0: shear with x_min == 0, where such a pair is in bin 0
 // (reference: NaN).  MZ: all m == 0, so K == W.
-template <typename T, int KIND, bool MZ, bool GUARD, bool TRACK0>
+template <typename T, int KIND, bool MZ, bool GUARD, bool TRACK0, bool SPH>
 __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                           const SrcRec<T> *__restrict__ r, const Seg<T> &sg, uint32_t count,
                                           int n_bin)
@@ -265,9 +351,8 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
     T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, sZ = 0;
     uint32_t n_hi = 0;   // pairs at or above the edge (wave total, scalar)
     stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
-        const T dx = L.xa - q.x, dy = L.ya - q.y;
-        const T dy2 = dy * dy;
-        const T d2 = fma(dx, dx, dy2);
+        const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+        const T d2 = g.d2;
         bool below;
         if constexpr (sizeof(T) == 4) {
             below = d2 < et.lo;
@@ -291,12 +376,8 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         }
         // the count of the upper bin is a wave total from the lane mask
         n_hi += __popcll(__builtin_amdgcn_ballot_w64(!below) & w.valid);
-        T d, inv;
-        Geo<T>::dinv(d2, d, inv);
-        if (GUARD) {
-            d = (d2 > (T)0) ? d : (T)0;
-            inv = (d2 > (T)0) ? inv : (T)0;
-        }
+        T d, inv, u, v;
+        pair_proj<T, SPH, GUARD>(L, g, d, inv, u, v);
         // selects rather than products with a 0/1 mask
         const T wm = below ? (T)0 : q.w;
         sT = fma(d, q.w, sT);
@@ -305,10 +386,8 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         if (!MZ) hK += below ? (T)0 : q.wk;
         if (KIND == SHEAR) {
             if (TRACK0) sZ += (d2 == (T)0 && below) ? (T)1 : (T)0;
-            const T u = fma(dx, dx, -dy2);
-            const T v = dx * dy;
             const T t1 = fma(q.s1w, u, q.s2w2 * v);
-            const T t2 = fma(q.s2w, u, -(q.s1w2 * v));
+            const T t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v));
             const T invm = below ? (T)0 : inv;
             sG1 = fma(t1, inv, sG1);
             sG2 = fma(t2, inv, sG2);
@@ -348,7 +427,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
 
 // General pass over a segment for the window bins wk (mask m) and wk+1 (m2),
 // bounded on both sides of the window, with the z test when needed.
-template <typename T, int KIND, bool MZ, bool ZCHECK>
+template <typename T, int KIND, bool MZ, bool ZCHECK, bool SPH>
 __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                          const SrcRec<T> *__restrict__ r, const T *__restrict__ zr, uint32_t count,
                                          int n_bin)
@@ -363,9 +442,8 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
     uint32_t n0 = 0, n1 = 0;
     for (uint32_t j = 0; j < count; ++j) {
         const SrcRec<T> q = r[j];
-        const T dx = L.xa - q.x, dy = L.ya - q.y;
-        const T dy2 = dy * dy;
-        const T d2 = fma(dx, dx, dy2);
+        const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+        const T d2 = g.d2;
         const bool plo = emid.below(d2, L.xa, L.ya, q.x, q.y);
         const bool ge_lo = has_lo ? !elo.below(d2, L.xa, L.ya, q.x, q.y) : true;
         const bool lt_hi = has_hi ? ehi.below(d2, L.xa, L.ya, q.x, q.y) : true;
@@ -376,21 +454,17 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
         const T m = in0 ? (T)1 : (T)0, m2 = in1 ? (T)1 : (T)0;
         n0 += __popcll(__builtin_amdgcn_ballot_w64(m != (T)0) & w.valid);
         n1 += __popcll(__builtin_amdgcn_ballot_w64(m2 != (T)0) & w.valid);
-        T d, inv;
-        Geo<T>::dinv(d2, d, inv);
         // keep every per-pair value finite so that 0/1 masks cannot leak a NaN
         // from an excluded pair; in-bin zero separations are tracked apart
-        d = (d2 > (T)0) ? d : (T)0;
-        inv = (d2 > (T)0) ? inv : (T)0;
+        T d, inv, u, v;
+        pair_proj<T, SPH, true>(L, g, d, inv, u, v);
         if (track0) sZ += (d2 == (T)0) ? m : (T)0;
         const T tv = d * q.w;
         sT0 = fma(tv, m, sT0);
         sT1 = fma(tv, m2, sT1);
         if (KIND == SHEAR) {
-            const T u = fma(dx, dx, -dy2);
-            const T v = dx * dy;
             const T g1 = fma(q.s1w, u, q.s2w2 * v) * inv;
-            const T g2 = fma(q.s2w, u, -(q.s1w2 * v)) * inv;
+            const T g2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)) * inv;
             sG10 = fma(g1, m, sG10);
             sG11 = fma(g1, m2, sG11);
             sG20 = fma(g2, m, sG20);
@@ -428,7 +502,7 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
 
 // The work of one item: lens chunk item.chunk against the source segments in
 // its share of cell rows, into the per-XCD accumulator copy `slot`.
-template <typename T, int KIND, bool MARKS, bool MZ>
+template <typename T, int KIND, bool MARKS, bool MZ, bool SPH>
 __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T> &C, const Item item, const int slot,
                                           const int lane, double *acc, const T e2_lane,
                                           unsigned long long *__restrict__ probe)
@@ -446,6 +520,14 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
     L.wa = L.valid ? C.lw[L.li] : (T)0;
     L.aa = L.valid ? C.la[L.li] : (T)0;
     L.vmask = L.valid ? (T)1 : (T)0;
+    double2 chp = make_double2(0.0, 0.0);
+    if constexpr (SPH) {
+        const double *f = C.sph_lf + 6 * (size_t)L.li;   // [n_lens][east 3, north 3]
+        L.pa = C.sph_lp[L.li];
+        L.fe0 = f[0]; L.fe1 = f[1]; L.fe2 = f[2];
+        L.fn0 = f[3]; L.fn1 = f[4]; L.fn2 = f[5];
+        chp = C.sph_chp[item.chunk];
+    }
     const int n_bin = A.n_bin;
 
     Win w;
@@ -501,8 +583,18 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                 const T dyl2 = fmax((T)0, fmax(sg.y0 - ch.by1, ch.by0 - sg.y1));
                 const T dxh = fmax(sg.x1 - ch.bx0, ch.bx1 - sg.x0);
                 const T dyh = fmax(sg.y1 - ch.by0, ch.by1 - sg.y0);
-                const T lo2 = (dxl * dxl + dyl2 * dyl2) * A.cls_lo;
-                const T hi2 = (dxh * dxh + dyh * dyh) * A.cls_hi;
+                T lo2 = dxl * dxl + dyl2 * dyl2;
+                T hi2 = dxh * dxh + dyh * dyh;
+                if constexpr (SPH) {
+                    // the third component: chord bounds between the 3-D boxes
+                    const double2 sp = C.sph_segp[si];
+                    const T dpl = fmax((T)0, fmax(sp.x - chp.y, chp.x - sp.y));
+                    const T dph = fmax(sp.y - chp.x, chp.y - sp.x);
+                    lo2 = fma(dpl, dpl, lo2);
+                    hi2 = fma(dph, dph, hi2);
+                }
+                lo2 *= A.cls_lo;
+                hi2 *= A.cls_hi;
                 const int kmin = __popcll(__ballot(e2_lane <= lo2)) - 1;
                 const int kmax = __popcll(__ballot(e2_lane <= hi2)) - 1;
                 if (kmax < 0 || kmin >= n_bin) continue;   // all pairs outside [x_min, x_max]
@@ -516,22 +608,18 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                     T sT = 0, sG1 = 0, sG2 = 0;
                     if (KIND == SHEAR) {
                         stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
-                            const T dx = L.xa - q.x, dy = L.ya - q.y;
-                            const T dy2 = dy * dy;
-                            const T d2 = fma(dx, dx, dy2);
-                            const T u = fma(dx, dx, -dy2);   // d^2 cos(2 phi)
-                            const T v = dx * dy;             // d^2 sin(2 phi) / 2
-                            T d, inv;
-                            Geo<T>::dinv(d2, d, inv);
+                            const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+                            T d, inv, u, v;
+                            pair_proj<T, SPH, false>(L, g, d, inv, u, v);
                             sT = fma(d, q.w, sT);
                             sG1 = fma(fma(q.s1w, u, q.s2w2 * v), inv, sG1);
-                            sG2 = fma(fma(q.s2w, u, -(q.s1w2 * v)), inv, sG2);
+                            sG2 = fma(fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)), inv, sG2);
                         });
                     } else {
                         stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
-                            const T dx = L.xa - q.x, dy = L.ya - q.y;
-                            const T d2 = fma(dx, dx, dy * dy);
-                            sT = fma(Geo<T>::dist(d2), q.w, sT);
+                            const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+                            if constexpr (SPH) sT = fma(sph_angle(g.d2), q.w, sT);
+                            else sT = fma(Geo<T>::dist(g.d2), q.w, sT);
                         });
                     }
                     const int o = (kmin - w.wk) * WQ;
@@ -566,11 +654,11 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                     // ---- edge: one bin edge inside the distance range ----
                     win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, kmin);
                     if (lo2 > (T)0) {
-                        edge_pass<T, KIND, MZ, false, false>(w, L, A, r, sg, cnt, n_bin);
+                        edge_pass<T, KIND, MZ, false, false, SPH>(w, L, A, r, sg, cnt, n_bin);
                     } else if (KIND == SHEAR && A.xmin_zero && kmin == 0) {
-                        edge_pass<T, KIND, MZ, true, true>(w, L, A, r, sg, cnt, n_bin);
+                        edge_pass<T, KIND, MZ, true, true, SPH>(w, L, A, r, sg, cnt, n_bin);
                     } else {
-                        edge_pass<T, KIND, MZ, true, false>(w, L, A, r, sg, cnt, n_bin);
+                        edge_pass<T, KIND, MZ, true, false, SPH>(w, L, A, r, sg, cnt, n_bin);
                     }
                 } else {
                     cls = 2;
@@ -578,8 +666,8 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                     const T *zr = (KIND == SHEAR) ? C.sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
                     for (int w0 = kmin; w0 <= kmax; w0 += 2) {
                         win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, w0);
-                        if (zc) win_pass<T, KIND, MZ, true>(w, L, A, r, zr, cnt, n_bin);
-                        else win_pass<T, KIND, MZ, false>(w, L, A, r, zr, cnt, n_bin);
+                        if (zc) win_pass<T, KIND, MZ, true, SPH>(w, L, A, r, zr, cnt, n_bin);
+                        else win_pass<T, KIND, MZ, false, SPH>(w, L, A, r, zr, cnt, n_bin);
                         win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
                     }
                 }
@@ -622,19 +710,24 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
 // waves (no spills).
 template <typename T, bool MZ>
 constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : 4; }
+// the spherical geometry (FP64) holds seven more lane values: 3 waves
+template <typename T, bool MZ, bool SPH>
+constexpr int pair_waves() { return SPH ? 3 : pair_waves_per_simd<T, MZ>(); }
 
 // One catalogue: one item per wave, items dealt so that consecutive items
 // (neighbouring chunks) run on one XCD and share its L2.  The catalogue's
 // pointers arrive as separate arguments (scalar loads on demand).
-template <typename T, int KIND, bool MARKS, bool MZ>
+template <typename T, int KIND, bool MARKS, bool MZ, bool SPH>
 __global__ __launch_bounds__(kWave *kWavesPerBlock)
-__attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pairs(
+__attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     const PairArgs<T> A, const Grid g, const int32_t n_pa, const int32_t n_pb, const double margin,
     const Chunk<T> *__restrict__ chunks, const T *__restrict__ lx, const T *__restrict__ ly,
     const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
     const uint32_t *__restrict__ seg_first, const int4 *__restrict__ pbox, const SrcRec<T> *__restrict__ srec,
     const T *__restrict__ sz, const T *__restrict__ e2lane, const Item *__restrict__ items, double *__restrict__ pp,
-    unsigned long long *__restrict__ ppn, double *__restrict__ marks, unsigned long long *__restrict__ probe)
+    unsigned long long *__restrict__ ppn, double *__restrict__ marks, unsigned long long *__restrict__ probe,
+    const double *__restrict__ sph_lp, const double *__restrict__ sph_lf, const double2 *__restrict__ sph_segp,
+    const double2 *__restrict__ sph_chp)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -659,7 +752,11 @@ __attribute__((amdgpu_waves_per_eu(pair_waves_per_simd<T, MZ>(), 8))) void k_pai
     C.pp = pp;
     C.ppn = ppn;
     C.marks = marks;
-    pair_item<T, KIND, MARKS, MZ>(A, C, items[it], slot, lane, s_acc[wv], e2lane[lane], probe);
+    C.sph_lp = sph_lp;
+    C.sph_lf = sph_lf;
+    C.sph_segp = sph_segp;
+    C.sph_chp = sph_chp;
+    pair_item<T, KIND, MARKS, MZ, SPH>(A, C, items[it], slot, lane, s_acc[wv], e2lane[lane], probe);
 }
 
 template <typename T>
@@ -720,6 +817,10 @@ CatArgs<T> cat_args(const Prepared<T> &p, int32_t n_pa, int32_t n_pb, double *pp
     C.pp = pp;
     C.ppn = ppn;
     C.marks = marks;
+    C.sph_lp = p.sph_lp;
+    C.sph_lf = p.sph_lf;
+    C.sph_segp = p.sph_segp;
+    C.sph_chp = p.sph_chp;
     return C;
 }
 
@@ -729,7 +830,7 @@ int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double 
 {
     PairArgs<T> A;
     const T *e2lane;
-    TPCF_TRY(make_pair_args<T>(ctx, a.bins, A, &e2lane));
+    TPCF_TRY(make_pair_args<T>(ctx, a.kbins ? a.kbins : a.bins, A, &e2lane));
     A.n_items = p.n_items;
     const int64_t nblk = (p.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
     A.per_xcd = (nblk + kSlots - 1) / kSlots;
@@ -754,18 +855,33 @@ int launch_pairs(tpcf_ctx *ctx, hipStream_t s, int kind, bool m_zero, bool marks
 {
     if (A.n_items == 0) return TPCF_OK;
     const dim3 grid((unsigned)(A.per_xcd * kSlots)), block(kWave * kWavesPerBlock);
-#define TPCF_LAUNCH(K, M, Z)                                                                                    \
-    hipLaunchKernelGGL((k_pairs<T, K, M, Z>), grid, block, 0, s, A, C.g, C.n_pa, C.n_pb, C.margin, C.chunks,      \
+#define TPCF_LAUNCH(K, M, Z, S)                                                                                   \
+    hipLaunchKernelGGL((k_pairs<T, K, M, Z, S>), grid, block, 0, s, A, C.g, C.n_pa, C.n_pb, C.margin, C.chunks,     \
                        C.lx, C.ly, C.lz, C.la, C.lw, C.segs, C.seg_first, C.pbox, C.srec, C.sz, e2lane, items, C.pp, \
-                       C.ppn, C.marks, probe)
-    if (kind == SHEAR) {
-        if (m_zero) {
-            if (marks) TPCF_LAUNCH(SHEAR, true, true); else TPCF_LAUNCH(SHEAR, false, true);
+                       C.ppn, C.marks, probe, C.sph_lp, C.sph_lf, C.sph_segp, C.sph_chp)
+    if (C.sph_lp) {
+        // spherical geometry: FP64 only
+        if constexpr (sizeof(T) == 8) {
+            if (kind == SHEAR) {
+                if (m_zero) {
+                    if (marks) TPCF_LAUNCH(SHEAR, true, true, true); else TPCF_LAUNCH(SHEAR, false, true, true);
+                } else {
+                    if (marks) TPCF_LAUNCH(SHEAR, true, false, true); else TPCF_LAUNCH(SHEAR, false, false, true);
+                }
+            } else {
+                if (marks) TPCF_LAUNCH(KAPPA, true, true, true); else TPCF_LAUNCH(KAPPA, false, true, true);
+            }
         } else {
-            if (marks) TPCF_LAUNCH(SHEAR, true, false); else TPCF_LAUNCH(SHEAR, false, false);
+            return fail(ctx, TPCF_ERR_UNSUPPORTED, "spherical geometry runs in FP64");
+        }
+    } else if (kind == SHEAR) {
+        if (m_zero) {
+            if (marks) TPCF_LAUNCH(SHEAR, true, true, false); else TPCF_LAUNCH(SHEAR, false, true, false);
+        } else {
+            if (marks) TPCF_LAUNCH(SHEAR, true, false, false); else TPCF_LAUNCH(SHEAR, false, false, false);
         }
     } else {
-        if (marks) TPCF_LAUNCH(KAPPA, true, true); else TPCF_LAUNCH(KAPPA, false, true);
+        if (marks) TPCF_LAUNCH(KAPPA, true, true, false); else TPCF_LAUNCH(KAPPA, false, true, false);
     }
 #undef TPCF_LAUNCH
     TPCF_HIP(ctx, hipGetLastError());

@@ -348,7 +348,9 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
 {
     hipStream_t s = ctx->stream;
     const bool shear = (a.kind == SHEAR);
-    const double x_max = (double)a.bins->x_max;
+    const double x_max = (double)(a.kbins ? a.kbins : a.bins)->x_max;
+    p.sph_lp = p.sph_lf = nullptr;
+    p.sph_segp = p.sph_chp = nullptr;
 
     // ---- bounding boxes and finiteness of both catalogues ----
     const unsigned nbs = (unsigned)std::min<int64_t>(1024, (a.n_b + kBlk - 1) / kBlk);

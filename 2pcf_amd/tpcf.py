@@ -204,11 +204,12 @@ def _host(a, dtype):
 def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str = "f32", marks: bool = False,
               patch_A=None, patch_B=None, n_patch_A: int = 1, n_patch_B: int = 1, real_bytes: int = 4,
               device_inputs: bool = False, verbose: bool = False, ctx: Optional[Context] = None,
-              n_A: Optional[int] = None, n_B: Optional[int] = None) -> Shear2pcfResults:
+              n_A: Optional[int] = None, n_B: Optional[int] = None, spherical: bool = False) -> Shear2pcfResults:
     """One call of the C-ABI.  ``lens``: x, y, val, w (+ z for shear);
     ``src``: x, y, w + (z, e1, e2, m) for shear or (kappa_E, kappa_B) for kappa.
     With ``device_inputs`` the values are device addresses (ints) of arrays of
-    the given precision and ``n_A``/``n_B`` give the sizes."""
+    the given precision and ``n_A``/``n_B`` give the sizes.  With ``spherical``
+    x, y are longitude and latitude and the bins angles, all in radians."""
     ctx = ctx or default_context()
     L = N.lib()
     dt = np.float32 if precision == "f32" else np.float64
@@ -251,7 +252,8 @@ def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str
         for k in ("pp_theta", "pp_gt", "pp_gx", "pp_w", "pp_k"):
             res[k] = np.zeros((P_B, P_A, nb), dtype=np.float64)
             setattr(o, k, res[k].ctypes.data_as(C.c_void_p))
-    flags = (N.FLAG_DEVICE_INPUTS if device_inputs else 0) | (N.FLAG_VERBOSE if verbose else 0)
+    flags = (N.FLAG_DEVICE_INPUTS if device_inputs else 0) | (N.FLAG_VERBOSE if verbose else 0) | \
+        (N.FLAG_SPHERICAL if spherical else 0)
     pa = ptr(patch_A, np.int32) if patch_A is not None else None
     pb = ptr(patch_B, np.int32) if patch_B is not None else None
     if kind == SHEAR:
@@ -285,16 +287,17 @@ def shear_2pcf(field: LensingSurveyField, bins: Binning, results=None, spherical
                ctx: Optional[Context] = None) -> Shear2pcfResults:
     """shear_2pcf(field, bins, results, spherical_coords, num_threads,
     use_kd_tree, verbose), src/tpcf.f90:134-160.  Returns the results object
-    (the reference fills its ``results`` argument)."""
-    if spherical_coords:
-        raise TpcfError(N.ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build")
+    (the reference fills its ``results`` argument).  With spherical_coords
+    positions are longitude / latitude and the bins angles, in radians (the
+    reference's driver converts degrees before the call, src/main.f90:427-431,
+    :597-603)."""
     L, S = field.lenses, field.shapes
     prec = precision or _precision_of(L.x)
     lens = dict(x=L.x, y=L.y, z=L.z if L.z is not None else np.zeros(L.n), val=L.val, w=L.w)
     src = dict(x=S.x, y=S.y, z=S.z, e1=S.e1, e2=S.e2, w=S.w, m=S.m)
     return correlate(SHEAR, lens, src, bins, precision=prec, marks=marks, patch_A=patch_A, patch_B=patch_B,
                      n_patch_A=n_patch_A, n_patch_B=n_patch_B, real_bytes=4 if prec == "f32" else 8,
-                     verbose=verbose, ctx=ctx)
+                     verbose=verbose, ctx=ctx, spherical=spherical_coords)
 
 
 def kappa_2pcf(field: LensingSurveyField, bins: Binning, results=None, spherical_coords: bool = False,
@@ -303,15 +306,13 @@ def kappa_2pcf(field: LensingSurveyField, bins: Binning, results=None, spherical
                ctx: Optional[Context] = None) -> Shear2pcfResults:
     """kappa_2pcf(field, bins, results, spherical_coords, num_threads,
     use_kd_tree, verbose), src/tpcf.f90:162-188."""
-    if spherical_coords:
-        raise TpcfError(N.ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build")
     L, K = field.lenses, field.convergence
     prec = precision or _precision_of(L.x)
     lens = dict(x=L.x, y=L.y, val=L.val, w=L.w)
     src = dict(x=K.x, y=K.y, kappa_E=K.kappa_E, kappa_B=K.kappa_B, w=K.w)
     return correlate(KAPPA, lens, src, bins, precision=prec, marks=marks, patch_A=patch_A, patch_B=patch_B,
                      n_patch_A=n_patch_A, n_patch_B=n_patch_B, real_bytes=4 if prec == "f32" else 8,
-                     verbose=verbose, ctx=ctx)
+                     verbose=verbose, ctx=ctx, spherical=spherical_coords)
 
 
 def stack_fields(results: list[Shear2pcfResults]) -> dict:
@@ -332,7 +333,7 @@ _GRAD_KEYS = {SHEAR: (("val", "w"), ("e1", "e2", "w", "m")), KAPPA: (("val", "w"
 
 def adjoint(kind: int, lens: dict, src: dict, bins: Binning, gbar1, gbar2, *, precision: str = "f32",
             device_inputs: bool = False, grads: Optional[dict] = None, ctx: Optional[Context] = None,
-            n_A: Optional[int] = None, n_B: Optional[int] = None) -> dict:
+            n_A: Optional[int] = None, n_B: Optional[int] = None, spherical: bool = False) -> dict:
     """Gradient of sum_b (gbar1_b g_t,b + gbar2_b g_x,b) (shear; xi_E / xi_B
     for kappa) with respect to the lens values and weights and the source
     components and weights: {"lens": {"val", "w"}, "src": {"e1", "e2", "w",
@@ -372,7 +373,7 @@ def adjoint(kind: int, lens: dict, src: dict, bins: Binning, gbar1, gbar2, *, pr
                 a = np.zeros(n, dtype=np.float64)
                 out[side][k] = a
                 gptr.append(a.ctypes.data_as(C.c_void_p))
-    flags = N.FLAG_DEVICE_INPUTS if device_inputs else 0
+    flags = (N.FLAG_DEVICE_INPUTS if device_inputs else 0) | (N.FLAG_SPHERICAL if spherical else 0)
     pc = lambda a: g1.ctypes.data_as(C.c_void_p) if a is g1 else g2.ctypes.data_as(C.c_void_p)  # noqa: E731
     if kind == SHEAR:
         fn = L.tpcf_adjoint_shear_f32 if precision == "f32" else L.tpcf_adjoint_shear_f64

@@ -53,7 +53,10 @@ enum tpcf_status {
 /* Flags of the correlation calls. */
 enum tpcf_flags {
     TPCF_DEVICE_INPUTS = 1u << 0,  /* input arrays (and patch labels) are device pointers */
-    TPCF_SPHERICAL = 1u << 1,      /* spherical_coords = .true. (not implemented: TPCF_ERR_UNSUPPORTED) */
+    TPCF_SPHERICAL = 1u << 1,      /* spherical_coords = .true.: x, y are longitude and latitude and the bin
+                                      edges angles, all in radians (src/tpcf.f90:313-320, :337-345); computed
+                                      in FP64 whatever the entry point's precision.  Not for tpcf_adjoint_*
+                                      (TPCF_ERR_UNSUPPORTED). */
     TPCF_VERBOSE = 1u << 2,        /* verbose = .true.: print a summary to stdout */
     TPCF_NO_KD_TREE = 1u << 3,     /* use_kd_tree = .false.: accepted; the pair set is identical */
     TPCF_BATCH_AUTOTUNE = 1u << 4, /* batch: time candidate cell / work-item sizes first */

@@ -46,7 +46,7 @@ class _OrcArgs(C.Structure):
         ("xa", _dp), ("ya", _dp), ("za", _dp), ("vala", _dp), ("wa", _dp),
         ("xb", _dp), ("yb", _dp), ("zb", _dp), ("s1b", _dp), ("s2b", _dp), ("wb", _dp), ("mb", _dp),
         ("edges", _dp), ("patch_a", _i32p), ("patch_b", _i32p), ("n_pa", C.c_int32), ("n_pb", C.c_int32),
-        ("brute", C.c_int32), ("n_threads", C.c_int32),
+        ("brute", C.c_int32), ("n_threads", C.c_int32), ("spherical", C.c_int32),
     ]
 
 
@@ -102,6 +102,8 @@ def lib():
         _lib.orc_correlate.argtypes = [C.POINTER(_OrcArgs), C.POINTER(_OrcOut)]
         _lib.orc_correlate.restype = C.c_int
         _lib.orc_finalize.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        _lib.orc_chord_edges_squared.argtypes = [_dp, C.c_int, _dp]
+        _lib.orc_sph_pair.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib.rp_create_bins.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.POINTER(_RpBins)]
         _lib.rp_create_bins.restype = C.c_int
         _lib.rp_find_bin_array.argtypes = [C.c_int64, _fp, C.POINTER(_RpBins), _i32p]
@@ -139,9 +141,11 @@ def bin_of_d2(d2, edges_: np.ndarray) -> np.ndarray:
 
 def correlate(kind: int, lens: dict, src: dict, edges_: np.ndarray, patch_a=None, patch_b=None,
               n_pa: int = 1, n_pb: int = 1, brute: bool = False, marks: bool = False,
-              patches: bool = False, n_threads: int | None = None) -> dict:
+              patches: bool = False, n_threads: int | None = None, spherical: bool = False) -> dict:
     """Raw FP64 sums.  ``lens`` has x, y, val, w (+ z for shear); ``src`` has
-    x, y, w and e1, e2, m, z (shear) or kappa_E, kappa_B (kappa)."""
+    x, y, w and e1, e2, m, z (shear) or kappa_E, kappa_B (kappa).  With
+    ``spherical`` x, y are longitude and latitude and the edges angles, all in
+    radians."""
     L = lib()
     e = _d(edges_)
     n_bin = len(e) - 1
@@ -182,6 +186,7 @@ def correlate(kind: int, lens: dict, src: dict, edges_: np.ndarray, patch_a=None
     a.n_pa, a.n_pb = n_pa, n_pb
     a.brute = 1 if brute else 0
     a.n_threads = n_threads or min(16, os.cpu_count() or 1)
+    a.spherical = 1 if spherical else 0
 
     res = {
         "n_pairs": np.zeros(n_bin, dtype=np.int64),
@@ -210,6 +215,24 @@ def correlate(kind: int, lens: dict, src: dict, edges_: np.ndarray, patch_a=None
     fin = finalize(res)
     res.update(fin)
     return res
+
+
+def chord_edges_squared(edges_: np.ndarray) -> np.ndarray:
+    """Squared chords 2 sin(e/2) of angular edges (the spherical bin test)."""
+    e = _d(edges_)
+    out = np.zeros_like(e)
+    lib().orc_chord_edges_squared(_ptr(e, _dp), len(e) - 1, _ptr(out, _dp))
+    return out
+
+
+def sph_pair(lon1, lat1, lon2, lat2):
+    """Per-pair great-circle angle and cos 2phi, sin 2phi of angles_sphere
+    (src/utils.f90:618-639) in FP64."""
+    a = [_d(np.atleast_1d(v)) for v in (lon1, lat1, lon2, lat2)]
+    n = len(a[0])
+    d, c2, s2 = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().orc_sph_pair(n, *(_ptr(v, _dp) for v in a), _ptr(d, _dp), _ptr(c2, _dp), _ptr(s2, _dp))
+    return d, c2, s2
 
 
 def finalize(res: dict) -> dict:
@@ -287,16 +310,28 @@ def ref_lib():
         L = C.CDLL(REF_LIB_PATH)
         ci, cf = C.c_int, C.c_float
         L.ref_shear_c.argtypes = [ci] + [_fp] * 5 + [ci] + [_fp] * 7 + [cf, cf, ci, ci, cf, ci, ci] + \
-            [_fp] * 4 + [_i64p] + [_fp] * 5
+            [_fp] * 4 + [_i64p] + [_fp] * 5 + [ci]
         L.ref_kappa_c.argtypes = [ci] + [_fp] * 4 + [ci] + [_fp] * 5 + [cf, cf, ci, ci, cf, ci, ci] + \
-            [_fp] * 4 + [_i64p] + [_fp] * 5
+            [_fp] * 4 + [_i64p] + [_fp] * 5 + [ci]
+        L.ref_sph_pair_c.argtypes = [ci] + [_fp] * 7
         L.ref_bins_c.argtypes = [cf, cf, ci, ci, cf, ci, _fp, _fp, _fp, _i32p]
         _ref_lib = L
     return _ref_lib
 
 
+def ref_sph_pair(lon1, lat1, lon2, lat2):
+    """The reference's own distance_sphere and angles_sphere (FP32) per pair,
+    with cos 2phi = 2 c^2 - 1 and sin 2phi = 2 c s as src/tpcf.f90:341-342."""
+    L = ref_lib()
+    a = [np.ascontiguousarray(np.atleast_1d(v), dtype=np.float32) for v in (lon1, lat1, lon2, lat2)]
+    n = len(a[0])
+    d, c2, s2 = (np.zeros(n, np.float32) for _ in range(3))
+    L.ref_sph_pair_c(n, *(_ptr(v, _fp) for v in a), _ptr(d, _fp), _ptr(c2, _fp), _ptr(s2, _fp))
+    return d, c2, s2
+
+
 def reference(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ihs_theta=1.0,
-              n_threads: int = 1, use_tree: bool = True) -> dict:
+              n_threads: int = 1, use_tree: bool = True, spherical: bool = False) -> dict:
     """Runs the reference's tangential_shear_tpcf (kind 0) or scalar_kappa_tpcf
     (kind 1), src/tpcf.f90:190-545, on FP32 copies of the inputs.  Returns its
     outputs: mean_theta, g_t, g_x, weights, n_pairs and the per-lens marks
@@ -324,9 +359,11 @@ def reference(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ih
     t0 = time.perf_counter()
     if kind == SHEAR:
         L.ref_shear_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "z", "val", "w")), n_b,
-                      *(f(src, k, n_b) for k in ("x", "y", "z", "e1", "e2", "w", "m")), *common, *outs)
+                      *(f(src, k, n_b) for k in ("x", "y", "z", "e1", "e2", "w", "m")), *common, *outs,
+                      1 if spherical else 0)
     else:
         L.ref_kappa_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "val", "w")), n_b,
-                      *(f(src, k, n_b) for k in ("x", "y", "kappa_E", "kappa_B", "w")), *common, *outs)
+                      *(f(src, k, n_b) for k in ("x", "y", "kappa_E", "kappa_B", "w")), *common, *outs,
+                      1 if spherical else 0)
     res["seconds"] = time.perf_counter() - t0
     return res

@@ -10,7 +10,9 @@
 !
 ! Arrays are FP32 (the reference's default real), marks are (n_bin, n_A) in
 ! Fortran order, n_pairs is int64.  use_tree /= 0 selects the kd-tree search,
-! n_threads is passed to the OpenMP region as num_threads.
+! n_threads is passed to the OpenMP region as num_threads, sph /= 0 sets
+! spherical_coords.  ref_sph_pair_c evaluates distance_sphere and
+! angles_sphere (src/utils.f90:512-528, :618-639) for arrays of pairs.
 module ref_binding
     use, intrinsic :: iso_c_binding
     use utils
@@ -20,8 +22,8 @@ contains
 
     subroutine ref_shear_c(n_a, xa, ya, za, va, wa, n_b, xb, yb, zb, e1, e2, wb, mb, &
                            x_min, x_max, n_bin, spacing, p, n_threads, use_tree, &
-                           theta, gt, gx, w, npairs, xie, xib, km, wm, mw) bind(C, name="ref_shear_c")
-        integer(c_int), value :: n_a, n_b, n_bin, spacing, n_threads, use_tree
+                           theta, gt, gx, w, npairs, xie, xib, km, wm, mw, sph) bind(C, name="ref_shear_c")
+        integer(c_int), value :: n_a, n_b, n_bin, spacing, n_threads, use_tree, sph
         real(c_float), value :: x_min, x_max, p
         real(c_float), intent(in) :: xa(n_a), ya(n_a), za(n_a), va(n_a), wa(n_a)
         real(c_float), intent(in) :: xb(n_b), yb(n_b), zb(n_b), e1(n_b), e2(n_b), wb(n_b), mb(n_b)
@@ -31,15 +33,15 @@ contains
         type(binning) :: bins
         call create_bins(x_min=x_min, x_max=x_max, n_bin=n_bin, p=p, spacing=spacing, bins=bins)
         call tangential_shear_tpcf(xa, ya, za, va, wa, xb, yb, zb, e1, e2, wb, mb, bins, &
-                                   theta, gt, gx, w, npairs, .false., int(n_threads), use_tree /= 0, .false., &
+                                   theta, gt, gx, w, npairs, sph /= 0, int(n_threads), use_tree /= 0, .false., &
                                    .true., xie, xib, km, wm, mw)
         call destroy_bins(bins)
     end subroutine ref_shear_c
 
     subroutine ref_kappa_c(n_a, xa, ya, va, wa, n_b, xb, yb, ke, kb, wb, &
                            x_min, x_max, n_bin, spacing, p, n_threads, use_tree, &
-                           theta, gt, gx, w, npairs, xie, xib, km, wm, mw) bind(C, name="ref_kappa_c")
-        integer(c_int), value :: n_a, n_b, n_bin, spacing, n_threads, use_tree
+                           theta, gt, gx, w, npairs, xie, xib, km, wm, mw, sph) bind(C, name="ref_kappa_c")
+        integer(c_int), value :: n_a, n_b, n_bin, spacing, n_threads, use_tree, sph
         real(c_float), value :: x_min, x_max, p
         real(c_float), intent(in) :: xa(n_a), ya(n_a), va(n_a), wa(n_a)
         real(c_float), intent(in) :: xb(n_b), yb(n_b), ke(n_b), kb(n_b), wb(n_b)
@@ -49,7 +51,7 @@ contains
         type(binning) :: bins
         call create_bins(x_min=x_min, x_max=x_max, n_bin=n_bin, p=p, spacing=spacing, bins=bins)
         call scalar_kappa_tpcf(xa, ya, va, wa, xb, yb, ke, kb, wb, bins, &
-                               theta, gt, gx, w, npairs, .false., int(n_threads), use_tree /= 0, .false., &
+                               theta, gt, gx, w, npairs, sph /= 0, int(n_threads), use_tree /= 0, .false., &
                                .true., xie, xib, km, wm, mw)
         call destroy_bins(bins)
     end subroutine ref_kappa_c
@@ -70,5 +72,19 @@ contains
         end do
         call destroy_bins(bins)
     end subroutine ref_bins_c
+
+    subroutine ref_sph_pair_c(n, lon1, lat1, lon2, lat2, d, c2, s2) bind(C, name="ref_sph_pair_c")
+        integer(c_int), value :: n
+        real(c_float), intent(in) :: lon1(n), lat1(n), lon2(n), lat2(n)
+        real(c_float), intent(out) :: d(n), c2(n), s2(n)
+        real :: cs(2)
+        integer :: i
+        do i = 1, n
+            d(i) = distance_sphere(lon1(i), lat1(i), lon2(i), lat2(i))
+            cs = angles_sphere(lon1(i), lat1(i), lon2(i), lat2(i), d(i))
+            c2(i) = 2.0*cs(1)**2 - 1.0
+            s2(i) = 2.0*cs(1)*cs(2)
+        end do
+    end subroutine ref_sph_pair_c
 
 end module ref_binding

@@ -24,6 +24,19 @@
  *                        src/tpcf.f90:1130-1137.
  *   orc_finalize         the normalisation of src/tpcf.f90:369-376 / :536-543
  *                        (mean_theta = Theta/W with no guard, g = G/K where K>0).
+ *   spherical = 1        the spherical_coords branches (src/tpcf.f90:313-320,
+ *                        :337-345): x, y are longitude and latitude in radians,
+ *                        the separation is the great-circle angle and the
+ *                        position angle follows angles_sphere
+ *                        (src/utils.f90:618-639) with FP64 trigonometry.  The
+ *                        bin of a pair is decided on its squared chord
+ *                        |u_A - u_B|^2 against the squared chords 2 sin(e/2) of
+ *                        the edges (the same order as the angle, exact
+ *                        arithmetic); the angle itself is atan2(|u_A x u_B|,
+ *                        u_A . u_B), accurate at every separation (the
+ *                        reference's FP32 acos of distance_sphere,
+ *                        src/utils.f90:512-528, is not).  orc_sph_pair exposes
+ *                        the per-pair angle and cos 2phi, sin 2phi.
  *
  * Arithmetic: coordinates arrive as double (FP32 inputs are widened exactly);
  * dx = x_A - x_B and dy = y_A - y_B are formed in FP64 and the squared
@@ -51,6 +64,7 @@ typedef struct {
     int32_t n_pa, n_pb;
     int32_t brute;                /* 1: every pair; 0: exact cell pruning */
     int32_t n_threads;
+    int32_t spherical;            /* 1: longitude / latitude in radians (always every pair) */
 } orc_args;
 
 typedef struct {
@@ -237,6 +251,101 @@ static inline void visit_pair(const orc_args *a, const double *e2, int64_t i, in
     }
 }
 
+/* Chord 2 sin(e/2) of each angular edge (angles past pi hold no pairs), squared. */
+void orc_chord_edges_squared(const double *edges, int n_bin, double *e2)
+{
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i <= n_bin; ++i) {
+        const double c = 2.0 * sin(0.5 * fmin(edges[i], pi));
+        e2[i] = c * c;
+    }
+}
+
+static inline void unit_vec(double lon, double lat, double *u)
+{
+    u[0] = cos(lat) * cos(lon);
+    u[1] = cos(lat) * sin(lon);
+    u[2] = sin(lat);
+}
+
+/* Great-circle angle between two unit vectors, accurate at every separation. */
+static inline double sph_angle(const double *u, const double *v)
+{
+    const double cx = u[1] * v[2] - u[2] * v[1];
+    const double cy = u[2] * v[0] - u[0] * v[2];
+    const double cz = u[0] * v[1] - u[1] * v[0];
+    return atan2(sqrt(cx * cx + cy * cy + cz * cz), u[0] * v[0] + u[1] * v[1] + u[2] * v[2]);
+}
+
+/* angles_sphere (src/utils.f90:618-639) in FP64 and the spin-2 factors
+ * cos2phi = 2 cos^2 - 1, sin2phi = 2 cos sin (src/tpcf.f90:340-342). */
+static inline void sph_c2s2(double lon1, double lat1, double lon2, double lat2, double d, double *c2, double *s2)
+{
+    const double sin_d = sin(d);
+    const double cos_lat2 = cos(lat2);
+    const double ct = cos_lat2 / sin_d * sin(lon2 - lon1);
+    const double st = (cos(lat1) * sin(lat2) - sin(lat1) * cos_lat2 * cos(lon2 - lon1)) / sin_d;
+    *c2 = 2.0 * ct * ct - 1.0;
+    *s2 = 2.0 * ct * st;
+}
+
+/* Per-pair spherical geometry for the tests: angle, cos 2phi, sin 2phi. */
+void orc_sph_pair(int64_t n, const double *lon1, const double *lat1, const double *lon2, const double *lat2,
+                  double *d, double *c2, double *s2)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double u[3], v[3];
+        unit_vec(lon1[i], lat1[i], u);
+        unit_vec(lon2[i], lat2[i], v);
+        d[i] = sph_angle(u, v);
+        sph_c2s2(lon1[i], lat1[i], lon2[i], lat2[i], d[i], &c2[i], &s2[i]);
+    }
+}
+
+/* One spherical (lens i, source j) pair: src/tpcf.f90:302-358 with the
+ * spherical branches (:313-314, :337-342); kappa: :491-525. */
+static inline void visit_pair_sph(const orc_args *a, const double *e2, int64_t i, int64_t j, const double *ua,
+                                  const double *ub, acc6 *bins, acc6 *pp_row, long double *mk)
+{
+    const int n_bin = a->n_bin;
+    const int shear = (a->kind == ORC_SHEAR);
+    if (shear && (a->za[i] < 0.0 || a->zb[j] < 0.0)) return;
+    if (shear && a->za[i] > a->zb[j]) return;
+    const double *u = ua + 3 * i, *v = ub + 3 * j;
+    const double dx = v[0] - u[0], dy = v[1] - u[1], dz = v[2] - u[2];
+    const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const int b = orc_bin_of_d2(c2, e2, n_bin);
+    if (b < 0) return;
+    const double d = sph_angle(u, v);
+    const double wA = a->wa[i], valA = a->vala[i], wB = a->wb[j];
+    double t, x, kw;
+    if (shear) {
+        double cc, ss;
+        sph_c2s2(a->xa[i], a->ya[i], a->xb[j], a->yb[j], d, &cc, &ss);
+        const double e1 = a->s1b[j], e2v = a->s2b[j], m = a->mb[j];
+        t = -(e1 * cc + e2v * ss) * wB * valA * wA;
+        x = -(-e1 * ss + e2v * cc) * wB * valA * wA;
+        kw = wB * (1.0 + m) * wA;
+    } else {
+        t = a->s1b[j] * wB * valA * wA;
+        x = a->s2b[j] * wB * valA * wA;
+        kw = wB * wA;
+    }
+    const double ww = wB * wA;
+    acc6 *ab = &bins[b];
+    ab->theta += d * wB * wA; ab->gt += t; ab->gx += x; ab->w += ww; ab->k += kw; ab->n += 1;
+    ab->gt_abs += fabs(t); ab->gx_abs += fabs(x);
+    if (pp_row) {
+        const int32_t pb = a->patch_b ? a->patch_b[j] : 0;
+        acc6 *ap = &pp_row[(size_t)pb * n_bin + b];
+        ap->theta += d * wB * wA; ap->gt += t; ap->gx += x; ap->w += ww; ap->k += kw; ap->n += 1;
+    }
+    if (mk) {
+        long double *m4 = mk + (size_t)4 * b;
+        m4[0] += t; m4[1] += x; m4[2] += kw; m4[3] += ww;
+    }
+}
+
 int orc_correlate(const orc_args *a, orc_out *o)
 {
     const int n_bin = a->n_bin;
@@ -247,11 +356,23 @@ int orc_correlate(const orc_args *a, orc_out *o)
     const int32_t n_pb = want_pp ? (a->n_pb > 0 ? a->n_pb : 1) : 1;
     double *e2 = (double *)malloc(sizeof(double) * (size_t)(n_bin + 1));
     if (!e2) return 2;
-    orc_edges_squared(a->edges, n_bin, e2);
+    const int sph = a->spherical != 0;
+    const int brute = a->brute || sph;
+    double *ua = NULL, *ub = NULL;
+    if (sph) {
+        orc_chord_edges_squared(a->edges, n_bin, e2);
+        ua = (double *)malloc(sizeof(double) * 3 * (size_t)(a->n_a > 0 ? a->n_a : 1));
+        ub = (double *)malloc(sizeof(double) * 3 * (size_t)(a->n_b > 0 ? a->n_b : 1));
+        if (!ua || !ub) { free(ua); free(ub); free(e2); return 2; }
+        for (int64_t i = 0; i < a->n_a; ++i) unit_vec(a->xa[i], a->ya[i], ua + 3 * i);
+        for (int64_t j = 0; j < a->n_b; ++j) unit_vec(a->xb[j], a->yb[j], ub + 3 * j);
+    } else {
+        orc_edges_squared(a->edges, n_bin, e2);
+    }
 
     orc_grid g;
     memset(&g, 0, sizeof g);
-    if (!a->brute && grid_build(&g, a, a->edges[n_bin])) { free(e2); grid_free(&g); return 2; }
+    if (!brute && grid_build(&g, a, a->edges[n_bin])) { free(e2); grid_free(&g); return 2; }
 
     int nt = a->n_threads > 0 ? a->n_threads : 1;
 #ifndef _OPENMP
@@ -286,7 +407,9 @@ int orc_correlate(const orc_args *a, orc_out *o)
                 memset(mk, 0, sizeof(long double) * 4 * (size_t)n_bin);
             }
             if (a->wa[i] != 0.0) {                     /* src/tpcf.f90:285 / :474 */
-                if (a->brute) {
+                if (sph) {
+                    for (int64_t j = 0; j < a->n_b; ++j) visit_pair_sph(a, e2, i, j, ua, ub, bins, pp_row, mk);
+                } else if (brute) {
                     for (int64_t j = 0; j < a->n_b; ++j) visit_pair(a, e2, i, j, bins, pp_row, mk);
                 } else {
                     int64_t cx = (int64_t)floor((a->xa[i] - g.x0) / g.cs);
@@ -343,8 +466,8 @@ int orc_correlate(const orc_args *a, orc_out *o)
                     }
         }
     }
-    free(tbins); free(tpp); free(tmk); free(e2);
-    if (!a->brute) grid_free(&g);
+    free(tbins); free(tpp); free(tmk); free(e2); free(ua); free(ub);
+    if (!brute) grid_free(&g);
     return fail ? 2 : 0;
 }
 
