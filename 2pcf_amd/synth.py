@@ -87,11 +87,12 @@ class Workload:
     patch_A: np.ndarray | None
     patch_B: np.ndarray | None
     auto: bool = False
+    spherical: bool = False   # x, y = longitude, latitude and x_min, x_max in radians
 
 
 def make_workload(name: str, n_a: int, n_b: int, side: float, kind: int, x_min: float, x_max: float, n_bin: int,
                   n_patch: int, seed: int, clustered: bool = False, npix: int = 2048, fwhm: float = 10.0,
-                  auto: bool = False, precision=np.float32) -> Workload:
+                  auto: bool = False, precision=np.float32, spherical: bool = False) -> Workload:
     rng = np.random.default_rng(seed)
     dens = lognormal_density(rng, npix, side, fwhm) if clustered else None
     xa, ya = sample_positions(rng, n_a, side, dens)
@@ -116,10 +117,18 @@ def make_workload(name: str, n_a: int, n_b: int, side: float, kind: int, x_min: 
         cen = kmeans_centroids(rng, xa, ya, n_patch)
         pa = assign(xa, ya, cen)
         pb = pa if auto else assign(src["x"], src["y"], cen)
+    if spherical:
+        # the patch laid on the sphere around (longitude pi, latitude 0): one
+        # arcminute of x or y is one arcminute of longitude or latitude
+        rad = np.pi / (180.0 * 60.0)
+        for d in (lens, src):
+            d["x"] = (np.pi + (d["x"].astype(np.float64) - side / 2) * rad).astype(np.float32)
+            d["y"] = ((d["y"].astype(np.float64) - side / 2) * rad).astype(np.float32)
+        x_min, x_max = x_min * rad, x_max * rad
     if precision != np.float32:
         lens = {k: v.astype(precision) for k, v in lens.items()}
         src = {k: v.astype(precision) for k, v in src.items()}
-    return Workload(name, kind, lens, src, x_min, x_max, n_bin, n_patch, pa, pb, auto)
+    return Workload(name, kind, lens, src, x_min, x_max, n_bin, n_patch, pa, pb, auto, spherical)
 
 
 # The benchmark configurations (sizes, patch sides in arcmin, bins, seeds).
@@ -133,6 +142,9 @@ CONFIGS = {
                                      n_bin=30, n_patch=128, seed=1003, clustered=True, npix=4096),
     "sk_5Mx5M_uniform_f32": dict(n_a=5_000_000, n_b=5_000_000, side=1200.0, kind=0, x_min=1.0, x_max=200.0,
                                  n_bin=20, n_patch=1, seed=1005),
+    # the 20M x 20M field on the sphere (50 x 50 degrees around longitude pi, latitude 0)
+    "sk_20Mx20M_sphere": dict(n_a=20_000_000, n_b=20_000_000, side=3000.0, kind=0, x_min=0.5, x_max=250.0,
+                              n_bin=30, n_patch=128, seed=1003, clustered=True, npix=4096, spherical=True),
 }
 
 

@@ -33,6 +33,10 @@ sys.path.insert(0, ROOT)
 
 FLOPS_PER_PAIR = {0: 33, 1: 17}     # shear-scalar, kappa-kappa (algorithmic count)
 FP32_PEAK_TFLOPS = 157.3            # MI355X published FP32 vector peak
+# spherical geometry (FP64): chord 3 + 5, bin 3, east / north 10, 1/sin^2 3 + 1,
+# cos / sin 2phi 4, projections 6, angle (sqrt + series) 16, weights and sums 11
+FLOPS_PER_PAIR_SPH = {0: 62, 1: 38}
+FP64_PEAK_TFLOPS = 78.6             # MI355X published FP64 vector peak
 METRIC = "pairs/sec on 1 MI355X vs host-core ref; FP32+FP64-edge vs FP64 at ≤1e-6 rel"
 
 
@@ -58,7 +62,8 @@ def cpu_baseline(wl, pairs_per_lens: float, budget_s: float, threads: int) -> di
     def run(sub):
         t0 = time.perf_counter()
         if use_ref:
-            r = oracle.reference(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
+            r = oracle.reference(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads,
+                                 spherical=wl.spherical)
             r["tree_seconds"] = 0.0
         else:
             r = oracle.ref_port(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
@@ -88,6 +93,12 @@ def cpu_baseline(wl, pairs_per_lens: float, budget_s: float, threads: int) -> di
             "tree_build_s": fixed,
             "code": ("reference tangential_shear_tpcf/scalar_kappa_tpcf (flang -O2 -fopenmp)" if use_ref
                      else "C port of the reference algorithm (gcc -O2 -fopenmp)")}
+
+
+def _have_ref() -> bool:
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle   # checker family: the CPU baseline only
+    return oracle.have_reference()
 
 
 def profiled_traffic(workload: str):
@@ -300,7 +311,7 @@ def main() -> int:
                             patch_A=pa.data_ptr() if pa is not None else None,
                             patch_B=pb.data_ptr() if pb is not None else None,
                             n_patch_A=wl.n_patch, n_patch_B=wl.n_patch, real_bytes=4, device_inputs=True,
-                            ctx=ctx, n_A=len(wl.lens["x"]), n_B=len(wl.src["x"]))
+                            ctx=ctx, n_A=len(wl.lens["x"]), n_B=len(wl.src["x"]), spherical=wl.spherical)
 
     for i in range(args.warmup):
         r = step()
@@ -320,26 +331,29 @@ def main() -> int:
     value = pairs / elapsed
     per_step_pairs = pairs / args.steps
     k_ms = float(np.mean(kernel_ms))
-    achieved = per_step_pairs * FLOPS_PER_PAIR[wl.kind] / (k_ms * 1e-3) / 1e12
+    fpp = (FLOPS_PER_PAIR_SPH if wl.spherical else FLOPS_PER_PAIR)[wl.kind]
+    peak = FP64_PEAK_TFLOPS if wl.spherical else FP32_PEAK_TFLOPS
+    achieved = per_step_pairs * fpp / (k_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": prec, "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if wl.spherical else prec, "data": "synthetic",
         "config": {"workload": wl.name, "n_lens": len(wl.lens["x"]), "n_source": len(wl.src["x"]),
                    "n_bin": wl.n_bin, "theta": [wl.x_min, wl.x_max], "patches": wl.n_patch,
                    "kind": "shear-scalar" if wl.kind == 0 else "kappa-kappa", "scale": args.scale,
                    "pairs_per_step": per_step_pairs, "cell_size": stats["cell_size"],
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": FLOPS_PER_PAIR[wl.kind]},
+                   "spherical": wl.spherical, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": fpp},
     }
     if args.scale == 1.0 and args.cell == 0.0:
         traffic, src = profiled_traffic(wl.name)
         if traffic is not None:
             out["roofline"]["traffic"] = traffic
             out["roofline"]["traffic_source"] = f"{src} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
-    if not args.no_cpu:
+    # the C port has no spherical mode: the spherical CPU baseline needs the reference build
+    if not args.no_cpu and (not wl.spherical or _have_ref()):
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         ppl = per_step_pairs / max(stats["n_lens_used"], 1)
         out["cpu_baseline"] = cpu_baseline(wl, ppl, args.cpu_seconds, threads)
