@@ -39,6 +39,7 @@ EXPORTS = (
     "tpcf_shear_f32", "tpcf_kappa_f32", "tpcf_shear_f64", "tpcf_kappa_f64",
     "tpcf_adjoint_shear_f32", "tpcf_adjoint_shear_f64", "tpcf_adjoint_kappa_f32", "tpcf_adjoint_kappa_f64",
     "tpcf_batch_create_f32", "tpcf_batch_create_f64", "tpcf_batch_run", "tpcf_batch_get_stats", "tpcf_batch_destroy",
+    "tpcf_bootstrap",
 )
 
 
@@ -113,6 +114,11 @@ class BatchStats(C.Structure):
         return d
 
 
+class Blocks(C.Structure):
+    _fields_ = [("n_block", C.c_int32), ("n_bin", C.c_int32), ("g_t", C.c_void_p), ("g_x", C.c_void_p),
+                ("k", C.c_void_p), ("weight", C.c_void_p)]
+
+
 _lib = None
 _vp = C.c_void_p
 _i64 = C.c_int64
@@ -169,5 +175,7 @@ def lib():
     L.tpcf_batch_get_stats.restype = C.c_int
     L.tpcf_batch_destroy.argtypes = [_vp]
     L.tpcf_batch_destroy.restype = C.c_int
+    L.tpcf_bootstrap.argtypes = [_vp, C.POINTER(Blocks), _i32, _vp, C.c_uint64, C.c_double, C.c_uint32] + [_vp] * 6
+    L.tpcf_bootstrap.restype = C.c_int
     _lib = L
     return L

@@ -390,6 +390,58 @@ def adjoint(kind: int, lens: dict, src: dict, bins: Binning, gbar1, gbar2, *, pr
     return out if not device_inputs else {}
 
 
+# ------------------------------------------------------------ bootstrap ----
+def block_sums(res: "Shear2pcfResults", patch_A, w_A, n_patch_A: int) -> dict:
+    """Per lens patch ("block") sums of a correlation with per-patch-pair
+    outputs: G_t, G_x, K [n_block][n_bin] (the patch-pair sums over source
+    patches, which equal the block sums of the marks, src/tpcf.f90:1130-1137)
+    and the block weights (sum of lens w).  Empty blocks (weight 0) are
+    dropped, as src/tpcf.f90:1054-1062 does."""
+    gt = np.asarray(res.pp_gt).sum(axis=0)
+    gx = np.asarray(res.pp_gx).sum(axis=0)
+    k = np.asarray(res.pp_k).sum(axis=0)
+    w = np.bincount(np.asarray(patch_A), weights=np.asarray(w_A, dtype=np.float64), minlength=n_patch_A)
+    keep = w > 0
+    return dict(g_t=gt[keep], g_x=gx[keep], k=k[keep], weight=w[keep])
+
+
+def bootstrap(blocks: dict, n_boot: int, *, multiplicity=None, seed: int = 0, supersampling: float = 1.0,
+              covariance: bool = True, ctx: Optional[Context] = None) -> dict:
+    """Block bootstrap on the GPU (tpcf_bootstrap): the resampled g_t, g_x
+    [n_boot][n_bin], weights [n_boot], the multiplicities [n_boot][n_block] and
+    the covariances of g_t and g_x [n_bin][n_bin] (src/tpcf.f90:1165-1191,
+    src/utils.f90:745-764).  ``blocks``: g_t, g_x, k [n_block][n_bin] and
+    optional weight [n_block] (see block_sums)."""
+    ctx = ctx or default_context()
+    gt = np.ascontiguousarray(blocks["g_t"], dtype=np.float64)
+    gx = np.ascontiguousarray(blocks["g_x"], dtype=np.float64)
+    k = np.ascontiguousarray(blocks["k"], dtype=np.float64)
+    w = blocks.get("weight")
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    P, nb = gt.shape
+    b = N.Blocks()
+    b.n_block, b.n_bin = P, nb
+    b.g_t, b.g_x, b.k = gt.ctypes.data, gx.ctypes.data, k.ctypes.data
+    b.weight = w.ctypes.data if w is not None else None
+    m_in = None
+    if multiplicity is not None:
+        m_in = np.ascontiguousarray(multiplicity, dtype=np.int32)
+        if m_in.shape != (n_boot, P):
+            raise ValueError("multiplicity must be [n_boot][n_block]")
+    out = dict(g_t=np.zeros((n_boot, nb)), g_x=np.zeros((n_boot, nb)), weights=np.zeros(n_boot),
+               multiplicity=np.zeros((n_boot, P), dtype=np.int32))
+    if covariance:
+        out["cov_t"] = np.zeros((nb, nb))
+        out["cov_x"] = np.zeros((nb, nb))
+    ptr = lambda a: None if a is None else a.ctypes.data   # noqa: E731
+    rc = N.lib().tpcf_bootstrap(ctx.handle, C.byref(b), int(n_boot), ptr(m_in), int(seed) & ((1 << 64) - 1),
+                                float(supersampling), 0, ptr(out["g_t"]), ptr(out["g_x"]), ptr(out["weights"]),
+                                ptr(out.get("cov_t")), ptr(out.get("cov_x")), ptr(out["multiplicity"]))
+    if rc:
+        raise TpcfError(rc, ctx.last_error())
+    return out
+
+
 # ---------------------------------------------------------------- batch ----
 class Batch:
     """Many catalogues run as one HIP graph (tpcf_batch_*).

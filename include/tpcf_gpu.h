@@ -277,6 +277,35 @@ int tpcf_batch_run(tpcf_batch *batch);
 int tpcf_batch_get_stats(const tpcf_batch *batch, tpcf_batch_stats *stats);
 int tpcf_batch_destroy(tpcf_batch *batch);
 
+/* ---- block bootstrap and covariance ------------------------------------------
+ * Replaces the resampling loop of survey_marked_point_block_bootstrap
+ * (src/tpcf.f90:1165-1191; survey_marked_point_bootstrap :1230-1262 is the
+ * same with one object per block) and covariance_matrix (src/utils.f90:745-764)
+ * as the driver applies it to the resampled g_t and g_x (src/main.f90:524-537).
+ *
+ * Blocks are lens patches: g_t, g_x, k hold each block's per-bin sums of the
+ * G_t, G_x and K marks over its lenses (the per-patch-pair sums of a
+ * correlation call summed over source patches), weight its lenses' summed w.
+ * Sample i with block multiplicities M[i][p] gives
+ *   g_t[i][b] = sum_p M g_t[p][b] / sum_p M k[p][b]  (0 where that K <= 0),
+ *   g_x likewise, weights[i] = sum_p M weight[p]
+ * and cov_t[a][b] = sum_i (g_t[i][a] - mean_a)(g_t[i][b] - mean_b) / (n_boot - 1)
+ * (cov_x likewise).  M comes from `multiplicity` ([n_boot][n_block] int32) or,
+ * when it is NULL, from n_draw = int(n_block * supersampling) uniform draws of a
+ * block per sample, from a counter-based generator keyed by `seed` (written to
+ * `multiplicity_out` when that is not NULL).  With TPCF_DEVICE_INPUTS every
+ * array argument is a device address, else a host address.  FP64 throughout;
+ * n_bin <= TPCF_MAX_BIN. */
+typedef struct tpcf_blocks {
+    int32_t n_block, n_bin;
+    const double *g_t, *g_x, *k;   /* [n_block][n_bin] */
+    const double *weight;          /* [n_block], nullable */
+} tpcf_blocks;
+
+int tpcf_bootstrap(tpcf_ctx *ctx, const tpcf_blocks *blocks, int32_t n_boot, const int32_t *multiplicity,
+                   uint64_t seed, double supersampling, uint32_t flags, double *g_t, double *g_x,
+                   double *weights, double *cov_t, double *cov_x, int32_t *multiplicity_out);
+
 #ifdef __cplusplus
 }
 #endif

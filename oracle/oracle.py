@@ -314,9 +314,21 @@ def ref_lib():
         L.ref_kappa_c.argtypes = [ci] + [_fp] * 4 + [ci] + [_fp] * 5 + [cf, cf, ci, ci, cf, ci, ci] + \
             [_fp] * 4 + [_i64p] + [_fp] * 5 + [ci]
         L.ref_sph_pair_c.argtypes = [ci] + [_fp] * 7
+        L.ref_covariance_c.argtypes = [ci, ci, _fp, _fp]
         L.ref_bins_c.argtypes = [cf, cf, ci, ci, cf, ci, _fp, _fp, _fp, _i32p]
         _ref_lib = L
     return _ref_lib
+
+
+def ref_covariance(x: np.ndarray) -> np.ndarray:
+    """The reference's covariance_matrix (src/utils.f90:745-764, FP32) of
+    samples x [n_samples][n_data]."""
+    L = ref_lib()
+    xf = np.asfortranarray(x, dtype=np.float32)
+    n, d = xf.shape
+    cov = np.zeros((d, d), dtype=np.float32, order="F")
+    L.ref_covariance_c(n, d, _ptr(xf, _fp), _ptr(cov, _fp))
+    return np.ascontiguousarray(cov)
 
 
 def ref_sph_pair(lon1, lat1, lon2, lat2):
@@ -367,3 +379,37 @@ def reference(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ih
                       1 if spherical else 0)
     res["seconds"] = time.perf_counter() - t0
     return res
+
+
+def bootstrap(blocks: dict, multiplicity: np.ndarray) -> dict:
+    """The resampling of survey_marked_point_block_bootstrap
+    (src/tpcf.f90:1165-1191) for given multiplicities M [n_boot][n_block]:
+    g = (M . G) / (M . K) where M . K > 0, else 0; weights = M . w.  Sums in
+    block order, FP64."""
+    M = np.asarray(multiplicity, dtype=np.float64)
+    gt, gx, k = (np.asarray(blocks[key], dtype=np.float64) for key in ("g_t", "g_x", "k"))
+    st = np.zeros((M.shape[0], gt.shape[1]))
+    sx = np.zeros_like(st)
+    sk = np.zeros_like(st)
+    for p in range(M.shape[1]):          # block order, as the resampling loop
+        m = M[:, p:p + 1]
+        st += m * gt[p]
+        sx += m * gx[p]
+        sk += m * k[p]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        g_t = np.where(sk > 0, st / np.where(sk > 0, sk, 1), 0.0)
+        g_x = np.where(sk > 0, sx / np.where(sk > 0, sk, 1), 0.0)
+    w = blocks.get("weight")
+    weights = M @ np.asarray(w, dtype=np.float64) if w is not None else np.zeros(M.shape[0])
+    return dict(g_t=g_t, g_x=g_x, weights=weights)
+
+
+def covariance(x: np.ndarray) -> np.ndarray:
+    """covariance_matrix (src/utils.f90:745-764) in FP64: column means, then
+    sum_i (x_ia - mean_a)(x_ib - mean_b) / (n - 1)."""
+    x = np.asarray(x, dtype=np.float64)
+    n = x.shape[0]
+    if n <= 1:
+        return np.zeros((x.shape[1], x.shape[1]))
+    d = x - x.mean(axis=0)
+    return d.T @ d / (n - 1)

@@ -73,6 +73,13 @@ contains
         call destroy_bins(bins)
     end subroutine ref_bins_c
 
+    subroutine ref_covariance_c(n_samples, n_data, x, cov) bind(C, name="ref_covariance_c")
+        integer(c_int), value :: n_samples, n_data
+        real(c_float), intent(in) :: x(n_samples, n_data)
+        real(c_float), intent(out) :: cov(n_data, n_data)
+        call covariance_matrix(x, cov)
+    end subroutine ref_covariance_c
+
     subroutine ref_sph_pair_c(n, lon1, lat1, lon2, lat2, d, c2, s2) bind(C, name="ref_sph_pair_c")
         integer(c_int), value :: n
         real(c_float), intent(in) :: lon1(n), lat1(n), lon2(n), lat2(n)
