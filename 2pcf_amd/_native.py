@@ -39,7 +39,7 @@ EXPORTS = (
     "tpcf_shear_f32", "tpcf_kappa_f32", "tpcf_shear_f64", "tpcf_kappa_f64",
     "tpcf_adjoint_shear_f32", "tpcf_adjoint_shear_f64", "tpcf_adjoint_kappa_f32", "tpcf_adjoint_kappa_f64",
     "tpcf_batch_create_f32", "tpcf_batch_create_f64", "tpcf_batch_run", "tpcf_batch_get_stats", "tpcf_batch_destroy",
-    "tpcf_bootstrap",
+    "tpcf_bootstrap", "tpcf_kmeans_f32", "tpcf_kmeans_f64",
 )
 
 
@@ -114,6 +114,11 @@ class BatchStats(C.Structure):
         return d
 
 
+class KmeansOpts(C.Structure):
+    _fields_ = [("k", C.c_int32), ("max_iterations", C.c_int32), ("init", C.c_int32), ("pad", C.c_int32),
+                ("reassignment_tol", C.c_double), ("weight_equalizer_scaling", C.c_double)]
+
+
 class Blocks(C.Structure):
     _fields_ = [("n_block", C.c_int32), ("n_bin", C.c_int32), ("g_t", C.c_void_p), ("g_x", C.c_void_p),
                 ("k", C.c_void_p), ("weight", C.c_void_p)]
@@ -177,5 +182,10 @@ def lib():
     L.tpcf_batch_destroy.restype = C.c_int
     L.tpcf_bootstrap.argtypes = [_vp, C.POINTER(Blocks), _i32, _vp, C.c_uint64, C.c_double, C.c_uint32] + [_vp] * 6
     L.tpcf_bootstrap.restype = C.c_int
+    for name in ("tpcf_kmeans_f32", "tpcf_kmeans_f64"):
+        f = getattr(L, name)
+        f.argtypes = [_vp, _i64, _vp, _vp, _vp, C.POINTER(KmeansOpts), C.c_uint32, _vp, _vp, _vp,
+                      C.POINTER(_i32), C.POINTER(_i64)]
+        f.restype = C.c_int
     _lib = L
     return L

@@ -200,7 +200,7 @@ struct tpcf_ctx {
         B_GRAD_A, B_GRAD_B, B_ONES, B_ZEROS,
         // spherical geometry: FP64 inputs, projected coordinates, lens frames, sorted extras
         B_SPH_IN, B_SPH_LP, B_SPH_LF, B_SPH_SEGP, B_SPH_CHP, B_SPH_PART,
-        B_BOOT,
+        B_BOOT, B_KMEANS,
         B_COUNT
     };
     tpcf::DevBuf buf[B_COUNT];

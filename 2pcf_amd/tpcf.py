@@ -442,6 +442,34 @@ def bootstrap(blocks: dict, n_boot: int, *, multiplicity=None, seed: int = 0, su
     return out
 
 
+# --------------------------------------------------------------- k-means ----
+def kmeans(x, y, w, k: int, *, max_iterations: int = 1000, reassignment_tol: float = 0.0, centroids=None,
+           spherical: bool = False, precision: Optional[str] = None, ctx: Optional[Context] = None) -> dict:
+    """k-means patch labels on the GPU (tpcf_kmeans_*): partition_data
+    (src/kmeans.f90:12-263) in its Voronoi form.  Returns 0-based labels,
+    centroids [k][2], cluster weights, the rounds run and the points moved in
+    the last round.  ``centroids`` (k x 2) replaces the fixed starting grid."""
+    ctx = ctx or default_context()
+    prec = precision or _precision_of(x)
+    dt = np.float32 if prec == "f32" else np.float64
+    xs, ys, ws = (np.ascontiguousarray(v, dtype=dt) for v in (x, y, w))
+    n = len(xs)
+    o = N.KmeansOpts()
+    o.k, o.max_iterations, o.reassignment_tol = int(k), int(max_iterations), float(reassignment_tol)
+    o.init = 0 if centroids is None else 1
+    cen = np.zeros((k, 2)) if centroids is None else np.array(centroids, dtype=np.float64).reshape(k, 2)
+    lab = np.zeros(n, dtype=np.int32)
+    cw = np.zeros(k)
+    rounds, moves = C.c_int32(0), C.c_int64(0)
+    fn = N.lib().tpcf_kmeans_f32 if prec == "f32" else N.lib().tpcf_kmeans_f64
+    rc = fn(ctx.handle, n, xs.ctypes.data, ys.ctypes.data, ws.ctypes.data, C.byref(o),
+            N.FLAG_SPHERICAL if spherical else 0, lab.ctypes.data, cen.ctypes.data, cw.ctypes.data,
+            C.byref(rounds), C.byref(moves))
+    if rc:
+        raise TpcfError(rc, ctx.last_error())
+    return dict(labels=lab, centroids=cen, cluster_weights=cw, iterations=rounds.value, last_moves=moves.value)
+
+
 # ---------------------------------------------------------------- batch ----
 class Batch:
     """Many catalogues run as one HIP graph (tpcf_batch_*).

@@ -306,6 +306,35 @@ int tpcf_bootstrap(tpcf_ctx *ctx, const tpcf_blocks *blocks, int32_t n_boot, con
                    uint64_t seed, double supersampling, uint32_t flags, double *g_t, double *g_x,
                    double *weights, double *cov_t, double *cov_x, int32_t *multiplicity_out);
 
+/* ---- k-means patch labels ----------------------------------------------------
+ * Replaces partition_data (src/kmeans.f90:12-263) in its Voronoi form
+ * (equal_size = .false., weight_equalizer_scaling = 0: the voronoi_blocks
+ * scheme of src/tpcf.f90:1100-1113): rounds of nearest-centroid assignment
+ * (first minimum kept) and weighted-mean centroids until a round moves no
+ * point, moves fewer than reassignment_tol * n, or max_iterations + 1 rounds
+ * have run.  init 0: the fixed starting grid of
+ * kmeans_compute_fixed_initial_centroids (src/kmeans.f90:85-110), labels
+ * int(i/n k); init 1: `centroids` holds the k starting centroids.  labels are
+ * 0-based.  TPCF_SPHERICAL: longitude / latitude in radians, great-circle
+ * distances.  A non-zero weight_equalizer_scaling returns TPCF_ERR_UNSUPPORTED
+ * (the reference moves cluster weights point by point, so that form depends
+ * on the visiting order).  2 <= k <= 1024; FP64 arithmetic. */
+typedef struct tpcf_kmeans_opts {
+    int32_t k;
+    int32_t max_iterations;            /* the reference's default: 1000 */
+    int32_t init;                      /* 0: fixed grid, 1: given centroids */
+    int32_t pad;
+    double reassignment_tol;           /* stop when moves / n < tol (0: run to convergence) */
+    double weight_equalizer_scaling;   /* must be 0 */
+} tpcf_kmeans_opts;
+
+int tpcf_kmeans_f32(tpcf_ctx *ctx, int64_t n, const float *x, const float *y, const float *w,
+                    const tpcf_kmeans_opts *opts, uint32_t flags, int32_t *labels, double *centroids,
+                    double *cluster_weights, int32_t *rounds, int64_t *last_moves);
+int tpcf_kmeans_f64(tpcf_ctx *ctx, int64_t n, const double *x, const double *y, const double *w,
+                    const tpcf_kmeans_opts *opts, uint32_t flags, int32_t *labels, double *centroids,
+                    double *cluster_weights, int32_t *rounds, int64_t *last_moves);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,9 +4,10 @@
 #
 #   ref_harness      src/constants.f90, src/utils.f90, src/kdtree.f90 (unmodified)
 #                    + oracle/ref_harness.f90 (binning and kd-tree vectors)
-#   libref2pcf.so    the same modules + the correlation subroutines
-#                    tangential_shear_tpcf and scalar_kappa_tpcf of src/tpcf.f90
-#                    + oracle/ref_binding.f90 (bind(C) entry points), OpenMP on
+#   libref2pcf.so    the same modules + src/kmeans.f90 (unmodified) + the
+#                    correlation subroutines tangential_shear_tpcf and
+#                    scalar_kappa_tpcf of src/tpcf.f90 + oracle/ref_binding.f90
+#                    (bind(C) entry points), OpenMP on
 #
 # src/tpcf.f90 as a whole does not parse with flang (line 1160: the I/O list
 # "((g_t(j), g_x(j), block_weight(i), 0), j=1,bins%n_bin)" in the bootstrap's
@@ -37,6 +38,7 @@ FFLAGS="-O2 -fPIC -fopenmp"
 "$FC" $FFLAGS -c "$REF/src/constants.f90" -o constants.o
 "$FC" $FFLAGS -c "$REF/src/utils.f90" -o utils.o
 "$FC" $FFLAGS -c "$REF/src/kdtree.f90" -o kdtree.o
+"$FC" $FFLAGS -c "$REF/src/kmeans.f90" -o kmeans.o
 "$FC" $FFLAGS -c "$HERE/ref_harness.f90" -o ref_harness.o
 "$FC" $FFLAGS -o "$OUT/ref_harness" ref_harness.o kdtree.o utils.o constants.o
 
@@ -52,5 +54,5 @@ FFLAGS="-O2 -fPIC -fopenmp"
 } > ref_tpcf_pairs.f90
 "$FC" $FFLAGS -c ref_tpcf_pairs.f90 -o ref_tpcf_pairs.o
 "$FC" $FFLAGS -c "$HERE/ref_binding.f90" -o ref_binding.o
-"$FC" $FFLAGS -shared -o "$OUT/libref2pcf.so" ref_binding.o ref_tpcf_pairs.o kdtree.o utils.o constants.o
+"$FC" $FFLAGS -shared -o "$OUT/libref2pcf.so" ref_binding.o ref_tpcf_pairs.o kmeans.o kdtree.o utils.o constants.o
 echo "build_ref: $OUT/ref_harness $OUT/libref2pcf.so"

@@ -103,6 +103,9 @@ def lib():
         _lib.orc_correlate.restype = C.c_int
         _lib.orc_finalize.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib.orc_chord_edges_squared.argtypes = [_dp, C.c_int, _dp]
+        _lib.orc_kmeans.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                    C.c_int, C.c_int, _i32p, _dp, _dp, C.POINTER(C.c_int64)]
+        _lib.orc_kmeans.restype = C.c_int
         _lib.orc_sph_pair.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib.rp_create_bins.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.POINTER(_RpBins)]
         _lib.rp_create_bins.restype = C.c_int
@@ -315,6 +318,7 @@ def ref_lib():
             [_fp] * 4 + [_i64p] + [_fp] * 5 + [ci]
         L.ref_sph_pair_c.argtypes = [ci] + [_fp] * 7
         L.ref_covariance_c.argtypes = [ci, ci, _fp, _fp]
+        L.ref_kmeans_c.argtypes = [ci, _fp, _fp, _fp, ci, ci, cf, ci, cf, cf, _i32p, _fp]
         L.ref_bins_c.argtypes = [cf, cf, ci, ci, cf, ci, _fp, _fp, _fp, _i32p]
         _ref_lib = L
     return _ref_lib
@@ -413,3 +417,35 @@ def covariance(x: np.ndarray) -> np.ndarray:
         return np.zeros((x.shape[1], x.shape[1]))
     d = x - x.mean(axis=0)
     return d.T @ d / (n - 1)
+
+
+def kmeans(x, y, w, k: int, max_iterations: int = 1000, reassignment_tol: float = 0.0, scaling: float = 0.0,
+           jacobi: bool = False, spherical: bool = False, centroids=None) -> dict:
+    """partition_data (src/kmeans.f90:12-263) restated in FP64 (see
+    orc_kmeans): 0-based labels, centroids [k][2], cluster weights, rounds."""
+    x, y, w = _d(x), _d(y), _d(w)
+    n = len(x)
+    lab = np.zeros(n, dtype=np.int32)
+    cen = np.zeros((k, 2)) if centroids is None else np.array(centroids, dtype=np.float64)
+    cw = np.zeros(k)
+    moves = C.c_int64(0)
+    it = lib().orc_kmeans(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(w, _dp), k, max_iterations, reassignment_tol, scaling,
+                          1 if jacobi else 0, 1 if spherical else 0, 0 if centroids is None else 1,
+                          _ptr(lab, _i32p), _ptr(cen, _dp), _ptr(cw, _dp), C.byref(moves))
+    if it < 0:
+        raise ValueError("k-means needs n >= 1 and k >= 2")
+    return dict(labels=lab, centroids=cen, cluster_weights=cw, iterations=it, last_moves=moves.value)
+
+
+def ref_kmeans(x, y, w, k: int, max_iterations: int = 1000, reassignment_tol: float = 0.0, scaling: float = 0.0,
+               equal_size: bool = False, weight_tol: float = 0.0) -> dict:
+    """The reference's own partition_data (src/kmeans.f90:12-263, FP32, one
+    thread, fixed initial centroids, Euclidean distance): 0-based labels."""
+    L = ref_lib()
+    xf, yf, wf = (np.ascontiguousarray(v, dtype=np.float32) for v in (x, y, w))
+    n = len(xf)
+    lab = np.zeros(n, dtype=np.int32)
+    cen = np.zeros((2, k), dtype=np.float32)   # Fortran (k, 2) storage
+    L.ref_kmeans_c(n, _ptr(xf, _fp), _ptr(yf, _fp), _ptr(wf, _fp), k, 1 if equal_size else 0, float(scaling),
+                   int(max_iterations), float(reassignment_tol), float(weight_tol), _ptr(lab, _i32p), _ptr(cen, _fp))
+    return dict(labels=lab - 1, centroids=cen.T.copy())

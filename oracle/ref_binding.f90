@@ -16,9 +16,37 @@
 module ref_binding
     use, intrinsic :: iso_c_binding
     use utils
+    use kmeans_module
     use ref_tpcf_pairs
     implicit none
 contains
+
+    ! distance_euclidean is elemental, which may not be passed as an actual
+    ! argument: this wrapper has the interface partition_data expects.
+    function ref_euclid(x1, y1, x2, y2) result(d)
+        real, intent(in) :: x1, y1, x2, y2
+        real :: d
+        d = distance_euclidean(x1, y1, x2, y2)
+    end function ref_euclid
+
+    ! partition_data (src/kmeans.f90:12-263) with fixed initial centroids and
+    ! one thread; labels are 1-based as the reference returns them.
+    subroutine ref_kmeans_c(n, x, y, w, k, equal_size, scaling, iterations, reassign_tol, weight_tol, &
+                            labels, centroids) bind(C, name="ref_kmeans_c")
+        integer(c_int), value :: n, k, equal_size, iterations
+        real(c_float), value :: scaling, reassign_tol, weight_tol
+        real(c_float), intent(in) :: x(n), y(n), w(n)
+        integer(c_int), intent(out) :: labels(n)
+        real(c_float), intent(inout) :: centroids(k, 2)
+        real, allocatable :: pts(:, :)
+        integer, allocatable :: lab(:)
+        allocate(pts(n, 2), lab(n))
+        pts(:, 1) = x
+        pts(:, 2) = y
+        call partition_data(pts, w, k, lab, centroids, ref_euclid, equal_size /= 0, &
+                            kmeans_compute_fixed_initial_centroids, 1, scaling, iterations, reassign_tol, weight_tol)
+        labels = lab
+    end subroutine ref_kmeans_c
 
     subroutine ref_shear_c(n_a, xa, ya, za, va, wa, n_b, xb, yb, zb, e1, e2, wb, mb, &
                            x_min, x_max, n_bin, spacing, p, n_threads, use_tree, &

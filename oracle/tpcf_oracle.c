@@ -488,3 +488,109 @@ void orc_bin_of_d2_array(int64_t n, const double *d2, const double *e2, int n_bi
 {
     for (int64_t i = 0; i < n; ++i) out[i] = orc_bin_of_d2(d2[i], e2, n_bin);
 }
+
+/* ---- k-means patch assignment: partition_data, src/kmeans.f90:12-263 ----
+ * Fixed initial centroids (kmeans_compute_fixed_initial_centroids,
+ * src/kmeans.f90:85-110) and initial labels int((i-1)/n k) + 1 (here 0-based);
+ * then rounds of assign_clusters (:156-200: the nearest centroid with the
+ * first minimum kept, distances scaled from the second round by
+ * 1 + (W_j / W_mean - 1) s when s > 0) and calculate_centroids (:202-212:
+ * weighted means of the member coordinates over the tracked cluster weights)
+ * until a round moves no point, moves fewer than tol n, or max_it + 1 rounds
+ * have run (the loop condition of :117).  jacobi = 0 updates the cluster
+ * weights while a round runs, point by point, as the reference's single
+ * thread does; jacobi = 1 uses the weights of the previous round (the GPU's
+ * order).  With s = 0 the two are the same.  The equal-size post-processing
+ * (:214-262) is not restated.  Spherical: distances are great-circle angles
+ * of (longitude, latitude) in radians and centroids the weighted means of the
+ * coordinates, as the reference's distance_sphere and calculate_centroids
+ * give.  FP64 throughout.  Returns the number of rounds run. */
+static double km_dist(int sph, double x1, double y1, double x2, double y2)
+{
+    if (!sph) return sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
+    double u[3], v[3];
+    unit_vec(x1, y1, u);
+    unit_vec(x2, y2, v);
+    return sph_angle(u, v);
+}
+
+int orc_kmeans(int64_t n, const double *x, const double *y, const double *w, int k, int max_it, double tol,
+               double scaling, int jacobi, int sph, int init_given, int32_t *labels, double *cen, double *cw,
+               int64_t *last_moves)
+{
+    if (n < 1 || k < 2) return -1;
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY, wsum = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        xmin = fmin(xmin, x[i]); xmax = fmax(xmax, x[i]);
+        ymin = fmin(ymin, y[i]); ymax = fmax(ymax, y[i]);
+        wsum += w[i];
+    }
+    const double diameter = sqrt((xmax - xmin) * (xmax - xmin) + (ymax - ymin) * (ymax - ymin));
+    const double mean_w = wsum / k;
+    if (!init_given) {
+        const int n_side = (int)sqrt((double)k);
+        if (xmax - xmin > ymax - ymin) {
+            const double dy = (ymax - ymin) / n_side, dx = (xmax - xmin) * n_side / k;
+            for (int j = 0; j < k; ++j) {
+                const int row = (int)((dx / 2.0 + dx * j) / (xmax - xmin));
+                cen[2 * j] = xmin + fmod(dx / 2.0 + dx * j, xmax - xmin);
+                cen[2 * j + 1] = ymin + dy / 2.0 + dy * row;
+            }
+        } else {
+            const double dx = (xmax - xmin) / n_side, dy = (ymax - ymin) * n_side / k;
+            for (int j = 0; j < k; ++j) {
+                const int row = (int)((dy / 2.0 + dy * j) / (ymax - ymin));
+                cen[2 * j + 1] = ymin + fmod(dy / 2.0 + dy * j, ymax - ymin);
+                cen[2 * j] = xmin + dx / 2.0 + dx * row;
+            }
+        }
+    }
+    for (int j = 0; j < k; ++j) cw[j] = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        int l = (int)((double)i / (double)n * k);
+        labels[i] = l < k ? l : k - 1;
+        cw[labels[i]] += w[i];
+    }
+    double *cw_prev = (double *)malloc(sizeof(double) * (size_t)k);
+    double *sx = (double *)malloc(sizeof(double) * 3 * (size_t)k);
+    if (!cw_prev || !sx) { free(cw_prev); free(sx); return -2; }
+    int it = 0;
+    int64_t moves = 0;
+    while (it <= max_it) {
+        ++it;
+        moves = 0;
+        memcpy(cw_prev, cw, sizeof(double) * (size_t)k);
+        const double *cwd = jacobi ? cw_prev : cw;
+        for (int64_t i = 0; i < n; ++i) {
+            double best = diameter * 10.0;
+            int bi = -1;
+            for (int j = 0; j < k; ++j) {
+                const double dw = (it > 1 && scaling > 0.0) ? 1.0 + (cwd[j] / mean_w - 1.0) * scaling : 1.0;
+                const double d = dw * km_dist(sph, x[i], y[i], cen[2 * j], cen[2 * j + 1]);
+                if (d < best) { best = d; bi = j; }
+            }
+            if (bi >= 0 && bi != labels[i]) {
+                cw[labels[i]] -= w[i];
+                cw[bi] += w[i];
+                labels[i] = bi;
+                ++moves;
+            }
+        }
+        for (int j = 0; j < 3 * k; ++j) sx[j] = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            sx[3 * labels[i]] += x[i] * w[i];
+            sx[3 * labels[i] + 1] += y[i] * w[i];
+            sx[3 * labels[i] + 2] += w[i];
+        }
+        for (int j = 0; j < k; ++j) {
+            cw[j] = sx[3 * j + 2];   /* the tracked weights, re-summed exactly */
+            cen[2 * j] = sx[3 * j] / cw[j];
+            cen[2 * j + 1] = sx[3 * j + 1] / cw[j];
+        }
+        if (moves == 0 || (double)moves / (double)n < tol) break;
+    }
+    free(cw_prev);
+    free(sx);
+    if (last_moves) *last_moves = moves;
+    return it;
+}

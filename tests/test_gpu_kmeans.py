@@ -1,0 +1,78 @@
+"""k-means patch labels on the GPU (tpcf_kmeans_*) against the oracle's
+restatement of partition_data (src/kmeans.f90:12-263), itself pinned against
+the reference's own code.
+
+The GPU compares squared distances and sums the centroids with atomics in no
+fixed order, so a point at an exact tie between two centroids could go either
+way: labels must agree for >= 99.99 % of the points (all of them in these
+cases so far), centroids within 1e-9 of the field size, rounds within one.
+Cases: clustered and uniform flat catalogues in FP64 and FP32, given starting
+centroids, spherical coordinates across longitude 0; the weight-equalised
+form and k outside [2, 1024] are refused.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def blobs(seed, n, k=12, dtype=np.float64):
+    r = np.random.default_rng(seed)
+    cent = r.uniform(0, 100, (k, 2))
+    lab = r.integers(0, k, n)
+    x = cent[lab, 0] + r.normal(0, 4, n)
+    y = cent[lab, 1] + r.normal(0, 4, n)
+    return x.astype(dtype), y.astype(dtype), r.uniform(0.5, 1.5, n).astype(dtype)
+
+
+def agree(g, o, size):
+    frac = np.mean(g["labels"] == o["labels"])
+    print(f"\nlabels agree {frac:.6f}, rounds {g['iterations']} / {o['iterations']}")
+    assert frac >= 0.9999
+    assert abs(g["iterations"] - o["iterations"]) <= 1
+    m = np.isfinite(o["centroids"]).all(axis=1)
+    assert np.max(np.abs(g["centroids"][m] - o["centroids"][m])) <= 1e-9 * size
+    np.testing.assert_allclose(g["cluster_weights"][m], o["cluster_weights"][m], rtol=1e-9)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_kmeans_clustered(tp, orc, ctx, dtype):
+    x, y, w = blobs(5, 200000, dtype=dtype)
+    g = tp.kmeans(x, y, w, 64, ctx=ctx)
+    o = orc.kmeans(x, y, w, 64)
+    agree(g, o, 100.0)
+
+
+def test_kmeans_uniform_given_centroids(tp, orc, ctx):
+    r = np.random.default_rng(6)
+    n = 300000
+    x, y, w = r.uniform(0, 3000, n), r.uniform(0, 2000, n), r.uniform(0.5, 1.5, n)
+    c0 = np.stack([r.uniform(0, 3000, 128), r.uniform(0, 2000, 128)], 1)
+    g = tp.kmeans(x, y, w, 128, centroids=c0, max_iterations=40, ctx=ctx)
+    o = orc.kmeans(x, y, w, 128, centroids=c0, max_iterations=40)
+    agree(g, o, 3000.0)
+    # reassignment tolerance stops early
+    g2 = tp.kmeans(x, y, w, 128, centroids=c0, reassignment_tol=0.01, ctx=ctx)
+    o2 = orc.kmeans(x, y, w, 128, centroids=c0, reassignment_tol=0.01)
+    agree(g2, o2, 3000.0)
+    assert g2["iterations"] < g["iterations"] or g["iterations"] == 41
+
+
+def test_kmeans_spherical_across_meridian(tp, orc, ctx):
+    r = np.random.default_rng(7)
+    n = 100000
+    lon = np.mod(r.uniform(-0.3, 0.3, n), 2 * np.pi)
+    lat = r.uniform(-0.3, 0.3, n)
+    w = r.uniform(0.5, 1.5, n)
+    c0 = np.stack([np.mod(r.uniform(-0.3, 0.3, 16), 2 * np.pi), r.uniform(-0.3, 0.3, 16)], 1)
+    g = tp.kmeans(lon, lat, w, 16, centroids=c0, spherical=True, max_iterations=30, ctx=ctx)
+    o = orc.kmeans(lon, lat, w, 16, centroids=c0, spherical=True, max_iterations=30)
+    agree(g, o, 2 * np.pi)
+
+
+def test_kmeans_refusals(tp, ctx):
+    x, y, w = blobs(8, 1000)
+    with pytest.raises(tp.TpcfError):
+        tp.kmeans(x, y, w, 1, ctx=ctx)
+    with pytest.raises(tp.TpcfError):
+        tp.kmeans(x, y, w, 2000, ctx=ctx)
