@@ -262,6 +262,58 @@ def bench_adjoint(args) -> int:
     return 0
 
 
+def bench_patches(args) -> int:
+    """Patch labels and bootstrap of the bootstrap chain: k-means with 128
+    patches over the 20M lenses of sk_20Mx20M_lognormal_f32 (Voronoi form of
+    partition_data, fixed starting grid, to convergence or 1000 rounds), then
+    the block bootstrap of 128 blocks x 30 bins with 10^4 resamples and the
+    covariances.  value = lens assignments per second of k-means (points x
+    rounds / time); the reference's partition_data is timed beside it on a
+    200k-lens subsample (one thread, as it runs)."""
+    import torch
+    tp = importlib.import_module("2pcf_amd")
+    synth = importlib.import_module("2pcf_amd.synth")
+    wl = synth.config("sk_20Mx20M_lognormal_f32", args.scale)
+    x, y, w = wl.lens["x"], wl.lens["y"], wl.lens["w"]
+    ctx = tp.Context(0)
+    tp.kmeans(x[:100000], y[:100000], w[:100000], 128, max_iterations=2, ctx=ctx)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    km = tp.kmeans(x, y, w, 128, ctx=ctx)
+    t_km = time.perf_counter() - t0
+    log(f"k-means: {len(x)} lenses, 128 patches, {km['iterations']} rounds in {t_km:.2f} s")
+    r = np.random.default_rng(9)
+    blocks = dict(g_t=r.normal(0, 1e-3, (128, 30)), g_x=r.normal(0, 1e-3, (128, 30)), k=r.uniform(1, 2, (128, 30)),
+                  weight=r.uniform(1, 2, 128))
+    tp.bootstrap(blocks, 100, seed=1, ctx=ctx)
+    t0 = time.perf_counter()
+    bt = tp.bootstrap(blocks, 10000, seed=2, ctx=ctx)
+    t_bt = time.perf_counter() - t0
+    log(f"bootstrap: 10^4 resamples of 128 blocks x 30 bins + covariances in {t_bt * 1e3:.1f} ms")
+    value = len(x) * km["iterations"] / t_km
+    res = {"metric": "k-means lens assignments/sec on 1 MI355X (patch labels of the bootstrap)", "value": value,
+           "unit": "assignments/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": t_km * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "patches_20M", "n_lens": len(x), "k": 128, "rounds": km["iterations"],
+                      "bootstrap_ms": t_bt * 1e3, "bootstrap": "10^4 resamples x 128 blocks x 30 bins"}}
+    if not args.no_cpu and _have_ref():
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle   # checker family: the CPU baseline only
+        sel = r.choice(len(x), size=200000, replace=False)
+        t0 = time.perf_counter()
+        oracle.ref_kmeans(x[sel], y[sel], w[sel], 128, max_iterations=20)
+        t_ref = time.perf_counter() - t0
+        # the rounds the same algorithm runs on this sample (the reference does not return them)
+        rounds = oracle.kmeans(x[sel], y[sel], w[sel], 128, max_iterations=20)["iterations"]
+        res["cpu_baseline"] = {"value": 200000 * rounds / t_ref, "unit": "assignments/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"reference partition_data, 200000 random lenses, 128 patches, {rounds} "
+                                         f"rounds (max_iterations 20) in {t_ref:.2f} s"}
+    print(json.dumps(res), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +336,8 @@ def main() -> int:
         return bench_mocks(args)
     if args.config == "adjoint_5Mx5M":
         return bench_adjoint(args)
+    if args.config == "patches_20M":
+        return bench_patches(args)
 
     import torch
 
