@@ -451,6 +451,9 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
         if (KIND == SHEAR && ZCHECK) {
             if (L.za > zr[j]) { in0 = false; in1 = false; }   // source in front of the lens, src/tpcf.f90:322-324
         }
+        // a segment spanning many bins is walked once per window: a source
+        // none of whose pairs fall in this window adds nothing (whole wave)
+        if (!__builtin_amdgcn_ballot_w64(in0 || in1)) continue;
         const T m = in0 ? (T)1 : (T)0, m2 = in1 ? (T)1 : (T)0;
         n0 += __popcll(__builtin_amdgcn_ballot_w64(m != (T)0) & w.valid);
         n1 += __popcll(__builtin_amdgcn_ballot_w64(m2 != (T)0) & w.valid);
