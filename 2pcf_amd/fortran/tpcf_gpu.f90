@@ -9,7 +9,11 @@
 !                     src/utils.f90:18-24) and create_bins (src/utils.f90:139-230),
 !                     so that a driver swaps "use tpcf" / "use utils, only: binning,
 !                     create_bins" for "use tpcf_gpu".  Extension: shear_2pcf_patches /
-!                     kappa_2pcf_patches also return the per-patch-pair raw sums.
+!                     kappa_2pcf_patches also return the per-patch-pair raw sums;
+!                     partition_data_gpu (the Voronoi form of partition_data,
+!                     src/kmeans.f90:12-263) and block_bootstrap_gpu (the resampling
+!                     of src/tpcf.f90:1165-1191 and covariance_matrix,
+!                     src/utils.f90:745-764) run the bootstrap chain on the GPU.
 !
 ! Errors follow the reference's style (src/utils.f90:36-45): a message and
 ! "error stop".  The default real kind may be 4 (FP32 path) or 8 (a
@@ -47,6 +51,16 @@ module tpcf_gpu_c
         type(c_ptr) :: pp_n = c_null_ptr, pp_theta = c_null_ptr, pp_gt = c_null_ptr, pp_gx = c_null_ptr, &
                        pp_w = c_null_ptr, pp_k = c_null_ptr
     end type tpcf_out
+
+    type, bind(C) :: tpcf_kmeans_opts
+        integer(c_int32_t) :: k = 0, max_iterations = 1000, init = 0, pad = 0
+        real(c_double) :: reassignment_tol = 0.0d0, weight_equalizer_scaling = 0.0d0
+    end type tpcf_kmeans_opts
+
+    type, bind(C) :: tpcf_blocks
+        integer(c_int32_t) :: n_block = 0, n_bin = 0
+        type(c_ptr) :: g_t = c_null_ptr, g_x = c_null_ptr, k = c_null_ptr, weight = c_null_ptr
+    end type tpcf_blocks
 
     interface
         integer(c_int) function tpcf_create_bins(x_min, x_max, n_bin, spacing, ihs_theta, bins) bind(C)
@@ -141,6 +155,38 @@ module tpcf_gpu_c
             integer(c_int32_t), value :: n_patch_A, n_patch_B
             type(tpcf_out), intent(inout) :: out
         end function tpcf_kappa_f64
+
+        integer(c_int) function tpcf_kmeans_f32(ctx, n, x, y, w, opts, flags, labels, centroids, cluster_weights, &
+                                                rounds, last_moves) bind(C)
+            import :: c_ptr, c_int, c_int32_t, c_int64_t, tpcf_kmeans_opts
+            type(c_ptr), value :: ctx, x, y, w, labels, centroids, cluster_weights
+            integer(c_int64_t), value :: n
+            type(tpcf_kmeans_opts), intent(in) :: opts
+            integer(c_int32_t), value :: flags
+            integer(c_int32_t), intent(out) :: rounds
+            integer(c_int64_t), intent(out) :: last_moves
+        end function tpcf_kmeans_f32
+
+        integer(c_int) function tpcf_kmeans_f64(ctx, n, x, y, w, opts, flags, labels, centroids, cluster_weights, &
+                                                rounds, last_moves) bind(C)
+            import :: c_ptr, c_int, c_int32_t, c_int64_t, tpcf_kmeans_opts
+            type(c_ptr), value :: ctx, x, y, w, labels, centroids, cluster_weights
+            integer(c_int64_t), value :: n
+            type(tpcf_kmeans_opts), intent(in) :: opts
+            integer(c_int32_t), value :: flags
+            integer(c_int32_t), intent(out) :: rounds
+            integer(c_int64_t), intent(out) :: last_moves
+        end function tpcf_kmeans_f64
+
+        integer(c_int) function tpcf_bootstrap(ctx, blocks, n_boot, multiplicity, seed, supersampling, flags, &
+                                               g_t, g_x, weights, cov_t, cov_x, multiplicity_out) bind(C)
+            import :: c_ptr, c_int, c_int32_t, c_int64_t, c_double, tpcf_blocks
+            type(c_ptr), value :: ctx, multiplicity, g_t, g_x, weights, cov_t, cov_x, multiplicity_out
+            type(tpcf_blocks), intent(in) :: blocks
+            integer(c_int32_t), value :: n_boot, flags
+            integer(c_int64_t), value :: seed
+            real(c_double), value :: supersampling
+        end function tpcf_bootstrap
     end interface
 end module tpcf_gpu_c
 
@@ -157,6 +203,7 @@ module tpcf_gpu
     public allocate_shear_2pcf_results, deallocate_shear_2pcf_results
     public shear_2pcf, kappa_2pcf, shear_2pcf_patches, kappa_2pcf_patches
     public tpcf_gpu_finalize, tpcf_gpu_set_option
+    public partition_data_gpu, block_bootstrap_gpu
 
     integer, parameter :: lin_binning = 0, log_binning = 1, ihs_binning = 2, sqrt_binning = 3   ! src/utils.f90:7
 
@@ -564,5 +611,103 @@ contains
         if (present(verbose)) v = verbose
         call run_kappa(field, bins, results, .false., .true., v, patch_A, patch_B, n_patch_A, n_patch_B, pp)
     end subroutine kappa_2pcf_patches
+
+    !> partition_data (src/kmeans.f90:12-263) in its Voronoi form on the GPU:
+    !> data_points(n, 2) and weights as the reference takes them, 1-based
+    !> cluster_indicies, cluster_centroids(k, 2).  given_centroids = .true.
+    !> starts from cluster_centroids instead of the fixed grid; spherical
+    !> coordinates are longitude / latitude in radians.
+    subroutine partition_data_gpu(data_points, weights, k, cluster_indicies, cluster_centroids, spherical_coords, &
+                                  given_centroids, iterations, reassignment_tol)
+        real, intent(in) :: data_points(:, :), weights(:)
+        integer, intent(in) :: k
+        integer, intent(out) :: cluster_indicies(:)
+        real, intent(inout) :: cluster_centroids(k, 2)
+        logical, optional, intent(in) :: spherical_coords, given_centroids
+        integer, optional, intent(in) :: iterations
+        real, optional, intent(in) :: reassignment_tol
+        real, allocatable, target :: x(:), y(:), w(:)
+        integer(c_int32_t), allocatable, target :: lab(:)
+        real(c_double), target :: cen(2, k), cw(k)
+        type(tpcf_kmeans_opts) :: o
+        integer(c_int32_t) :: flags, rounds
+        integer(c_int64_t) :: moves, n
+        integer(c_int) :: rc
+        call ensure_ctx()
+        n = size(data_points, 1)
+        allocate(x(n), y(n), w(n), lab(n))
+        x = data_points(:, 1)
+        y = data_points(:, 2)
+        w = weights
+        o%k = k
+        if (present(iterations)) o%max_iterations = iterations
+        if (present(reassignment_tol)) o%reassignment_tol = real(reassignment_tol, c_double)
+        o%init = 0
+        if (present(given_centroids)) then
+            if (given_centroids) o%init = 1
+        end if
+        cen(1, :) = real(cluster_centroids(:, 1), c_double)
+        cen(2, :) = real(cluster_centroids(:, 2), c_double)
+        flags = 0
+        if (present(spherical_coords)) then
+            if (spherical_coords) flags = TPCF_SPHERICAL
+        end if
+        if (kind(1.0) == 4) then
+            rc = tpcf_kmeans_f32(ctx, n, c_loc(x), c_loc(y), c_loc(w), o, flags, c_loc(lab), c_loc(cen), &
+                                 c_loc(cw), rounds, moves)
+        else
+            rc = tpcf_kmeans_f64(ctx, n, c_loc(x), c_loc(y), c_loc(w), o, flags, c_loc(lab), c_loc(cen), &
+                                 c_loc(cw), rounds, moves)
+        end if
+        call check(rc, "tpcf_kmeans")
+        cluster_indicies = lab + 1
+        cluster_centroids(:, 1) = real(cen(1, :))
+        cluster_centroids(:, 2) = real(cen(2, :))
+    end subroutine partition_data_gpu
+
+    !> The block bootstrap of src/tpcf.f90:1165-1191 and covariance_matrix
+    !> (src/utils.f90:745-764) on the GPU.  block_g_t, block_g_x, block_K
+    !> (n_bin, n_blocks) and block_weight(n_blocks) as the reference forms
+    !> them (src/tpcf.f90:1130-1137); n_draw = int(n_blocks * supersampling_factor)
+    !> draws per sample from a generator keyed by seed; g_t_bootstrap and
+    !> g_x_bootstrap (n_bootstrap_samples, n_bin) as src/main.f90:524-532.
+    subroutine block_bootstrap_gpu(block_g_t, block_g_x, block_K, block_weight, n_bootstrap_samples, &
+                                   supersampling_factor, seed, g_t_bootstrap, g_x_bootstrap, weights, &
+                                   g_t_covariance, g_x_covariance)
+        real, intent(in) :: block_g_t(:, :), block_g_x(:, :), block_K(:, :), block_weight(:)
+        integer, intent(in) :: n_bootstrap_samples
+        real, intent(in) :: supersampling_factor
+        integer(kind=8), intent(in) :: seed
+        real, intent(out) :: g_t_bootstrap(:, :), g_x_bootstrap(:, :), weights(:)
+        real, intent(out) :: g_t_covariance(:, :), g_x_covariance(:, :)
+        real(c_double), allocatable, target :: bt(:, :), bx(:, :), bk(:, :), bw(:)
+        real(c_double), allocatable, target :: ot(:, :), ox(:, :), ow(:), ct(:, :), cx(:, :)
+        type(tpcf_blocks) :: b
+        integer :: n_bin, n_blk
+        call ensure_ctx()
+        n_bin = size(block_g_t, 1)
+        n_blk = size(block_g_t, 2)
+        allocate(bt(n_bin, n_blk), bx(n_bin, n_blk), bk(n_bin, n_blk), bw(n_blk))
+        allocate(ot(n_bin, n_bootstrap_samples), ox(n_bin, n_bootstrap_samples), ow(n_bootstrap_samples))
+        allocate(ct(n_bin, n_bin), cx(n_bin, n_bin))
+        bt = block_g_t
+        bx = block_g_x
+        bk = block_K
+        bw = block_weight
+        b%n_block = n_blk
+        b%n_bin = n_bin
+        b%g_t = c_loc(bt)
+        b%g_x = c_loc(bx)
+        b%k = c_loc(bk)
+        b%weight = c_loc(bw)
+        call check(tpcf_bootstrap(ctx, b, int(n_bootstrap_samples, c_int32_t), c_null_ptr, int(seed, c_int64_t), &
+                                  real(supersampling_factor, c_double), 0_c_int32_t, c_loc(ot), c_loc(ox), &
+                                  c_loc(ow), c_loc(ct), c_loc(cx), c_null_ptr), "tpcf_bootstrap")
+        g_t_bootstrap = real(transpose(ot))
+        g_x_bootstrap = real(transpose(ox))
+        weights = real(ow)
+        g_t_covariance = real(ct)
+        g_x_covariance = real(cx)
+    end subroutine block_bootstrap_gpu
 
 end module tpcf_gpu

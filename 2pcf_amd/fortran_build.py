@@ -6,8 +6,9 @@ Products (in-tree, next to the HIP library):
                                     against lib2pcf_amd.so
   2pcf_amd/fortran/include/*.mod    the module files a Fortran driver compiles
                                     against (flang's module format)
-  2pcf_amd/fortran/fortran_driver   tests/fortran/fortran_driver.f90, the
-                                    Fortran host program the GPU tests run
+  2pcf_amd/fortran/fortran_driver   tests/fortran/fortran_driver.f90 and
+  2pcf_amd/fortran/fortran_patches  tests/fortran/fortran_patches.f90, the
+                                    Fortran host programs the GPU tests run
 Both shared objects find each other through an $ORIGIN run path.  Skipped
 (with a message) when flang is absent.
 """
@@ -24,6 +25,7 @@ MODDIR = os.path.join(FDIR, "include")
 BUILD = os.path.join(ROOT, "_build", "fortran")
 LIB = os.path.join(PKG, "libtpcf_gpu_fortran.so")
 DRIVER = os.path.join(FDIR, "fortran_driver")
+PROGRAMS = ("fortran_driver", "fortran_patches")
 FC = os.environ.get("FC", "/opt/rocm/llvm/bin/flang")
 FFLAGS = ["-O2", "-fPIC"]
 
@@ -46,13 +48,15 @@ def build() -> str | None:
     tmp = LIB + ".tmp"
     _run([FC, *FFLAGS, "-shared", "-o", tmp, obj, "-L" + PKG, "-l2pcf_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, LIB)
-    drv_obj = os.path.join(BUILD, "fortran_driver.o")
-    _run([FC, *FFLAGS, "-c", os.path.join(ROOT, "tests", "fortran", "fortran_driver.f90"), "-I", MODDIR,
-          "-module-dir", BUILD, "-o", drv_obj])
-    tmp = DRIVER + ".tmp"
-    _run([FC, *FFLAGS, "-o", tmp, drv_obj, "-L" + PKG, "-ltpcf_gpu_fortran", "-l2pcf_amd",
-          "-Wl,-rpath,$ORIGIN/.."])
-    os.replace(tmp, DRIVER)
+    for prog in PROGRAMS:
+        obj_p = os.path.join(BUILD, prog + ".o")
+        _run([FC, *FFLAGS, "-c", os.path.join(ROOT, "tests", "fortran", prog + ".f90"), "-I", MODDIR,
+              "-module-dir", BUILD, "-o", obj_p])
+        exe = os.path.join(FDIR, prog)
+        tmp = exe + ".tmp"
+        _run([FC, *FFLAGS, "-o", tmp, obj_p, "-L" + PKG, "-ltpcf_gpu_fortran", "-l2pcf_amd",
+              "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(tmp, exe)
     return LIB
 
 

@@ -71,3 +71,56 @@ def test_fortran_host_matches_ctypes(tp, ctx, tmp_path, kind, patches):
         np.testing.assert_array_equal(fo["pp_n"], g.pp_n)
         for k in ("pp_theta", "pp_gt", "pp_gx", "pp_w", "pp_k"):
             close(fo[k], getattr(g, k))
+
+
+def test_fortran_patch_labels_and_bootstrap(tp, ctx, tmp_path):
+    """tests/fortran/fortran_patches.f90: partition_data_gpu and
+    block_bootstrap_gpu from Fortran give the ctypes results (1-based labels,
+    the reference's (n_boot, n_bin) bootstrap layout)."""
+    import os
+    r = np.random.default_rng(41)
+    n, k, n_blk, n_bin, n_boot, seed, ss = 50000, 24, 24, 10, 400, 77, 1.0
+    cen = r.uniform(0, 100, (8, 2))
+    lab0 = r.integers(0, 8, n)
+    x = (cen[lab0, 0] + r.normal(0, 6, n)).astype(np.float32)
+    y = (cen[lab0, 1] + r.normal(0, 6, n)).astype(np.float32)
+    w = r.uniform(0.5, 1.5, n).astype(np.float32)
+    bk = r.uniform(1, 3, (n_blk, n_bin)).astype(np.float32)
+    bt = (r.normal(0, 0.01, (n_blk, n_bin)) * bk).astype(np.float32)
+    bx = (r.normal(0, 0.01, (n_blk, n_bin)) * bk).astype(np.float32)
+    bw = r.uniform(1, 2, n_blk).astype(np.float32)
+    with open(tmp_path / "in", "wb") as f:
+        f.write(np.array([n, k, n_blk, n_bin, n_boot], np.int32).tobytes())
+        f.write(np.array([seed], np.int64).tobytes())
+        f.write(np.array([ss], np.float32).tobytes())
+        f.write(x.tobytes() + y.tobytes() + w.tobytes())
+        # Fortran (n_bin, n_blk) column-major == C [n_blk][n_bin]
+        f.write(bt.tobytes() + bx.tobytes() + bk.tobytes() + bw.tobytes())
+    exe = os.path.join(os.path.dirname(F.DRIVER), "fortran_patches")
+    p = subprocess.run([exe, str(tmp_path / "in"), str(tmp_path / "out")], capture_output=True, text=True,
+                       timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0
+    raw = open(tmp_path / "out", "rb").read()
+    pos = 0
+
+    def take(dt, count, shape=None, order="C"):
+        nonlocal pos
+        a = np.frombuffer(raw, dt, count, pos)
+        pos += a.nbytes
+        return a.reshape(shape, order=order) if shape else a
+
+    lab = take(np.int32, n)
+    fcen = take(np.float32, 2 * k, (k, 2), "F")
+    gt = take(np.float32, n_boot * n_bin, (n_boot, n_bin), "F")
+    gx = take(np.float32, n_boot * n_bin, (n_boot, n_bin), "F")
+    ww = take(np.float32, n_boot)
+    ct = take(np.float32, n_bin * n_bin, (n_bin, n_bin), "F")
+    km = tp.kmeans(x, y, w, k, ctx=ctx)
+    np.testing.assert_array_equal(lab, km["labels"] + 1)
+    np.testing.assert_allclose(fcen, km["centroids"], rtol=1e-6, atol=1e-5)
+    b = tp.bootstrap(dict(g_t=bt, g_x=bx, k=bk, weight=bw), n_boot, seed=seed, supersampling=ss, ctx=ctx)
+    close(gt, b["g_t"])
+    close(gx, b["g_x"])
+    close(ww, b["weights"])
+    close(ct, b["cov_t"])
