@@ -22,12 +22,15 @@
 //              dval_i = w_i sum_b (alpha S^E + alpha' S^B),  dw_i = val_i sum_b (...) - sum_b beta Om_ib,
 //              dkE_j = w_j sum_b alpha A,  dkB_j = w_j sum_b alpha' A,  dw_j = sum_b (alpha kE_j + alpha' kB_j) A - sum_b beta Om_jb.
 // The per-object sums are the pair kernel's marks: once with unit lens values
-// and weights (lens side), once with the roles of the catalogues swapped
+// and weights (lens side; the forward per-bin sums G, X, K are contractions of
+// these marks with a_i and w_i, so no forward pass of its own), once with the
+// roles of the catalogues swapped
 // (source side: the sources become the lanes, the lenses are streamed with
 // their a_i as the first component; the z test is reversed by storing -z).
 // Lenses with w = 0 (skipped by the reference) get the gradient of the
 // smooth extension; objects masked by z < 0 get zero.
 #include <cmath>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -46,6 +49,40 @@ __global__ void k_fill_value(T *p, int64_t n, T v)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// The forward sums from the lens-side marks (unit lens values and weights):
+// G_b = sum_i a_i P_ib, X_b = sum_i a_i Q_ib, K_b = sum_i w_i kap_ib (kappa:
+// E, B and W with S^E, S^B, Om) -- the per-bin sums of the forward pass
+// without a pass of their own.  blockIdx.y = bin; out[b][3] by atomics.
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_fwd(int64_t n_lens, int32_t n_bin, const uint32_t *__restrict__ lidx,
+                                                 const double *__restrict__ mk, const T *__restrict__ val,
+                                                 const T *__restrict__ w, int kind, double *__restrict__ out)
+{
+    __shared__ double sm[3][256];
+    const int b = blockIdx.y;
+    double g = 0.0, x = 0.0, k = 0.0;
+    for (int64_t li = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; li < n_lens;
+         li += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = lidx[li];
+        const double wi = (double)w[o], ai = (double)val[o] * wi;
+        const double *m4 = mk + ((size_t)li * n_bin + b) * 4;
+        g = fma(ai, m4[0], g);
+        x = fma(ai, m4[1], x);
+        k = fma(wi, kind == SHEAR ? m4[2] : m4[3], k);
+    }
+    sm[0][threadIdx.x] = g;
+    sm[1][threadIdx.x] = x;
+    sm[2][threadIdx.x] = k;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int q = 0; q < 3; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) unsafeAtomicAdd(&out[3 * b + q], sm[q][0]);
 }
 
 // Lens side: marks[li][b] = (P or S^E, Q or S^B, kap, Om) of sorted lens li.
@@ -171,27 +208,8 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
     }
     if (!shear) { base.za = nullptr; base.zb = nullptr; base.mb = nullptr; }
 
-    // ---- forward: per-bin raw sums ----
-    double G[TPCF_MAX_BIN], X[TPCF_MAX_BIN], K[TPCF_MAX_BIN];
-    {
-        tpcf_out o;
-        std::memset(&o, 0, sizeof o);
-        o.real_bytes = 8;
-        o.sum_gt = G;
-        o.sum_gx = X;
-        o.sum_k = K;
-        CallArgs<T> f = base;
-        TPCF_TRY(run_call<T>(ctx, f, run_flags, &o, t0, nullptr));
-    }
     Coef c;
     std::memset(&c, 0, sizeof c);
-    for (int b = 0; b < n_bin; ++b) {
-        if (!(K[b] > 0.0)) continue;   // g = 0 there: no dependence
-        const double g1 = G[b] / K[b], g2 = X[b] / K[b];
-        c.at[b] = gbar1[b] / K[b];
-        c.ax[b] = gbar2[b] / K[b];
-        c.be[b] = (gbar1[b] * g1 + gbar2[b] * g2) / K[b];
-    }
 
     // ---- gradient buffers (device), zero for masked objects ----
     double *gA, *gB;
@@ -229,8 +247,9 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
     tpcf_out none;
     std::memset(&none, 0, sizeof none);
     none.real_bytes = 8;
-    // ---- lens side: unit lens values and weights, real sources ----
-    if (n_a > 0 && n_b > 0 && (dvA || dwA)) {
+    // ---- lens side: unit lens values and weights, real sources; the forward
+    // sums G, X, K per bin follow from these marks ----
+    if (n_a > 0 && n_b > 0) {
         CallArgs<T> l = base;
         l.vala = ones;
         l.wa = ones;
@@ -239,7 +258,27 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
         Prepared<T> p;
         TPCF_TRY(run_call<T>(ctx, l, run_flags, &none, t0, &p));
         const double *mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
+        double *fs;
+        TPCF_TRY(ensure(ctx, tpcf_ctx::B_TMP, sizeof(double) * 3 * n_bin, (void **)&fs));
+        fill_f64(s, fs, 3 * n_bin);
+        std::vector<double> h(3 * n_bin, 0.0);
         if (p.n_lens > 0) {
+            const unsigned gx = (unsigned)std::min<int64_t>(1024, (p.n_lens + 255) / 256);
+            hipLaunchKernelGGL(k_adj_fwd<T>, dim3(gx, (unsigned)n_bin), dim3(256), 0, s, p.n_lens, n_bin, p.lidx, mk,
+                               base.vala, base.wa, kind, fs);
+            TPCF_HIP(ctx, hipGetLastError());
+            TPCF_HIP(ctx, hipMemcpyAsync(h.data(), fs, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+            TPCF_HIP(ctx, hipStreamSynchronize(s));
+        }
+        for (int b = 0; b < n_bin; ++b) {
+            const double G = h[3 * b], X = h[3 * b + 1], K = h[3 * b + 2];
+            if (!(K > 0.0)) continue;   // g = 0 there: no dependence
+            const double g1 = G / K, g2 = X / K;
+            c.at[b] = gbar1[b] / K;
+            c.ax[b] = gbar2[b] / K;
+            c.be[b] = (gbar1[b] * g1 + gbar2[b] * g2) / K;
+        }
+        if (p.n_lens > 0 && (dvA || dwA)) {
             hipLaunchKernelGGL(k_adj_lens<T>, dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s, p.n_lens, n_bin,
                                p.lidx, mk, base.vala, base.wa, c, dvA, dwA);
             TPCF_HIP(ctx, hipGetLastError());

@@ -256,7 +256,9 @@ def bench_adjoint(args) -> int:
            "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": "adjoint_5Mx5M", "n": n, "forward_pairs_per_step": fwd_pairs,
-                      "passes_per_step": "2 forward + per correlation 1 forward, 1 lens-side, 1 source-side"}}
+                      "passes_per_step": "per correlation: the autograd forward, then a lens-side and a "
+                                         "source-side marks pass (the adjoint's forward sums come from the "
+                                         "lens-side marks)"}}
     print(json.dumps(res), flush=True)
     ctx.close()
     return 0
