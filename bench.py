@@ -327,6 +327,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell", type=float, default=0.0, help="source cell side (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true", help="mocks_256: skip the batch autotuner")
+    ap.add_argument("--marks", action="store_true",
+                    help="compute the per-lens marks as well (the reference's call always does)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -367,7 +369,8 @@ def main() -> int:
                             patch_A=pa.data_ptr() if pa is not None else None,
                             patch_B=pb.data_ptr() if pb is not None else None,
                             n_patch_A=wl.n_patch, n_patch_B=wl.n_patch, real_bytes=4, device_inputs=True,
-                            ctx=ctx, n_A=len(wl.lens["x"]), n_B=len(wl.src["x"]), spherical=wl.spherical)
+                            ctx=ctx, n_A=len(wl.lens["x"]), n_B=len(wl.src["x"]), spherical=wl.spherical,
+                            marks=args.marks)
 
     for i in range(args.warmup):
         r = step()
@@ -398,7 +401,7 @@ def main() -> int:
                    "n_bin": wl.n_bin, "theta": [wl.x_min, wl.x_max], "patches": wl.n_patch,
                    "kind": "shear-scalar" if wl.kind == 0 else "kappa-kappa", "scale": args.scale,
                    "pairs_per_step": per_step_pairs, "cell_size": stats["cell_size"],
-                   "spherical": wl.spherical, "parallelism": "single GPU"},
+                   "spherical": wl.spherical, "marks": args.marks, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": fpp},
