@@ -29,6 +29,7 @@ FLAG_VERBOSE = 1 << 2
 FLAG_NO_KD_TREE = 1 << 3
 FLAG_BATCH_AUTOTUNE = 1 << 4
 FLAG_BATCH_NO_GRAPH = 1 << 5
+FLAG_DEVICE_MARKS = 1 << 6
 
 OPT_CELL_SIZE, OPT_COUNTERS, OPT_CELL_TARGET, OPT_SPLIT = 1, 2, 3, 4
 

@@ -108,24 +108,41 @@ __global__ void k_bin_totals(int64_t n_pp, int32_t n_bin, const double *__restri
 
 // Per-lens marks from sorted order back to the caller's order (columns of
 // real(n_bin, n_A), src/tpcf.f90:204-205) plus marks_weight = sum of w_marks.
+// Any of the five outputs may be NULL.
 template <typename R>
-__global__ void k_marks_out(int64_t n_lens, int32_t n_bin, int64_t n_a, const uint32_t *__restrict__ lidx,
-                            const double *__restrict__ mk, R *__restrict__ out)
+struct MarksOut {
+    R *q[5];
+};
+
+// One thread per (lens, bin): a lens' row of n_bin reals is written by
+// consecutive lanes (the lenses themselves land in the caller's order, i.e.
+// scattered); the bin-0 thread also sums the row for marks_weight.
+template <typename R>
+__global__ void k_marks_out(int64_t n_lens, int32_t n_bin, const uint32_t *__restrict__ lidx,
+                            const double *__restrict__ mk, MarksOut<R> out)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n_lens) return;
-    const size_t o = lidx[i];
-    const size_t plane = (size_t)n_a * n_bin;
-    double mw = 0.0;
-    for (int b = 0; b < n_bin; ++b) {
-        const double *m4 = mk + ((size_t)i * n_bin + b) * 4;
-        out[0 * plane + o * n_bin + b] = (R)m4[0];
-        out[1 * plane + o * n_bin + b] = (R)m4[1];
-        out[2 * plane + o * n_bin + b] = (R)m4[2];
-        out[3 * plane + o * n_bin + b] = (R)m4[3];
-        mw += m4[3];
+    const int64_t n = n_lens * n_bin;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t / n_bin;
+        const int b = (int)(t - i * n_bin);
+        const size_t o = lidx[i];
+        const double *m4 = mk + (size_t)t * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (out.q[q]) out.q[q][o * n_bin + b] = (R)m4[q];
+        if (b == 0 && out.q[4]) {
+            double mw = 0.0;
+            for (int c = 0; c < n_bin; ++c) mw += m4[4 * c + 3];
+            out.q[4][o] = (R)mw;
+        }
     }
-    out[4 * plane + o] = (R)mw;
+}
+
+// marks [n][4]: K (component 2) = W (component 3)
+__global__ void k_marks_k_from_w(int64_t n, double *__restrict__ mk)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        mk[4 * i + 2] = mk[4 * i + 3];
 }
 
 template <typename R>
@@ -133,6 +150,25 @@ __global__ void k_zero(R *p, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = (R)0;
+}
+
+// Zero the marks outputs (the caller's lenses outside the pair set keep 0)
+// and write the pair set's marks into them.
+template <typename R>
+int marks_to(tpcf_ctx *ctx, hipStream_t s, int64_t n_lens, int32_t n_bin, int64_t n_a, const uint32_t *lidx,
+             const double *mk, const MarksOut<R> &out)
+{
+    for (int q = 0; q < 5; ++q) {
+        const int64_t n = (q < 4) ? n_a * n_bin : n_a;
+        if (!out.q[q] || n == 0) continue;
+        hipLaunchKernelGGL(k_zero<R>, dim3((unsigned)std::min<int64_t>(8192, (n + 255) / 256)), dim3(256), 0, s,
+                           out.q[q], n);
+    }
+    if (n_lens > 0)
+        hipLaunchKernelGGL(k_marks_out<R>, dim3((unsigned)std::min<int64_t>(1 << 20, (n_lens * n_bin + 255) / 256)),
+                           dim3(256), 0, s, n_lens, n_bin, lidx, mk, out);
+    TPCF_HIP(ctx, hipGetLastError());
+    return TPCF_OK;
 }
 
 void put_real(void *dst, int rb, int i, double v)
@@ -178,7 +214,7 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
     TPCF_TRY(check_bins(ctx, bins));
     if (!out) return fail(ctx, TPCF_ERR_ARG, "out is NULL");
     if (out->real_bytes != 4 && out->real_bytes != 8) return fail(ctx, TPCF_ERR_ARG, "out->real_bytes must be 4 or 8");
-    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE | TPCF_SPHERICAL))
+    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE | TPCF_SPHERICAL | TPCF_DEVICE_MARKS))
         return fail(ctx, TPCF_ERR_ARG, "unknown flag");
     if (n_a < 0 || n_b < 0 || n_a >= ((int64_t)1 << 31) || n_b >= ((int64_t)1 << 31))
         return fail(ctx, TPCF_ERR_ARG, "catalogue sizes must be in [0, 2^31)");
@@ -277,6 +313,13 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
             fill_f64(s, marks, 4 * p.n_lens * n_bin);
         }
         TPCF_TRY(run_pairs<T>(ctx, a, p, pp, ppn, marks));
+        if (a.marks && p.n_lens > 0 && (a.kind == KAPPA || p.m_zero)) {
+            // the pair kernel leaves K marks to this copy when K == W
+            const int64_t nm = p.n_lens * n_bin;
+            hipLaunchKernelGGL(k_marks_k_from_w, dim3((unsigned)std::min<int64_t>(8192, (nm + 255) / 256)),
+                               dim3(256), 0, s, nm, marks);
+            TPCF_HIP(ctx, hipGetLastError());
+        }
         if (p.n_items == 0) {
             TPCF_HIP(ctx, hipEventRecord(ctx->ev[2], s));
             TPCF_HIP(ctx, hipEventRecord(ctx->ev[3], s));
@@ -299,30 +342,29 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
             TPCF_HIP(ctx, hipMemcpyAsync(hppn.data(), ppno, sizeof(long long) * hppn.size(), hipMemcpyDeviceToHost, s));
         }
         if (a.marks && !a.marks_device) {
-            void *mo;
-            const size_t plane = (size_t)n_a * n_bin;
-            TPCF_TRY(ensure(ctx, tpcf_ctx::B_TMP, (size_t)rb * (4 * plane + n_a), &mo));
-            const int64_t nz = (int64_t)(4 * plane + n_a);
-            const unsigned zb = (unsigned)std::min<int64_t>(8192, (nz + 255) / 256);
-            if (rb == 8) {
-                hipLaunchKernelGGL(k_zero<double>, dim3(zb), dim3(256), 0, s, (double *)mo, nz);
-                if (p.n_lens > 0)
-                    hipLaunchKernelGGL(k_marks_out<double>, dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
-                                       p.n_lens, n_bin, n_a, p.lidx, marks, (double *)mo);
-            } else {
-                hipLaunchKernelGGL(k_zero<float>, dim3(zb), dim3(256), 0, s, (float *)mo, nz);
-                if (p.n_lens > 0)
-                    hipLaunchKernelGGL(k_marks_out<float>, dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
-                                       p.n_lens, n_bin, n_a, p.lidx, marks, (float *)mo);
-            }
-            TPCF_HIP(ctx, hipGetLastError());
             void *dsts[5] = {out->xi_E_marks, out->xi_B_marks, out->K_marks, out->w_marks, out->marks_weight};
-            for (int q = 0; q < 5; ++q) {
+            const size_t plane = (size_t)n_a * n_bin;
+            void *mo = nullptr;
+            if (!(flags & TPCF_DEVICE_MARKS))
+                TPCF_TRY(ensure(ctx, tpcf_ctx::B_TMP, (size_t)rb * (4 * plane + n_a), &mo));
+            // with TPCF_DEVICE_MARKS straight into the caller's arrays, else
+            // into one staging buffer copied out below
+            void *q5[5];
+            for (int q = 0; q < 5; ++q)
+                q5[q] = !dsts[q] ? nullptr : mo ? static_cast<char *>(mo) + (size_t)rb * q * plane : dsts[q];
+            if (rb == 8)
+                TPCF_TRY(marks_to<double>(ctx, s, p.n_lens, n_bin, n_a, p.lidx, marks,
+                                          MarksOut<double>{{(double *)q5[0], (double *)q5[1], (double *)q5[2],
+                                                            (double *)q5[3], (double *)q5[4]}}));
+            else
+                TPCF_TRY(marks_to<float>(ctx, s, p.n_lens, n_bin, n_a, p.lidx, marks,
+                                         MarksOut<float>{{(float *)q5[0], (float *)q5[1], (float *)q5[2],
+                                                          (float *)q5[3], (float *)q5[4]}}));
+            for (int q = 0; q < 5 && mo; ++q) {
                 if (!dsts[q]) continue;
                 const size_t cnt = (q < 4) ? plane : (size_t)n_a;
                 if (cnt == 0) continue;
-                TPCF_HIP(ctx, hipMemcpyAsync(dsts[q], static_cast<char *>(mo) + (size_t)rb * q * plane,
-                                             (size_t)rb * cnt, hipMemcpyDeviceToHost, s));
+                TPCF_HIP(ctx, hipMemcpyAsync(dsts[q], q5[q], (size_t)rb * cnt, hipMemcpyDeviceToHost, s));
             }
         }
     } else {
@@ -334,8 +376,19 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
         }
         const size_t plane = (size_t)n_a * n_bin;
         void *dsts[5] = {out->xi_E_marks, out->xi_B_marks, out->K_marks, out->w_marks, out->marks_weight};
-        for (int q = 0; q < 5 && !a.marks_device; ++q)
-            if (dsts[q]) std::memset(dsts[q], 0, (size_t)rb * ((q < 4) ? plane : (size_t)n_a));
+        if (a.marks && !a.marks_device && (flags & TPCF_DEVICE_MARKS)) {
+            if (rb == 8)
+                TPCF_TRY(marks_to<double>(ctx, s, 0, n_bin, n_a, nullptr, nullptr,
+                                          MarksOut<double>{{(double *)dsts[0], (double *)dsts[1], (double *)dsts[2],
+                                                            (double *)dsts[3], (double *)dsts[4]}}));
+            else
+                TPCF_TRY(marks_to<float>(ctx, s, 0, n_bin, n_a, nullptr, nullptr,
+                                         MarksOut<float>{{(float *)dsts[0], (float *)dsts[1], (float *)dsts[2],
+                                                          (float *)dsts[3], (float *)dsts[4]}}));
+        } else {
+            for (int q = 0; q < 5 && !a.marks_device; ++q)
+                if (dsts[q]) std::memset(dsts[q], 0, (size_t)rb * ((q < 4) ? plane : (size_t)n_a));
+        }
     }
     TPCF_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     TPCF_HIP(ctx, hipStreamSynchronize(s));

@@ -312,14 +312,23 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
             w.t[QW] += sW;
             w.t[QK] += sK;
         }
+        // A lens' chunk may be split over several waves: add atomically.  The
+        // atomics are what a marks pass costs, so a lane whose four terms are
+        // all zero (no source with weight in this bin, or a masked lens) adds
+        // nothing (the marks start at +0 and adding zeros leaves them as they
+        // are), and with all m == 0 the K marks equal the W marks and are
+        // copied from them after the kernel.
         if (MARKS && L.valid) {
-            // a lens' chunk may be split over several waves: add atomically
-            double *mk = marks + ((size_t)L.li * n_bin + bin) * 4;
             const double nan_or_0 = zero_sep ? __builtin_nan("") : 0.0;
-            unsafeAtomicAdd(&mk[0], gs * daa * aG1 + nan_or_0);
-            unsafeAtomicAdd(&mk[1], gs * daa * aG2 + nan_or_0);
-            unsafeAtomicAdd(&mk[2], dwa * aK);
-            unsafeAtomicAdd(&mk[3], dwa * aW);
+            const double m0 = gs * daa * aG1 + nan_or_0, m1 = gs * daa * aG2 + nan_or_0;
+            const double m2 = dwa * aK, m3 = dwa * aW;
+            if (m0 != 0.0 || m1 != 0.0 || m2 != 0.0 || m3 != 0.0) {   // NaN != 0
+                double *mk = marks + ((size_t)L.li * n_bin + bin) * 4;
+                unsafeAtomicAdd(&mk[0], m0);
+                unsafeAtomicAdd(&mk[1], m1);
+                if (!MZ) unsafeAtomicAdd(&mk[2], m2);
+                unsafeAtomicAdd(&mk[3], m3);
+            }
         }
     }
     win_clear<MZ>(w, lane);

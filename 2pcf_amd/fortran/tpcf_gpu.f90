@@ -28,7 +28,7 @@ module tpcf_gpu_c
                                      TPCF_ERR_HIP = 4, TPCF_ERR_UNSUPPORTED = 5, TPCF_ERR_NONFINITE = 6, &
                                      TPCF_ERR_NODEVICE = 7, TPCF_ERR_PATCH = 8, TPCF_ERR_GRID = 9
     integer(c_int32_t), parameter :: TPCF_DEVICE_INPUTS = 1, TPCF_SPHERICAL = 2, TPCF_VERBOSE = 4, &
-                                     TPCF_NO_KD_TREE = 8
+                                     TPCF_NO_KD_TREE = 8, TPCF_DEVICE_MARKS = 64
     integer(c_int32_t), parameter :: TPCF_OPT_CELL_SIZE = 1, TPCF_OPT_COUNTERS = 2, TPCF_OPT_CELL_TARGET = 3, &
                                      TPCF_OPT_SPLIT = 4
 

@@ -204,12 +204,15 @@ def _host(a, dtype):
 def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str = "f32", marks: bool = False,
               patch_A=None, patch_B=None, n_patch_A: int = 1, n_patch_B: int = 1, real_bytes: int = 4,
               device_inputs: bool = False, verbose: bool = False, ctx: Optional[Context] = None,
-              n_A: Optional[int] = None, n_B: Optional[int] = None, spherical: bool = False) -> Shear2pcfResults:
+              n_A: Optional[int] = None, n_B: Optional[int] = None, spherical: bool = False,
+              marks_device: bool = False) -> Shear2pcfResults:
     """One call of the C-ABI.  ``lens``: x, y, val, w (+ z for shear);
     ``src``: x, y, w + (z, e1, e2, m) for shear or (kappa_E, kappa_B) for kappa.
     With ``device_inputs`` the values are device addresses (ints) of arrays of
     the given precision and ``n_A``/``n_B`` give the sizes.  With ``spherical``
-    x, y are longitude and latitude and the bins angles, all in radians."""
+    x, y are longitude and latitude and the bins angles, all in radians.  With
+    ``marks`` and ``marks_device`` the marks are torch tensors on the current
+    GPU (TPCF_DEVICE_MARKS: no copy to the host)."""
     ctx = ctx or default_context()
     L = N.lib()
     dt = np.float32 if precision == "f32" else np.float64
@@ -237,7 +240,18 @@ def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str
     o.real_bytes = real_bytes
     for k, v in res.items():
         setattr(o, k, v.ctypes.data_as(C.c_void_p))
-    if marks:
+    if marks and marks_device:
+        import torch
+        tt = torch.float32 if real_bytes == 4 else torch.float64
+        for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
+            res[k] = torch.empty((n_A, nb), dtype=tt, device="cuda")
+            setattr(o, k, C.c_void_p(res[k].data_ptr()))
+        res["marks_weight"] = torch.empty(n_A, dtype=tt, device="cuda")
+        o.marks_weight = C.c_void_p(res["marks_weight"].data_ptr())
+        # the library writes them on its own stream: torch's pending work on
+        # this memory (a reuse by the caching allocator) ends first
+        torch.cuda.current_stream().synchronize()
+    elif marks:
         for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
             res[k] = np.zeros((n_A, nb), dtype=rt)
             setattr(o, k, res[k].ctypes.data_as(C.c_void_p))
@@ -253,7 +267,7 @@ def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str
             res[k] = np.zeros((P_B, P_A, nb), dtype=np.float64)
             setattr(o, k, res[k].ctypes.data_as(C.c_void_p))
     flags = (N.FLAG_DEVICE_INPUTS if device_inputs else 0) | (N.FLAG_VERBOSE if verbose else 0) | \
-        (N.FLAG_SPHERICAL if spherical else 0)
+        (N.FLAG_SPHERICAL if spherical else 0) | (N.FLAG_DEVICE_MARKS if marks and marks_device else 0)
     pa = ptr(patch_A, np.int32) if patch_A is not None else None
     pb = ptr(patch_B, np.int32) if patch_B is not None else None
     if kind == SHEAR:

@@ -370,7 +370,7 @@ def main() -> int:
                             patch_B=pb.data_ptr() if pb is not None else None,
                             n_patch_A=wl.n_patch, n_patch_B=wl.n_patch, real_bytes=4, device_inputs=True,
                             ctx=ctx, n_A=len(wl.lens["x"]), n_B=len(wl.src["x"]), spherical=wl.spherical,
-                            marks=args.marks)
+                            marks=args.marks, marks_device=args.marks)
 
     for i in range(args.warmup):
         r = step()

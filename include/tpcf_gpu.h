@@ -60,7 +60,11 @@ enum tpcf_flags {
     TPCF_VERBOSE = 1u << 2,        /* verbose = .true.: print a summary to stdout */
     TPCF_NO_KD_TREE = 1u << 3,     /* use_kd_tree = .false.: accepted; the pair set is identical */
     TPCF_BATCH_AUTOTUNE = 1u << 4, /* batch: time candidate cell / work-item sizes first */
-    TPCF_BATCH_NO_GRAPH = 1u << 5  /* batch: run eagerly instead of capturing a HIP graph */
+    TPCF_BATCH_NO_GRAPH = 1u << 5, /* batch: run eagerly instead of capturing a HIP graph */
+    TPCF_DEVICE_MARKS = 1u << 6    /* tpcf_shear_* / tpcf_kappa_*: the marks outputs (xi_E_marks, xi_B_marks,
+                                      K_marks, w_marks, marks_weight) are device pointers, written in place with
+                                      no copy to the host (the marks of a large catalogue are 4 n_bin n_A reals:
+                                      a consumer on the GPU does not pay the PCIe transfer) */
 };
 
 /* Catalogue kinds. */

@@ -139,6 +139,70 @@ def test_shear_masks_and_zero_weights(tp, orc, ctx):
     check(tp, orc, SHEAR, lens, src, bins, marks=True, ctx=ctx)
 
 
+def test_shear_marks_m_zero_cancelling_weights(tp, orc, ctx):
+    """m == 0 (the K marks are copied from the W marks after the kernel) and
+    lenses whose first bin holds only two sources of weights +w and -w: the
+    bin's W and K marks are exactly 0 while xi_E / xi_B are not, so a marks
+    pass that skipped lanes by their weight sum alone would lose them."""
+    lens, src = uniform_case(52, 3000, 4000, 70.0, SHEAR)
+    src["m"][:] = 0
+    bins = tp.create_bins(1.0, 20.0, 12, tp.LOG)
+    e1 = bins.bin_edges[1]
+    planted = np.arange(0, 3000, 300)
+    keep = np.ones(len(src["x"]), bool)
+    for i in planted:
+        keep &= np.hypot(src["x"] - lens["x"][i], src["y"] - lens["y"][i]) > 1.2 * e1
+    src = {k: v[keep] for k, v in src.items()}
+    r = _rng(53)
+    add = {k: [] for k in src}
+    for i in planted:
+        ang = r.uniform(0, 2 * np.pi)
+        for sgn in (1.0, -1.0):
+            add["x"].append(lens["x"][i] + 1.1 * np.cos(ang))
+            add["y"].append(lens["y"][i] + 1.1 * np.sin(ang))
+            add["z"].append(1.0)
+            add["e1"].append(r.normal(0, 0.27))
+            add["e2"].append(r.normal(0, 0.27))
+            add["w"].append(sgn * 0.75)
+            add["m"].append(0.0)
+    src = {k: np.concatenate([v, np.asarray(add[k], np.float32)]) for k, v in src.items()}
+    g, o = check(tp, orc, SHEAR, lens, src, bins, marks=True, ctx=ctx)
+    for i in planted:
+        assert o["w_marks"][i, 0] == 0 and o["k_marks"][i, 0] == 0 and o["xi_e"][i, 0] != 0
+        assert g.w_marks[0, i] == 0 and g.K_marks[0, i] == 0
+        np.testing.assert_allclose(g.xi_E_marks[0, i], o["xi_e"][i, 0], rtol=1e-6)
+        np.testing.assert_allclose(g.xi_B_marks[0, i], o["xi_b"][i, 0], rtol=1e-6)
+
+
+@pytest.mark.parametrize("kind", [KAPPA, SHEAR])
+@pytest.mark.parametrize("real_bytes", [4, 8])
+def test_device_marks(tp, ctx, kind, real_bytes):
+    """TPCF_DEVICE_MARKS: the marks written into device arrays equal the
+    marks copied to host arrays (the FP64 atomics add in any order: equal to
+    1e-12 of each lens' row scale), lenses outside the pair set (zero weight,
+    or no source in range) included; and an empty pair set zeroes them."""
+    import torch
+    lens, src = uniform_case(58 + kind, 2500, 3000, 70.0, kind, zero_w=True)
+    lens["x"][:50] += 500.0   # far from every source
+    bins = tp.create_bins(0.5, 20.0, 12, tp.LOG)
+    h = tp.correlate(kind, lens, src, bins, marks=True, real_bytes=real_bytes, ctx=ctx)
+    d = tp.correlate(kind, lens, src, bins, marks=True, real_bytes=real_bytes, ctx=ctx, marks_device=True)
+    np.testing.assert_array_equal(d.n_pairs, h.n_pairs)
+    for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
+        dv, hv = getattr(d, k), np.asarray(getattr(h, k), np.float64)
+        assert isinstance(dv, torch.Tensor) and dv.is_cuda and tuple(dv.shape) == hv.shape
+        dv = dv.cpu().numpy().astype(np.float64)
+        scale = np.abs(hv).sum(axis=0, keepdims=True) + 1e-300
+        assert np.max(np.abs(dv - hv) / scale) <= (1e-6 if real_bytes == 4 else 1e-12), k
+        assert np.all(dv[:, :50] == 0), k
+    np.testing.assert_allclose(d.marks_weight.cpu().numpy(), h.marks_weight, rtol=1e-6 if real_bytes == 4 else 1e-12)
+    far = {k: (v + 1000.0 if k == "x" else v) for k, v in src.items()}
+    e = tp.correlate(kind, lens, far, bins, marks=True, real_bytes=real_bytes, ctx=ctx, marks_device=True)
+    assert e.n_pairs.sum() == 0
+    for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks", "marks_weight"):
+        assert not torch.any(getattr(e, k) != 0), k
+
+
 @pytest.mark.parametrize("kind", [KAPPA, SHEAR])
 def test_fp64_path(tp, orc, ctx, kind):
     lens, src = uniform_case(61 + kind, 3000, 3000, 60.0, kind)
