@@ -4,7 +4,10 @@ A step is one full correlation call (tpcf_shear_f32 / tpcf_kappa_*) over the
 workload with the catalogues already resident in HBM: cell sort, segment and
 chunk tables, the pair kernel, the patch-pair reduction and the copy of the
 per-bin and per-patch-pair results to the host.  ``value`` is in-bin pairs per
-second of wall-clock time over the timed steps.
+second of wall-clock time over the timed steps.  ``--host-io`` instead passes
+host arrays in and out, as a host driver calls the C-ABI (the copies over PCIe
+are then inside every step); ``--marks`` adds the per-lens marks (into device
+arrays, or host arrays with ``--host-io``).
 
 Default workload: 20M kappa lenses x 20M shear sources on a 3000'x3000'
 lognormal-clustered patch, 30 log bins 0.5'-250', 128 patches, FP32 geometry
@@ -329,6 +332,9 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true", help="mocks_256: skip the batch autotuner")
     ap.add_argument("--marks", action="store_true",
                     help="compute the per-lens marks as well (the reference's call always does)")
+    ap.add_argument("--host-io", action="store_true",
+                    help="host arrays in and out (the C-ABI as a host driver calls it: H2D of the catalogues "
+                         "and D2H of every output inside each step)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -363,7 +369,15 @@ def main() -> int:
     if args.cell > 0:
         ctx.set_option(1, args.cell)
 
-    def step():
+    hl = {k: np.ascontiguousarray(v) for k, v in wl.lens.items()}
+    hs = {k: np.ascontiguousarray(v) for k, v in wl.src.items()}
+
+    def host_step():
+        return tp.correlate(wl.kind, hl, hs, bins, precision=prec, patch_A=wl.patch_A, patch_B=wl.patch_B,
+                            n_patch_A=wl.n_patch, n_patch_B=wl.n_patch, real_bytes=4, ctx=ctx,
+                            spherical=wl.spherical, marks=args.marks)
+
+    def device_step():
         return tp.correlate(wl.kind, {k: v.data_ptr() for k, v in dl.items()},
                             {k: v.data_ptr() for k, v in ds.items()}, bins, precision=prec,
                             patch_A=pa.data_ptr() if pa is not None else None,
@@ -372,6 +386,7 @@ def main() -> int:
                             ctx=ctx, n_A=len(wl.lens["x"]), n_B=len(wl.src["x"]), spherical=wl.spherical,
                             marks=args.marks, marks_device=args.marks)
 
+    step = host_step if args.host_io else device_step
     for i in range(args.warmup):
         r = step()
         log(f"warmup {i}: {r.stats['ms_total']:.1f} ms (pairs kernel {r.stats['ms_pairs']:.1f} ms)")
@@ -383,8 +398,8 @@ def main() -> int:
         kernel_ms.append(r.stats["ms_pairs"])
         pairs += int(r.n_pairs.sum())
         stats = r.stats
-        log(f"step {i}: {r.stats['ms_total']:.1f} ms (prepare {r.stats['ms_prepare']:.1f}, "
-            f"pairs kernel {r.stats['ms_pairs']:.1f}, finish {r.stats['ms_finish']:.1f})")
+        log(f"step {i}: {r.stats['ms_total']:.1f} ms (upload {r.stats['ms_upload']:.1f}, prepare "
+            f"{r.stats['ms_prepare']:.1f}, pairs kernel {r.stats['ms_pairs']:.1f}, finish {r.stats['ms_finish']:.1f})")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = pairs / elapsed
@@ -401,7 +416,9 @@ def main() -> int:
                    "n_bin": wl.n_bin, "theta": [wl.x_min, wl.x_max], "patches": wl.n_patch,
                    "kind": "shear-scalar" if wl.kind == 0 else "kappa-kappa", "scale": args.scale,
                    "pairs_per_step": per_step_pairs, "cell_size": stats["cell_size"],
-                   "spherical": wl.spherical, "marks": args.marks, "parallelism": "single GPU"},
+                   "spherical": wl.spherical, "marks": args.marks,
+                   "io": "host arrays in and out" if args.host_io else "inputs resident in HBM",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": fpp},
