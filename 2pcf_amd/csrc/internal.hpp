@@ -193,7 +193,7 @@ struct tpcf_ctx {
         B_PATCH_A, B_PATCH_B,
         B_PARTIAL, B_ERR,
         B_SKEY, B_SKEY2, B_SIDX, B_SIDX2, B_COUNTS, B_CSTART, B_SEGFIRST, B_SREC, B_SZ, B_SEGS, B_PBOX,
-        B_LKEY, B_LKEY2, B_LIDX, B_LIDX2, B_PCOUNT, B_CHOFF, B_PASTART,
+        B_LKEY, B_LKEY2, B_LIDX, B_LIDX2, B_PCOUNT,
         B_LX, B_LY, B_LZ, B_LA, B_LW, B_CHUNKS,
         B_PP, B_PPN, B_MARKS, B_OUT, B_TMP, B_PROBE, B_E2,
         B_BLK, B_BLKCH, B_WORK, B_ITEMS,

@@ -148,6 +148,7 @@ class Context:
     """Owns a ``tpcf_ctx`` (device, stream, reusable device buffers)."""
 
     def __init__(self, device: int = -1):
+        self.device = device   # -1: the device current at creation
         self._h = C.c_void_p()
         rc = N.lib().tpcf_ctx_create(device, C.byref(self._h))
         if rc:
@@ -243,14 +244,15 @@ def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str
     if marks and marks_device:
         import torch
         tt = torch.float32 if real_bytes == 4 else torch.float64
+        dev = torch.device("cuda", ctx.device) if ctx.device >= 0 else torch.device("cuda")
         for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
-            res[k] = torch.empty((n_A, nb), dtype=tt, device="cuda")
+            res[k] = torch.empty((n_A, nb), dtype=tt, device=dev)
             setattr(o, k, C.c_void_p(res[k].data_ptr()))
-        res["marks_weight"] = torch.empty(n_A, dtype=tt, device="cuda")
+        res["marks_weight"] = torch.empty(n_A, dtype=tt, device=dev)
         o.marks_weight = C.c_void_p(res["marks_weight"].data_ptr())
         # the library writes them on its own stream: torch's pending work on
         # this memory (a reuse by the caching allocator) ends first
-        torch.cuda.current_stream().synchronize()
+        torch.cuda.current_stream(dev).synchronize()
     elif marks:
         for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
             res[k] = np.zeros((n_A, nb), dtype=rt)
