@@ -8,11 +8,12 @@ correlations with per-lens marks and per-patch-pair sums, behind the C-ABI of
 from . import _native  # noqa: F401
 from ._native import TpcfError  # noqa: F401
 from .tpcf import (IHS, KAPPA, LIN, LOG, SHEAR, SQRT, Batch, Binning, Context, KappaMap, LensCatalog,  # noqa: F401
-                   LensingSurveyField, Shear2pcfResults, ShapeCatalog, adjoint, block_sums, bootstrap, correlate,
-                   create_bins, default_context, find_bin_index, kappa_2pcf, kmeans, shear_2pcf, stack_fields)
+                   LensingSurveyField, Shear2pcfResults, ShapeCatalog, Tape, adjoint, adjoint_record, adjoint_replay,
+                   block_sums, bootstrap, correlate, create_bins, default_context, find_bin_index, kappa_2pcf, kmeans,
+                   shear_2pcf, stack_fields)
 
 __all__ = [
     "TpcfError", "Batch", "Binning", "Context", "KappaMap", "LensCatalog", "LensingSurveyField", "Shear2pcfResults",
-    "ShapeCatalog", "adjoint", "block_sums", "bootstrap", "correlate", "create_bins", "default_context",
+    "ShapeCatalog", "Tape", "adjoint", "adjoint_record", "adjoint_replay", "block_sums", "bootstrap", "correlate", "create_bins", "default_context",
     "find_bin_index", "kappa_2pcf", "kmeans", "shear_2pcf", "stack_fields", "LIN", "LOG", "IHS", "SQRT", "SHEAR", "KAPPA",
 ]

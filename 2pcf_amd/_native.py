@@ -41,6 +41,8 @@ EXPORTS = (
     "tpcf_adjoint_shear_f32", "tpcf_adjoint_shear_f64", "tpcf_adjoint_kappa_f32", "tpcf_adjoint_kappa_f64",
     "tpcf_batch_create_f32", "tpcf_batch_create_f64", "tpcf_batch_run", "tpcf_batch_get_stats", "tpcf_batch_destroy",
     "tpcf_bootstrap", "tpcf_kmeans_f32", "tpcf_kmeans_f64",
+    "tpcf_adjoint_record_shear_f32", "tpcf_adjoint_record_shear_f64", "tpcf_adjoint_record_kappa_f32",
+    "tpcf_adjoint_record_kappa_f64", "tpcf_adjoint_replay", "tpcf_tape_destroy",
 )
 
 
@@ -171,6 +173,17 @@ def lib():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int
+    rec_shear = shear_args[:-7] + [C.POINTER(Bins), C.c_uint32, _vp, _vp, C.POINTER(_vp)]
+    rec_kappa = kappa_args[:-7] + [C.POINTER(Bins), C.c_uint32, _vp, _vp, C.POINTER(_vp)]
+    for name, args in (("tpcf_adjoint_record_shear_f32", rec_shear), ("tpcf_adjoint_record_shear_f64", rec_shear),
+                       ("tpcf_adjoint_record_kappa_f32", rec_kappa), ("tpcf_adjoint_record_kappa_f64", rec_kappa)):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.tpcf_adjoint_replay.argtypes = [_vp, _vp, _vp, _vp] + [_vp] * 6
+    L.tpcf_adjoint_replay.restype = C.c_int
+    L.tpcf_tape_destroy.argtypes = [_vp]
+    L.tpcf_tape_destroy.restype = C.c_int
     for name in ("tpcf_batch_create_f32", "tpcf_batch_create_f64"):
         f = getattr(L, name)
         f.argtypes = [_vp, _i32, C.POINTER(Catalog), C.POINTER(Bins), C.c_uint32, C.POINTER(_vp)]

@@ -5,14 +5,16 @@
 
 Arguments are CUDA (HIP) tensors of one float dtype (float32: the FP32 path,
 float64: the FP64 path).  The outputs are float64 tensors [n_bin] on the same
-device.  Backward runs the library's adjoint (tpcf_adjoint_*) and returns
-gradients for val and w of the lenses and for the value components, w (and m)
-of the sources; positions and z get none.  The same tensor may be passed on
-both sides (auto-correlation): autograd adds the two contributions.
+device.  The forward records the library's lens-side adjoint pass
+(tpcf_adjoint_record_*), whose per-lens sums also give the outputs; backward
+replays it (tpcf_adjoint_replay: the source-side pass only), so a forward and
+a backward cost two pair passes.  Backward returns gradients for val and w of
+the lenses and for the value components, w (and m) of the sources; positions
+and z get none.  The same tensor may be passed on both sides
+(auto-correlation): autograd adds the two contributions.
 """
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from . import tpcf as T
@@ -42,15 +44,15 @@ class _Correlation(torch.autograd.Function):
                 raise ValueError("inputs must be contiguous device tensors")
         prec = _prec(lens["x"])
         torch.cuda.current_stream().synchronize()
-        r = T.correlate(kind, {k: v.data_ptr() for k, v in lens.items()}, {k: v.data_ptr() for k, v in src.items()},
-                        bins, precision=prec, real_bytes=8, device_inputs=True, ctx=tctx,
-                        n_A=lens["x"].numel(), n_B=src["x"].numel())
-        ctx.kind, ctx.bins, ctx.tctx, ctx.prec = kind, bins, tctx, prec
+        g1, g2, tape = T.adjoint_record(kind, {k: v.data_ptr() for k, v in lens.items()},
+                                        {k: v.data_ptr() for k, v in src.items()}, bins, precision=prec, ctx=tctx,
+                                        n_A=lens["x"].numel(), n_B=src["x"].numel())
+        ctx.kind, ctx.bins, ctx.tape = kind, bins, tape
+        # keeps the inputs alive (the tape holds their addresses) and detects
+        # their modification in place before backward
         ctx.save_for_backward(*tensors)
         dev = lens["x"].device
-        out1 = torch.as_tensor(np.asarray(r.g_t, np.float64), device=dev)
-        out2 = torch.as_tensor(np.asarray(r.g_x, np.float64), device=dev)
-        return out1, out2
+        return torch.as_tensor(g1, device=dev), torch.as_tensor(g2, device=dev)
 
     @staticmethod
     def backward(ctx, gbar1, gbar2):
@@ -74,9 +76,7 @@ class _Correlation(torch.autograd.Function):
         g2 = (gbar2 if gbar2 is not None else torch.zeros(ctx.bins.n_bin)).detach().double().cpu().numpy()
         if bufs:
             torch.cuda.current_stream().synchronize()
-            T.adjoint(kind, {k: v.data_ptr() for k, v in lens.items()}, {k: v.data_ptr() for k, v in src.items()},
-                      ctx.bins, g1, g2, precision=ctx.prec, device_inputs=True, grads=grads, ctx=ctx.tctx,
-                      n_A=lens["x"].numel(), n_B=src["x"].numel())
+            T.adjoint_replay(ctx.tape, g1, g2, grads)
         out = []
         for side, keys, tensors_ in (("lens", lk, lens), ("src", sk, src)):
             for k in keys:

@@ -152,20 +152,156 @@ int upload_in(tpcf_ctx *ctx, int which, const T *src, int64_t n, const T **dst)
     return TPCF_OK;
 }
 
+// The lens-side pass and what the rest of the adjoint needs from it: the
+// marks of the sorted lenses (P or S^E, Q or S^B, kap, Om per bin), the lens
+// order and the forward per-bin sums G, X, K (kappa: E, B, W).
+struct LensSide {
+    const double *mk = nullptr;
+    const uint32_t *lidx = nullptr;
+    int64_t n_lens = 0;
+    std::vector<double> fwd;   // [n_bin][3]
+};
+
+// Device arrays of n ones and n zeros (the unit values and weights of the
+// passes), filled on the context's stream.
 template <typename T>
-int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala, const T *wa,
-            int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b, const T *wb,
-            const T *mb, const tpcf_bins *bins, uint32_t flags, const double *gbar1, const double *gbar2,
-            double *dval_a, double *dw_a, double *d1_b, double *d2_b, double *dw_b, double *dm_b)
+int unit_arrays(tpcf_ctx *ctx, int64_t n, T **ones, T **zeros)
 {
-    if (!ctx) return TPCF_ERR_ARG;
-    ctx->err.clear();
+    const int64_t n_max = std::max<int64_t>(n, 1);
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_ONES, sizeof(T) * n_max, (void **)ones));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_ZEROS, sizeof(T) * n_max, (void **)zeros));
+    const unsigned fb = (unsigned)std::min<int64_t>(4096, (n_max + 255) / 256);
+    hipLaunchKernelGGL(k_fill_value<T>, dim3(fb), dim3(256), 0, ctx->stream, *ones, n_max, (T)1);
+    hipLaunchKernelGGL(k_fill_value<T>, dim3(fb), dim3(256), 0, ctx->stream, *zeros, n_max, (T)0);
+    TPCF_HIP(ctx, hipGetLastError());
+    return TPCF_OK;
+}
+
+// Lens side: unit lens values and weights, real sources.  The marks stay in
+// the context's marks buffer and the lens order in its preparation buffers
+// (both valid until the next pass on this context); the forward sums are
+// contracted from the marks with a_i and w_i.
+template <typename T>
+int lens_pass(tpcf_ctx *ctx, const CallArgs<T> &base, uint32_t run_flags, double t0, LensSide &ls)
+{
+    hipStream_t s = ctx->stream;
+    const int n_bin = base.bins->n_bin;
+    ls.fwd.assign(3 * (size_t)n_bin, 0.0);
+    ls.n_lens = 0;
+    if (base.n_a == 0 || base.n_b == 0) return TPCF_OK;
+    T *ones, *zeros;
+    TPCF_TRY(unit_arrays<T>(ctx, std::max(base.n_a, base.n_b), &ones, &zeros));
+    CallArgs<T> l = base;
+    l.vala = ones;
+    l.wa = ones;
+    l.marks = true;
+    l.marks_device = true;
+    tpcf_out none;
+    std::memset(&none, 0, sizeof none);
+    none.real_bytes = 8;
+    Prepared<T> p;
+    TPCF_TRY(run_call<T>(ctx, l, run_flags, &none, t0, &p));
+    ls.mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
+    ls.lidx = p.lidx;
+    ls.n_lens = p.n_lens;
+    if (p.n_lens > 0) {
+        double *fs;
+        TPCF_TRY(ensure(ctx, tpcf_ctx::B_TMP, sizeof(double) * 3 * n_bin, (void **)&fs));
+        fill_f64(s, fs, 3 * n_bin);
+        const unsigned gx = (unsigned)std::min<int64_t>(1024, (p.n_lens + 255) / 256);
+        hipLaunchKernelGGL(k_adj_fwd<T>, dim3(gx, (unsigned)n_bin), dim3(256), 0, s, p.n_lens, n_bin, p.lidx, ls.mk,
+                           base.vala, base.wa, base.kind, fs);
+        TPCF_HIP(ctx, hipGetLastError());
+        TPCF_HIP(ctx, hipMemcpyAsync(ls.fwd.data(), fs, sizeof(double) * ls.fwd.size(), hipMemcpyDeviceToHost, s));
+        TPCF_HIP(ctx, hipStreamSynchronize(s));
+    }
+    return TPCF_OK;
+}
+
+// Per-bin coefficients of the contraction: alpha = lbar_1 / K, alpha' =
+// lbar_2 / K, beta = (lbar_1 g_1 + lbar_2 g_2) / K, zero where K <= 0 (the
+// output is 0 there, with no dependence).
+Coef coef_of(const std::vector<double> &fwd, int n_bin, const double *gbar1, const double *gbar2)
+{
+    Coef c;
+    std::memset(&c, 0, sizeof c);
+    for (int b = 0; b < n_bin; ++b) {
+        const double G = fwd[3 * b], X = fwd[3 * b + 1], K = fwd[3 * b + 2];
+        if (!(K > 0.0)) continue;
+        const double g1 = G / K, g2 = X / K;
+        c.at[b] = gbar1[b] / K;
+        c.ax[b] = gbar2[b] / K;
+        c.be[b] = (gbar1[b] * g1 + gbar2[b] * g2) / K;
+    }
+    return c;
+}
+
+template <typename T>
+int lens_grad(tpcf_ctx *ctx, const LensSide &ls, const CallArgs<T> &base, const Coef &c, double *dvA, double *dwA)
+{
+    if (ls.n_lens == 0 || !(dvA || dwA)) return TPCF_OK;
+    hipLaunchKernelGGL(k_adj_lens<T>, dim3((unsigned)((ls.n_lens + 255) / 256)), dim3(256), 0, ctx->stream, ls.n_lens,
+                       base.bins->n_bin, ls.lidx, ls.mk, base.vala, base.wa, c, dvA, dwA);
+    TPCF_HIP(ctx, hipGetLastError());
+    return TPCF_OK;
+}
+
+// Source side: the sources as lanes, the lenses streamed with (a_i, 0).
+template <typename T>
+int src_pass(tpcf_ctx *ctx, const CallArgs<T> &base, uint32_t run_flags, double t0, const Coef &c, double *d1B,
+             double *d2B, double *dwB, double *dmB)
+{
+    if (base.n_a == 0 || base.n_b == 0 || !(d1B || d2B || dwB || dmB)) return TPCF_OK;
+    hipStream_t s = ctx->stream;
+    const bool shear = base.kind == SHEAR;
+    T *ones, *zeros;
+    TPCF_TRY(unit_arrays<T>(ctx, std::max(base.n_a, base.n_b), &ones, &zeros));
+    CallArgs<T> r = base;
+    r.n_a = base.n_b;
+    r.xa = base.xb; r.ya = base.yb; r.za = base.zb;
+    r.vala = ones;
+    r.wa = ones;
+    r.n_b = base.n_a;
+    r.xb = base.xa; r.yb = base.ya; r.zb = base.za;
+    r.s1b = base.vala;
+    r.s2b = zeros;
+    r.wb = base.wa;
+    r.mb = shear ? zeros : nullptr;
+    r.zsign = -1;   // the z test z_A > z_B of the original roles
+    r.marks = true;
+    r.marks_device = true;
+    tpcf_out none;
+    std::memset(&none, 0, sizeof none);
+    none.real_bytes = 8;
+    Prepared<T> p;
+    TPCF_TRY(run_call<T>(ctx, r, run_flags, &none, t0, &p));
+    const double *mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
+    if (p.n_lens > 0) {
+        const int n_bin = base.bins->n_bin;
+        if (shear)
+            hipLaunchKernelGGL((k_adj_src<T, SHEAR>), dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
+                               p.n_lens, n_bin, p.lidx, mk, base.s1b, base.s2b, base.wb, base.mb, c, d1B, d2B, dwB,
+                               dmB);
+        else
+            hipLaunchKernelGGL((k_adj_src<T, KAPPA>), dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
+                               p.n_lens, n_bin, p.lidx, mk, base.s1b, base.s2b, base.wb, (const T *)nullptr, c, d1B,
+                               d2B, dwB, (double *)nullptr);
+        TPCF_HIP(ctx, hipGetLastError());
+    }
+    return TPCF_OK;
+}
+
+// Argument checks shared by the adjoint entry points.
+template <typename T>
+int adj_check(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala,
+              const T *wa, int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b,
+              const T *wb, const T *mb, const tpcf_bins *bins, uint32_t flags)
+{
     if (!bins || bins->n_bin < 1 || bins->n_bin > TPCF_MAX_BIN) return fail(ctx, TPCF_ERR_BINS, "bad bins");
     if (flags & TPCF_SPHERICAL)
         return fail(ctx, TPCF_ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build");
     if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE))
         return fail(ctx, TPCF_ERR_ARG, "unknown flag");
-    if (!gbar1 || !gbar2) return fail(ctx, TPCF_ERR_ARG, "upstream gradients are NULL");
     if (n_a < 0 || n_b < 0 || n_a >= ((int64_t)1 << 31) || n_b >= ((int64_t)1 << 31))
         return fail(ctx, TPCF_ERR_ARG, "catalogue sizes must be in [0, 2^31)");
     const bool shear = (kind == SHEAR);
@@ -173,13 +309,14 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
         return fail(ctx, TPCF_ERR_ARG, "a required lens array is NULL");
     if (n_b > 0 && (!xb || !yb || !s1b || !s2b || !wb || (shear && (!zb || !mb))))
         return fail(ctx, TPCF_ERR_ARG, "a required source array is NULL");
-    TPCF_HIP(ctx, hipSetDevice(ctx->device));
-    const double t0 = now_ms();
-    hipStream_t s = ctx->stream;
-    const int n_bin = bins->n_bin;
-    const bool dev = (flags & TPCF_DEVICE_INPUTS) != 0;
-    const uint32_t run_flags = flags | TPCF_DEVICE_INPUTS;
+    return TPCF_OK;
+}
 
+template <typename T>
+CallArgs<T> adj_base(int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala, const T *wa,
+                     int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b, const T *wb,
+                     const T *mb, const tpcf_bins *bins, uint32_t flags)
+{
     CallArgs<T> base;
     base.kind = kind;
     base.n_a = n_a;
@@ -189,10 +326,31 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
     base.bins = bins;
     base.marks = false;
     base.verbose = (flags & TPCF_VERBOSE) != 0;
-    if (dev) {
-        base.xa = xa; base.ya = ya; base.za = za; base.vala = vala; base.wa = wa;
-        base.xb = xb; base.yb = yb; base.zb = zb; base.s1b = s1b; base.s2b = s2b; base.wb = wb; base.mb = mb;
-    } else {
+    base.xa = xa; base.ya = ya; base.za = za; base.vala = vala; base.wa = wa;
+    base.xb = xb; base.yb = yb; base.zb = zb; base.s1b = s1b; base.s2b = s2b; base.wb = wb; base.mb = mb;
+    if (kind != SHEAR) { base.za = nullptr; base.zb = nullptr; base.mb = nullptr; }
+    return base;
+}
+
+template <typename T>
+int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala, const T *wa,
+            int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b, const T *wb,
+            const T *mb, const tpcf_bins *bins, uint32_t flags, const double *gbar1, const double *gbar2,
+            double *dval_a, double *dw_a, double *d1_b, double *d2_b, double *dw_b, double *dm_b)
+{
+    if (!ctx) return TPCF_ERR_ARG;
+    ctx->err.clear();
+    TPCF_TRY(adj_check<T>(ctx, kind, n_a, xa, ya, za, vala, wa, n_b, xb, yb, zb, s1b, s2b, wb, mb, bins, flags));
+    if (!gbar1 || !gbar2) return fail(ctx, TPCF_ERR_ARG, "upstream gradients are NULL");
+    TPCF_HIP(ctx, hipSetDevice(ctx->device));
+    const double t0 = now_ms();
+    hipStream_t s = ctx->stream;
+    const bool shear = (kind == SHEAR);
+    const bool dev = (flags & TPCF_DEVICE_INPUTS) != 0;
+    const uint32_t run_flags = flags | TPCF_DEVICE_INPUTS;
+
+    CallArgs<T> base = adj_base<T>(kind, n_a, xa, ya, za, vala, wa, n_b, xb, yb, zb, s1b, s2b, wb, mb, bins, flags);
+    if (!dev) {
         TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN0, xa, n_a, &base.xa));
         TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN1, ya, n_a, &base.ya));
         TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN2, shear ? za : (const T *)nullptr, n_a, &base.za));
@@ -206,10 +364,6 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
         TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN10, wb, n_b, &base.wb));
         TPCF_TRY(upload_in(ctx, tpcf_ctx::B_IN11, shear ? mb : (const T *)nullptr, n_b, &base.mb));
     }
-    if (!shear) { base.za = nullptr; base.zb = nullptr; base.mb = nullptr; }
-
-    Coef c;
-    std::memset(&c, 0, sizeof c);
 
     // ---- gradient buffers (device), zero for masked objects ----
     double *gA, *gB;
@@ -218,110 +372,146 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
     double *dvA = dev ? dval_a : gA, *dwA = dev ? dw_a : gA + n_a;
     double *d1B = dev ? d1_b : gB, *d2B = dev ? d2_b : gB + n_b, *dwB = dev ? dw_b : gB + 2 * n_b,
            *dmB = dev ? dm_b : gB + 3 * n_b;
-    if (!dev) {
-        // host outputs: every device slot is computed and only the requested ones copied
-    } else {
-        // device outputs: only the requested ones are computed
-        if (!dval_a) dvA = nullptr;
-        if (!dw_a) dwA = nullptr;
-        if (!d1_b) d1B = nullptr;
-        if (!d2_b) d2B = nullptr;
-        if (!dw_b) dwB = nullptr;
-        if (!dm_b || !shear) dmB = nullptr;
-    }
-    if (!shear && !dev) dmB = nullptr;
+    // host outputs: every device slot is computed and only the requested ones
+    // copied; device outputs: only the requested ones are computed
+    if (!shear) dmB = nullptr;
     double *zero_list[6] = {dvA, dwA, d1B, d2B, dwB, dmB};
     const int64_t zero_n[6] = {n_a, n_a, n_b, n_b, n_b, n_b};
     for (int q = 0; q < 6; ++q)
         if (zero_list[q] && zero_n[q] > 0) fill_f64(s, zero_list[q], zero_n[q]);
 
-    T *ones, *zeros;
-    const int64_t n_max = std::max<int64_t>(std::max(n_a, n_b), 1);
-    TPCF_TRY(ensure(ctx, tpcf_ctx::B_ONES, sizeof(T) * n_max, (void **)&ones));
-    TPCF_TRY(ensure(ctx, tpcf_ctx::B_ZEROS, sizeof(T) * n_max, (void **)&zeros));
-    const unsigned fb = (unsigned)std::min<int64_t>(4096, (n_max + 255) / 256);
-    hipLaunchKernelGGL(k_fill_value<T>, dim3(fb), dim3(256), 0, s, ones, n_max, (T)1);
-    hipLaunchKernelGGL(k_fill_value<T>, dim3(fb), dim3(256), 0, s, zeros, n_max, (T)0);
-    TPCF_HIP(ctx, hipGetLastError());
-
-    tpcf_out none;
-    std::memset(&none, 0, sizeof none);
-    none.real_bytes = 8;
-    // ---- lens side: unit lens values and weights, real sources; the forward
-    // sums G, X, K per bin follow from these marks ----
-    if (n_a > 0 && n_b > 0) {
-        CallArgs<T> l = base;
-        l.vala = ones;
-        l.wa = ones;
-        l.marks = true;
-        l.marks_device = true;
-        Prepared<T> p;
-        TPCF_TRY(run_call<T>(ctx, l, run_flags, &none, t0, &p));
-        const double *mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
-        double *fs;
-        TPCF_TRY(ensure(ctx, tpcf_ctx::B_TMP, sizeof(double) * 3 * n_bin, (void **)&fs));
-        fill_f64(s, fs, 3 * n_bin);
-        std::vector<double> h(3 * n_bin, 0.0);
-        if (p.n_lens > 0) {
-            const unsigned gx = (unsigned)std::min<int64_t>(1024, (p.n_lens + 255) / 256);
-            hipLaunchKernelGGL(k_adj_fwd<T>, dim3(gx, (unsigned)n_bin), dim3(256), 0, s, p.n_lens, n_bin, p.lidx, mk,
-                               base.vala, base.wa, kind, fs);
-            TPCF_HIP(ctx, hipGetLastError());
-            TPCF_HIP(ctx, hipMemcpyAsync(h.data(), fs, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
-            TPCF_HIP(ctx, hipStreamSynchronize(s));
-        }
-        for (int b = 0; b < n_bin; ++b) {
-            const double G = h[3 * b], X = h[3 * b + 1], K = h[3 * b + 2];
-            if (!(K > 0.0)) continue;   // g = 0 there: no dependence
-            const double g1 = G / K, g2 = X / K;
-            c.at[b] = gbar1[b] / K;
-            c.ax[b] = gbar2[b] / K;
-            c.be[b] = (gbar1[b] * g1 + gbar2[b] * g2) / K;
-        }
-        if (p.n_lens > 0 && (dvA || dwA)) {
-            hipLaunchKernelGGL(k_adj_lens<T>, dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s, p.n_lens, n_bin,
-                               p.lidx, mk, base.vala, base.wa, c, dvA, dwA);
-            TPCF_HIP(ctx, hipGetLastError());
-        }
-    }
-    // ---- source side: the sources as lanes, the lenses streamed (a_i, 0) ----
-    if (n_a > 0 && n_b > 0 && (d1B || d2B || dwB || dmB)) {
-        CallArgs<T> r = base;
-        r.n_a = n_b;
-        r.xa = base.xb; r.ya = base.yb; r.za = base.zb;
-        r.vala = ones;
-        r.wa = ones;
-        r.n_b = n_a;
-        r.xb = base.xa; r.yb = base.ya; r.zb = base.za;
-        r.s1b = base.vala;
-        r.s2b = zeros;
-        r.wb = base.wa;
-        r.mb = shear ? zeros : nullptr;
-        r.zsign = -1;   // the z test z_A > z_B of the original roles
-        r.marks = true;
-        r.marks_device = true;
-        Prepared<T> p;
-        TPCF_TRY(run_call<T>(ctx, r, run_flags, &none, t0, &p));
-        const double *mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
-        if (p.n_lens > 0) {
-            if (shear)
-                hipLaunchKernelGGL((k_adj_src<T, SHEAR>), dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
-                                   p.n_lens, n_bin, p.lidx, mk, base.s1b, base.s2b, base.wb, base.mb, c, d1B, d2B, dwB,
-                                   dmB);
-            else
-                hipLaunchKernelGGL((k_adj_src<T, KAPPA>), dim3((unsigned)((p.n_lens + 255) / 256)), dim3(256), 0, s,
-                                   p.n_lens, n_bin, p.lidx, mk, base.s1b, base.s2b, base.wb, (const T *)nullptr, c, d1B,
-                                   d2B, dwB, (double *)nullptr);
-            TPCF_HIP(ctx, hipGetLastError());
-        }
-    }
+    LensSide ls;
+    TPCF_TRY(lens_pass<T>(ctx, base, run_flags, t0, ls));
+    const Coef c = coef_of(ls.fwd, bins->n_bin, gbar1, gbar2);
+    TPCF_TRY(lens_grad<T>(ctx, ls, base, c, dvA, dwA));
+    TPCF_TRY(src_pass<T>(ctx, base, run_flags, t0, c, d1B, d2B, dwB, dmB));
     if (!dev) {
         double *dst[6] = {dval_a, dw_a, d1_b, d2_b, dw_b, shear ? dm_b : nullptr};
         for (int q = 0; q < 6; ++q)
             if (dst[q] && zero_n[q] > 0)
                 TPCF_HIP(ctx, hipMemcpyAsync(dst[q], zero_list[q], sizeof(double) * zero_n[q], hipMemcpyDeviceToHost, s));
-        if (!shear && dm_b && n_b > 0) std::memset(dm_b, 0, sizeof(double) * n_b);
     }
+    TPCF_HIP(ctx, hipStreamSynchronize(s));
+    ctx->stats.ms_total = now_ms() - t0;
+    return TPCF_OK;
+}
+
+}  // namespace
+}  // namespace tpcf
+
+// A recorded lens-side pass: the input pointers (caller-owned device arrays)
+// and copies of the lens-side marks, the lens order and the forward sums, so
+// that the gradient for any upstream gradient needs only the source-side pass.
+struct tpcf_tape {
+    int kind = 0;
+    int real_bytes = 4;
+    int device = 0;
+    tpcf_bins bins{};
+    int64_t n_a = 0, n_b = 0;
+    const void *in[12] = {};   // x_A, y_A, z_A, val_A, w_A, x_B, y_B, z_B, s1_B, s2_B, w_B, m_B
+    double *mk = nullptr;      // [n_lens][n_bin][4]
+    uint32_t *lidx = nullptr;  // [n_lens]
+    int64_t n_lens = 0;
+    std::vector<double> fwd;   // [n_bin][3]
+};
+
+namespace tpcf {
+namespace {
+
+void tape_free(tpcf_tape *t)
+{
+    if (!t) return;
+    (void)hipSetDevice(t->device);
+    if (t->mk) (void)hipFree(t->mk);
+    if (t->lidx) (void)hipFree(t->lidx);
+    delete t;
+}
+
+template <typename T>
+CallArgs<T> tape_base(const tpcf_tape &t)
+{
+    const T *const *in = reinterpret_cast<const T *const *>(t.in);
+    return adj_base<T>(t.kind, t.n_a, in[0], in[1], in[2], in[3], in[4], t.n_b, in[5], in[6], in[7], in[8], in[9],
+                       in[10], in[11], &t.bins, TPCF_DEVICE_INPUTS);
+}
+
+template <typename T>
+int record(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala, const T *wa,
+           int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b, const T *wb, const T *mb,
+           const tpcf_bins *bins, uint32_t flags, double *g1, double *g2, tpcf_tape **tape)
+{
+    if (!ctx) return TPCF_ERR_ARG;
+    ctx->err.clear();
+    if (!tape) return fail(ctx, TPCF_ERR_ARG, "tape is NULL");
+    *tape = nullptr;
+    TPCF_TRY(adj_check<T>(ctx, kind, n_a, xa, ya, za, vala, wa, n_b, xb, yb, zb, s1b, s2b, wb, mb, bins, flags));
+    if (!(flags & TPCF_DEVICE_INPUTS))
+        return fail(ctx, TPCF_ERR_ARG, "a recorded adjoint keeps the input pointers: TPCF_DEVICE_INPUTS is required");
+    TPCF_HIP(ctx, hipSetDevice(ctx->device));
+    const double t0 = now_ms();
+    hipStream_t s = ctx->stream;
+    tpcf_tape *t = new tpcf_tape;
+    t->kind = kind;
+    t->real_bytes = (int)sizeof(T);
+    t->device = ctx->device;
+    t->bins = *bins;
+    t->n_a = n_a;
+    t->n_b = n_b;
+    const T *in[12] = {xa, ya, za, vala, wa, xb, yb, zb, s1b, s2b, wb, mb};
+    for (int q = 0; q < 12; ++q) t->in[q] = in[q];
+    const CallArgs<T> base = tape_base<T>(*t);
+    LensSide ls;
+    int rc = lens_pass<T>(ctx, base, flags, t0, ls);
+    if (rc == TPCF_OK && ls.n_lens > 0) {
+        const size_t nm = sizeof(double) * 4 * (size_t)ls.n_lens * bins->n_bin;
+        if (hipMalloc((void **)&t->mk, nm) != hipSuccess || hipMalloc((void **)&t->lidx, 4 * ls.n_lens) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = fail(ctx, TPCF_ERR_ALLOC, "tape: device allocation of the lens-side marks failed");
+        } else {
+            t->n_lens = ls.n_lens;
+            const hipError_t e1 = hipMemcpyAsync(t->mk, ls.mk, nm, hipMemcpyDeviceToDevice, s);
+            const hipError_t e2 = hipMemcpyAsync(t->lidx, ls.lidx, 4 * ls.n_lens, hipMemcpyDeviceToDevice, s);
+            const hipError_t e3 = hipStreamSynchronize(s);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+                rc = fail(ctx, TPCF_ERR_HIP, "tape: copy of the lens-side marks failed");
+        }
+    }
+    if (rc != TPCF_OK) {
+        tape_free(t);
+        return rc;
+    }
+    t->fwd = ls.fwd;
+    for (int b = 0; b < bins->n_bin; ++b) {
+        const double G = t->fwd[3 * b], X = t->fwd[3 * b + 1], K = t->fwd[3 * b + 2];
+        if (g1) g1[b] = K > 0.0 ? G / K : 0.0;
+        if (g2) g2[b] = K > 0.0 ? X / K : 0.0;
+    }
+    ctx->stats.ms_total = now_ms() - t0;
+    *tape = t;
+    return TPCF_OK;
+}
+
+template <typename T>
+int replay(tpcf_ctx *ctx, const tpcf_tape *t, const double *gbar1, const double *gbar2, double *dval_a,
+           double *dw_a, double *d1_b, double *d2_b, double *dw_b, double *dm_b)
+{
+    TPCF_HIP(ctx, hipSetDevice(ctx->device));
+    const double t0 = now_ms();
+    hipStream_t s = ctx->stream;
+    const CallArgs<T> base = tape_base<T>(*t);
+    if (t->kind != SHEAR) dm_b = nullptr;
+    double *outs[6] = {dval_a, dw_a, d1_b, d2_b, dw_b, dm_b};
+    const int64_t n[6] = {t->n_a, t->n_a, t->n_b, t->n_b, t->n_b, t->n_b};
+    for (int q = 0; q < 6; ++q)
+        if (outs[q] && n[q] > 0) fill_f64(s, outs[q], n[q]);
+    LensSide ls;
+    ls.mk = t->mk;
+    ls.lidx = t->lidx;
+    ls.n_lens = t->n_lens;
+    ls.fwd = t->fwd;
+    const Coef c = coef_of(t->fwd, t->bins.n_bin, gbar1, gbar2);
+    TPCF_TRY(lens_grad<T>(ctx, ls, base, c, dval_a, dw_a));
+    TPCF_TRY(src_pass<T>(ctx, base, TPCF_DEVICE_INPUTS, t0, c, d1_b, d2_b, dw_b, dm_b));
     TPCF_HIP(ctx, hipStreamSynchronize(s));
     ctx->stats.ms_total = now_ms() - t0;
     return TPCF_OK;
@@ -376,6 +566,64 @@ int tpcf_adjoint_kappa_f64(tpcf_ctx *ctx, int64_t n_A, const double *x_A, const 
     return adjoint<double>(ctx, KAPPA, n_A, x_A, y_A, nullptr, val_A, w_A, n_B, x_B, y_B, nullptr, kappa_E, kappa_B,
                            w_B, nullptr, bins, flags, gbar_E, gbar_B, val_A_bar, w_A_bar, kE_B_bar, kB_B_bar, w_B_bar,
                            nullptr);
+}
+
+int tpcf_adjoint_record_shear_f32(tpcf_ctx *ctx, int64_t n_A, const float *x_A, const float *y_A, const float *z_A,
+                                  const float *val_A, const float *w_A, int64_t n_B, const float *x_B,
+                                  const float *y_B, const float *z_B, const float *e1_B, const float *e2_B,
+                                  const float *w_B, const float *m_B, const tpcf_bins *bins, uint32_t flags,
+                                  double *g_t, double *g_x, tpcf_tape **tape)
+{
+    return record<float>(ctx, SHEAR, n_A, x_A, y_A, z_A, val_A, w_A, n_B, x_B, y_B, z_B, e1_B, e2_B, w_B, m_B, bins,
+                         flags, g_t, g_x, tape);
+}
+
+int tpcf_adjoint_record_shear_f64(tpcf_ctx *ctx, int64_t n_A, const double *x_A, const double *y_A,
+                                  const double *z_A, const double *val_A, const double *w_A, int64_t n_B,
+                                  const double *x_B, const double *y_B, const double *z_B, const double *e1_B,
+                                  const double *e2_B, const double *w_B, const double *m_B, const tpcf_bins *bins,
+                                  uint32_t flags, double *g_t, double *g_x, tpcf_tape **tape)
+{
+    return record<double>(ctx, SHEAR, n_A, x_A, y_A, z_A, val_A, w_A, n_B, x_B, y_B, z_B, e1_B, e2_B, w_B, m_B, bins,
+                          flags, g_t, g_x, tape);
+}
+
+int tpcf_adjoint_record_kappa_f32(tpcf_ctx *ctx, int64_t n_A, const float *x_A, const float *y_A, const float *val_A,
+                                  const float *w_A, int64_t n_B, const float *x_B, const float *y_B,
+                                  const float *kappa_E, const float *kappa_B, const float *w_B, const tpcf_bins *bins,
+                                  uint32_t flags, double *xi_E, double *xi_B, tpcf_tape **tape)
+{
+    return record<float>(ctx, KAPPA, n_A, x_A, y_A, nullptr, val_A, w_A, n_B, x_B, y_B, nullptr, kappa_E, kappa_B,
+                         w_B, nullptr, bins, flags, xi_E, xi_B, tape);
+}
+
+int tpcf_adjoint_record_kappa_f64(tpcf_ctx *ctx, int64_t n_A, const double *x_A, const double *y_A,
+                                  const double *val_A, const double *w_A, int64_t n_B, const double *x_B,
+                                  const double *y_B, const double *kappa_E, const double *kappa_B, const double *w_B,
+                                  const tpcf_bins *bins, uint32_t flags, double *xi_E, double *xi_B, tpcf_tape **tape)
+{
+    return record<double>(ctx, KAPPA, n_A, x_A, y_A, nullptr, val_A, w_A, n_B, x_B, y_B, nullptr, kappa_E, kappa_B,
+                          w_B, nullptr, bins, flags, xi_E, xi_B, tape);
+}
+
+int tpcf_adjoint_replay(tpcf_ctx *ctx, const tpcf_tape *tape, const double *gbar_1, const double *gbar_2,
+                        double *val_A_bar, double *w_A_bar, double *s1_B_bar, double *s2_B_bar, double *w_B_bar,
+                        double *m_B_bar)
+{
+    if (!ctx) return TPCF_ERR_ARG;
+    ctx->err.clear();
+    if (!tape) return fail(ctx, TPCF_ERR_ARG, "tape is NULL");
+    if (!gbar_1 || !gbar_2) return fail(ctx, TPCF_ERR_ARG, "upstream gradients are NULL");
+    if (tape->device != ctx->device) return fail(ctx, TPCF_ERR_ARG, "the tape was recorded on another device");
+    if (tape->real_bytes == 4)
+        return replay<float>(ctx, tape, gbar_1, gbar_2, val_A_bar, w_A_bar, s1_B_bar, s2_B_bar, w_B_bar, m_B_bar);
+    return replay<double>(ctx, tape, gbar_1, gbar_2, val_A_bar, w_A_bar, s1_B_bar, s2_B_bar, w_B_bar, m_B_bar);
+}
+
+int tpcf_tape_destroy(tpcf_tape *tape)
+{
+    tape_free(tape);
+    return TPCF_OK;
 }
 
 }  // extern "C"

@@ -406,6 +406,82 @@ def adjoint(kind: int, lens: dict, src: dict, bins: Binning, gbar1, gbar2, *, pr
     return out if not device_inputs else {}
 
 
+class Tape:
+    """A recorded lens-side pass (``tpcf_tape``): ``adjoint_replay`` gives the
+    gradient for any upstream gradient with the source-side pass only.  Holds
+    device memory (4 n_bin doubles per used lens) until closed or collected."""
+
+    def __init__(self, handle: C.c_void_p, ctx: Context, kind: int, n_A: int, n_B: int, n_bin: int):
+        self._h, self.ctx, self.kind, self.n_A, self.n_B, self.n_bin = handle, ctx, kind, n_A, n_B, n_bin
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().tpcf_tape_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+
+def adjoint_record(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str = "f32",
+                   ctx: Optional[Context] = None, n_A: int, n_B: int):
+    """Forward outputs (g_t, g_x or xi_E, xi_B: FP64 arrays [n_bin]) and a
+    :class:`Tape` for ``adjoint_replay``.  ``lens`` / ``src`` hold device
+    addresses (ints) of arrays of the given precision, which must stay valid
+    and unchanged while the tape is replayed."""
+    ctx = ctx or default_context()
+    L = N.lib()
+    p = lambda v: C.c_void_p(int(v)) if v is not None else None   # noqa: E731
+    g1 = np.zeros(bins.n_bin, np.float64)
+    g2 = np.zeros(bins.n_bin, np.float64)
+    h = C.c_void_p()
+    if kind == SHEAR:
+        fn = L.tpcf_adjoint_record_shear_f32 if precision == "f32" else L.tpcf_adjoint_record_shear_f64
+        rc = fn(ctx.handle, n_A, p(lens["x"]), p(lens["y"]), p(lens["z"]), p(lens["val"]), p(lens["w"]),
+                n_B, p(src["x"]), p(src["y"]), p(src["z"]), p(src["e1"]), p(src["e2"]), p(src["w"]), p(src["m"]),
+                C.byref(bins.c), N.FLAG_DEVICE_INPUTS, g1.ctypes.data_as(C.c_void_p),
+                g2.ctypes.data_as(C.c_void_p), C.byref(h))
+    else:
+        fn = L.tpcf_adjoint_record_kappa_f32 if precision == "f32" else L.tpcf_adjoint_record_kappa_f64
+        rc = fn(ctx.handle, n_A, p(lens["x"]), p(lens["y"]), p(lens["val"]), p(lens["w"]),
+                n_B, p(src["x"]), p(src["y"]), p(src["kappa_E"]), p(src["kappa_B"]), p(src["w"]),
+                C.byref(bins.c), N.FLAG_DEVICE_INPUTS, g1.ctypes.data_as(C.c_void_p),
+                g2.ctypes.data_as(C.c_void_p), C.byref(h))
+    if rc:
+        raise TpcfError(rc, ctx.last_error())
+    return g1, g2, Tape(h, ctx, kind, n_A, n_B, bins.n_bin)
+
+
+def adjoint_replay(tape: Tape, gbar1, gbar2, grads: dict) -> None:
+    """Fill ``grads`` ({"lens": {"val", "w"}, "src": {...}} -> device
+    addresses of FP64 arrays; missing keys are not computed) with the gradient
+    of sum_b (gbar1_b g_1,b + gbar2_b g_2,b) for the recorded pass."""
+    L = N.lib()
+    g1 = np.ascontiguousarray(gbar1, dtype=np.float64)
+    g2 = np.ascontiguousarray(gbar2, dtype=np.float64)
+    if len(g1) != tape.n_bin or len(g2) != tape.n_bin:
+        raise ValueError("upstream gradients must have n_bin entries")
+    lk, sk = _GRAD_KEYS[tape.kind]
+    gptr = []
+    for side, keys in (("lens", lk), ("src", sk)):
+        for k in keys:
+            addr = (grads or {}).get(side, {}).get(k)
+            gptr.append(C.c_void_p(int(addr)) if addr is not None else None)
+    while len(gptr) < 6:
+        gptr.append(None)
+    rc = L.tpcf_adjoint_replay(tape.ctx.handle, tape.handle, g1.ctypes.data_as(C.c_void_p),
+                               g2.ctypes.data_as(C.c_void_p), *gptr)
+    if rc:
+        raise TpcfError(rc, tape.ctx.last_error())
+
+
 # ------------------------------------------------------------ bootstrap ----
 def block_sums(res: "Shear2pcfResults", patch_A, w_A, n_patch_A: int) -> dict:
     """Per lens patch ("block") sums of a correlation with per-patch-pair

@@ -259,9 +259,9 @@ def bench_adjoint(args) -> int:
            "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": "adjoint_5Mx5M", "n": n, "forward_pairs_per_step": fwd_pairs,
-                      "passes_per_step": "per correlation: the autograd forward, then a lens-side and a "
-                                         "source-side marks pass (the adjoint's forward sums come from the "
-                                         "lens-side marks)"}}
+                      "passes_per_step": "per correlation: the autograd forward records the lens-side marks "
+                                         "pass (its marks give the forward sums), the backward replays it with "
+                                         "the source-side marks pass"}}
     print(json.dumps(res), flush=True)
     ctx.close()
     return 0

@@ -236,6 +236,55 @@ int tpcf_adjoint_kappa_f64(tpcf_ctx *ctx,
                            double *val_A_bar, double *w_A_bar,
                            double *kE_B_bar, double *kB_B_bar, double *w_B_bar);
 
+/* Recorded adjoint (no counterpart in the reference): the adjoint above runs
+ * two pair passes, a lens-side pass whose per-lens sums also give the forward
+ * outputs, and a source-side pass.  tpcf_adjoint_record_* runs the lens-side
+ * pass alone, returns the forward outputs (g_t, g_x or xi_E, xi_B per bin,
+ * host arrays [n_bin], nullable; 0 where K or W is 0) and keeps that pass's
+ * per-lens sums (4 n_bin doubles per used lens, device memory) in *tape.
+ * tpcf_adjoint_replay then gives the gradient for any upstream gradient with
+ * the source-side pass only, as often as wanted; an autodiff forward records,
+ * its backward replays, so forward + backward is two passes instead of three.
+ * Inputs must be device pointers (TPCF_DEVICE_INPUTS; the tape keeps the
+ * pointers, which must stay valid and unchanged until the last replay);
+ * gradient outputs of the replay are device pointers, any may be NULL
+ * (s1/s2 = e1/e2 for shear, kappa_E/kappa_B for kappa; m_B_bar is ignored for
+ * kappa).  A tape belongs to the device it was recorded on and is freed by
+ * tpcf_tape_destroy. */
+typedef struct tpcf_tape tpcf_tape;
+int tpcf_adjoint_record_shear_f32(tpcf_ctx *ctx,
+                                  int64_t n_A, const float *x_A, const float *y_A, const float *z_A,
+                                  const float *val_A, const float *w_A,
+                                  int64_t n_B, const float *x_B, const float *y_B, const float *z_B,
+                                  const float *e1_B, const float *e2_B, const float *w_B, const float *m_B,
+                                  const tpcf_bins *bins, uint32_t flags, double *g_t, double *g_x,
+                                  tpcf_tape **tape);
+int tpcf_adjoint_record_shear_f64(tpcf_ctx *ctx,
+                                  int64_t n_A, const double *x_A, const double *y_A, const double *z_A,
+                                  const double *val_A, const double *w_A,
+                                  int64_t n_B, const double *x_B, const double *y_B, const double *z_B,
+                                  const double *e1_B, const double *e2_B, const double *w_B, const double *m_B,
+                                  const tpcf_bins *bins, uint32_t flags, double *g_t, double *g_x,
+                                  tpcf_tape **tape);
+int tpcf_adjoint_record_kappa_f32(tpcf_ctx *ctx,
+                                  int64_t n_A, const float *x_A, const float *y_A, const float *val_A,
+                                  const float *w_A,
+                                  int64_t n_B, const float *x_B, const float *y_B, const float *kappa_E,
+                                  const float *kappa_B, const float *w_B,
+                                  const tpcf_bins *bins, uint32_t flags, double *xi_E, double *xi_B,
+                                  tpcf_tape **tape);
+int tpcf_adjoint_record_kappa_f64(tpcf_ctx *ctx,
+                                  int64_t n_A, const double *x_A, const double *y_A, const double *val_A,
+                                  const double *w_A,
+                                  int64_t n_B, const double *x_B, const double *y_B, const double *kappa_E,
+                                  const double *kappa_B, const double *w_B,
+                                  const tpcf_bins *bins, uint32_t flags, double *xi_E, double *xi_B,
+                                  tpcf_tape **tape);
+int tpcf_adjoint_replay(tpcf_ctx *ctx, const tpcf_tape *tape, const double *gbar_1, const double *gbar_2,
+                        double *val_A_bar, double *w_A_bar,
+                        double *s1_B_bar, double *s2_B_bar, double *w_B_bar, double *m_B_bar);
+int tpcf_tape_destroy(tpcf_tape *tape);
+
 /* ---- many catalogues (mock fields) as one HIP graph ----
  * No counterpart in the reference beyond its field loop (src/main.f90:596-660,
  * one shear_2pcf / kappa_2pcf call per field, results stacked as in

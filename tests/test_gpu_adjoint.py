@@ -7,7 +7,9 @@ Checked against
   * central finite differences of the library's own FP64 forward with steps
     of 1e-4;
 on catalogues with z masks (the reversed z test of the source-side pass),
-zero lens weights, m != 0, and a kappa auto-correlation through autograd.
+zero lens weights, m != 0, and a kappa auto-correlation through autograd
+(the autograd forward records the lens-side pass, its backward replays it);
+the recorded form against the one-call adjoint.
 Tolerance: FP32 path within 2e-5 of the largest |gradient| of each array
 (per-pair terms are FP32); FP64 path within 1e-10; finite differences within
 1e-6 relative (+1e-9 absolute)."""
@@ -210,3 +212,54 @@ def test_autograd_kappa_auto_correlation(tp, orc, ctx):
     for have, want in ((K.grad, Kt.grad), (W.grad, Wt.grad)):
         h, wnt = have.detach().cpu().double().numpy(), want.numpy()
         assert np.abs(h - wnt).max() <= 2e-5 * np.abs(wnt).max()
+
+
+@pytest.mark.parametrize("kind", [SHEAR, KAPPA])
+def test_record_replay_matches_adjoint(tp, ctx, kind):
+    """tpcf_adjoint_record_* + tpcf_adjoint_replay: the recorded forward
+    equals the plain forward (FP32 path: within 1e-5 of the largest |g|), and
+    each replay equals tpcf_adjoint_* for the same upstream gradient (the same
+    passes: within 1e-9 of each array's largest |gradient|), also after other
+    calls on the same context have reused its buffers, and twice in a row."""
+    dev = torch.device("cuda:0")
+    lens, src = catalogue(330 + kind, 500, 600, 40.0, kind)
+    bins = tp.create_bins(0.5, 12.0, 9, tp.LOG)
+    dl = {k: torch.tensor(v, device=dev) for k, v in lens.items()}
+    ds = {k: torch.tensor(v, device=dev) for k, v in src.items()}
+    torch.cuda.synchronize()
+    g1, g2, tape = tp.adjoint_record(kind, {k: v.data_ptr() for k, v in dl.items()},
+                                     {k: v.data_ptr() for k, v in ds.items()}, bins, precision="f32", ctx=ctx,
+                                     n_A=len(lens["x"]), n_B=len(src["x"]))
+    fwd = tp.correlate(kind, lens, src, bins, real_bytes=8, ctx=ctx)
+    for have, want in ((g1, fwd.g_t), (g2, fwd.g_x)):
+        assert np.abs(have - want).max() <= 1e-5 * np.abs(want).max() + 1e-12
+    r = np.random.default_rng(21)
+    lk = ("val", "w")
+    sk = ("e1", "e2", "w", "m") if kind == SHEAR else ("kappa_E", "kappa_B", "w")
+    for trial in range(3):
+        gbar1, gbar2 = r.normal(size=bins.n_bin), r.normal(size=bins.n_bin)
+        bufs = {"lens": {k: torch.full((len(lens["x"]),), 7.0, dtype=torch.float64, device=dev) for k in lk},
+                "src": {k: torch.full((len(src["x"]),), 7.0, dtype=torch.float64, device=dev) for k in sk}}
+        torch.cuda.synchronize()
+        tp.adjoint_replay(tape, gbar1, gbar2, {s: {k: b.data_ptr() for k, b in d.items()} for s, d in bufs.items()})
+        ref = tp.adjoint(kind, lens, src, bins, gbar1, gbar2, precision="f32", ctx=ctx)
+        for side in ("lens", "src"):
+            for k, want in ref[side].items():
+                have = bufs[side][k].cpu().numpy()
+                assert np.abs(have - want).max() <= 1e-9 * np.abs(want).max() + 1e-300, (trial, side, k)
+    tape.close()
+
+
+def test_record_needs_device_inputs(tp, ctx):
+    import ctypes as C
+    N = importlib.import_module("2pcf_amd._native")
+    lens, src = catalogue(340, 50, 60, 10.0, KAPPA)
+    bins = tp.create_bins(0.5, 5.0, 4, tp.LOG)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+    h = C.c_void_p()
+    rc = N.lib().tpcf_adjoint_record_kappa_f32(ctx.handle, 50, p(lens["x"]), p(lens["y"]), p(lens["val"]),
+                                               p(lens["w"]), 60, p(src["x"]), p(src["y"]), p(src["kappa_E"]),
+                                               p(src["kappa_B"]), p(src["w"]), C.byref(bins.c), 0, None, None,
+                                               C.byref(h))
+    assert rc == N.ERR_ARG and not h.value
+    assert b"TPCF_DEVICE_INPUTS" in N.lib().tpcf_last_error(ctx.handle)
