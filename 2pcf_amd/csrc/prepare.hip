@@ -42,6 +42,29 @@ void launch_fill(hipStream_t s, U *p, U v, int64_t n)
     hipLaunchKernelGGL(k_fill<U>, dim3((unsigned)blocks), dim3(kBlk), 0, s, p, v, n);
 }
 
+// counter[key] += 1 for every active lane with add set.  When all of them
+// share one key (a single patch, or neighbouring objects) the wave adds its
+// total with one atomic instead of up to 64 on one address.
+__device__ __forceinline__ void wave_count(uint32_t *counter, uint32_t key, bool add)
+{
+    const uint64_t act = __ballot(add);
+    if (!act) return;
+    const int lead = __ffsll((unsigned long long)act) - 1;
+    const uint32_t k0 = __shfl(key, lead);
+    if (__ballot(add && key != k0) == 0) {
+        if ((int)__lane_id() == lead) atomicAdd(&counter[k0], (uint32_t)__popcll(act));
+    } else if (add) {
+        atomicAdd(&counter[key], 1u);
+    }
+}
+
+// *flags |= bit if any active lane has set: one atomic per wave.
+__device__ __forceinline__ void wave_flag(int32_t *flags, int32_t bit, bool set)
+{
+    const uint64_t m = __ballot(set);
+    if (m && (int)__lane_id() == __ffsll((unsigned long long)m) - 1) atomicOr(flags, bit);
+}
+
 // Per-block min/max of x and y and count of non-finite coordinates.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_bbox(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
@@ -87,10 +110,9 @@ __global__ __launch_bounds__(kBlk) void k_src_keys(int64_t n, const T *__restric
     const uint32_t drop_key = (uint32_t)(g.ncells * n_pb);
     uint32_t key;
     int32_t pb = patch ? patch[i] : 0;
-    if (pb < 0 || pb >= n_pb) {
-        atomicOr(err, 1);
-        pb = 0;
-    }
+    const bool bad = pb < 0 || pb >= n_pb;
+    wave_flag(err, 1, bad);
+    if (bad) pb = 0;
     if (z && z[i] < (T)0) {
         key = drop_key;
     } else {
@@ -100,7 +122,7 @@ __global__ __launch_bounds__(kBlk) void k_src_keys(int64_t n, const T *__restric
     }
     keys[i] = key;
     idx[i] = (uint32_t)i;
-    atomicAdd(&counts[key], 1u);
+    wave_count(counts, key, true);
 }
 
 // Number of segments of each key: runs longer than kSegMax are split.
@@ -121,7 +143,7 @@ __global__ __launch_bounds__(kBlk) void k_src_gather(int64_t n, int kind, const 
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = order[i];
-    if (kind == SHEAR && m && m[j] != (T)0) atomicOr(flags, 4);
+    wave_flag(flags, 4, kind == SHEAR && m && m[j] != (T)0);
     SrcRec<T> r;
     r.x = x[j];
     r.y = y[j];
@@ -142,10 +164,10 @@ __global__ __launch_bounds__(kBlk) void k_segs(int64_t key_space, Grid g, const 
                                                Seg<T> *__restrict__ segs, int4 *__restrict__ pbox)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= key_space) return;
-    const uint32_t total = counts[k];
-    if (total == 0) return;
-    const uint32_t pb = (uint32_t)(k / g.ncells);
+    // every lane stays to the end (the wave reduction below reads them all)
+    const uint32_t total = k < key_space ? counts[k] : 0u;
+    const bool live = total > 0;
+    const uint32_t pb = live ? (uint32_t)(k / g.ncells) : 0u;
     for (uint32_t part = 0; part * kSegMax < total; ++part) {
         Seg<T> s;
         s.start = cstart[k] + part * kSegMax;
@@ -167,13 +189,37 @@ __global__ __launch_bounds__(kBlk) void k_segs(int64_t key_space, Grid g, const 
         s.W = W; s.K = K; s.S1 = S1; s.S2 = S2;
         segs[seg_first[k] + part] = s;
     }
-    const int64_t cell = k % g.ncells;
+    const int64_t cell = live ? k % g.ncells : 0;
     const int32_t cx = (int32_t)(cell % g.nx), cy = (int32_t)(cell / g.nx);
-    int4 *b = &pbox[pb];
-    atomicMin(&b->x, cx);
-    atomicMax(&b->y, cx);
-    atomicMin(&b->z, cy);
-    atomicMax(&b->w, cy);
+    // the patch's cell box: one set of atomics per wave when its live keys
+    // share a patch (always with one source patch), else one per key
+    const uint64_t act = __ballot(live);
+    if (!act) return;
+    const int lead = __ffsll((unsigned long long)act) - 1;
+    const uint32_t pb0 = __shfl(pb, lead);
+    if (__ballot(live && pb != pb0) == 0) {
+        int32_t x0 = live ? cx : INT32_MAX, x1 = live ? cx : INT32_MIN;
+        int32_t y0 = live ? cy : INT32_MAX, y1 = live ? cy : INT32_MIN;
+        for (int off = 32; off > 0; off >>= 1) {
+            x0 = min(x0, __shfl_xor(x0, off));
+            x1 = max(x1, __shfl_xor(x1, off));
+            y0 = min(y0, __shfl_xor(y0, off));
+            y1 = max(y1, __shfl_xor(y1, off));
+        }
+        if ((int)__lane_id() == lead) {
+            int4 *b = &pbox[pb0];
+            atomicMin(&b->x, x0);
+            atomicMax(&b->y, x1);
+            atomicMin(&b->z, y0);
+            atomicMax(&b->w, y1);
+        }
+    } else if (live) {
+        int4 *b = &pbox[pb];
+        atomicMin(&b->x, cx);
+        atomicMax(&b->y, cx);
+        atomicMin(&b->z, cy);
+        atomicMax(&b->w, cy);
+    }
 }
 
 // Lens sort key: (pa << 32) | Hilbert index of the 16-bit quantised position,
@@ -189,10 +235,9 @@ __global__ __launch_bounds__(kBlk) void k_lens_keys(int64_t n, const T *__restri
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     int32_t pa = patch ? patch[i] : 0;
-    if (pa < 0 || pa >= n_pa) {
-        atomicOr(err, 2);
-        pa = 0;
-    }
+    const bool bad = pa < 0 || pa >= n_pa;
+    wave_flag(err, 2, bad);
+    if (bad) pa = 0;
     const bool drop = (w[i] == (T)0) || (z && z[i] < (T)0);
     uint64_t key;
     if (drop) {
@@ -202,8 +247,8 @@ __global__ __launch_bounds__(kBlk) void k_lens_keys(int64_t n, const T *__restri
         uint32_t qx = (uint32_t)fmin(fmax(fx, 0.0), 65535.0);
         uint32_t qy = (uint32_t)fmin(fmax(fy, 0.0), 65535.0);
         key = ((uint64_t)pa << 32) | (uint64_t)hilbert16(qx, qy);
-        atomicAdd(&pcount[pa], 1u);
     }
+    wave_count(pcount, (uint32_t)pa, !drop);
     keys[i] = key;
     idx[i] = (uint32_t)i;
 }
