@@ -140,6 +140,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python 2pcf_amd/build.py)")
+    # PyTorch-ROCm ships its own HIP runtime (libamdhip64.so) and makes its
+    # symbols global.  Loaded first, it is also the runtime this library binds
+    # to; loaded after this library (which needs the system's
+    # libamdhip64.so.7) it would be a second HIP runtime in the process, and
+    # torch's device initialisation then fails ("No HIP GPUs are available").
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.tpcf_create_bins.argtypes = [C.c_float, C.c_float, _i32, _i32, C.c_float, C.POINTER(Bins)]
     L.tpcf_create_bins.restype = C.c_int

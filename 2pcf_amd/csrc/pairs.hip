@@ -343,94 +343,162 @@ __device__ __forceinline__ void win_open(Win &w, const LaneData<T> &L, int n_bin
     w.open = true;
 }
 
+// The edge decision of the edge passes: FP32 d2 < lo, re-decided in FP64 for
+// the lanes inside the band; the branch is taken by the wave only when some
+// lane needs it.
+template <typename T>
+__device__ __forceinline__ bool edge_below(const EdgeTest<T> &et, T d2, const LaneData<T> &L, const SrcRec<T> &q)
+{
+    if constexpr (sizeof(T) == 4) {
+        bool below = d2 < et.lo;
+        const bool amb = !below && d2 <= et.hi;
+        if (__ballot(amb)) {
+            // keeps this rare path a branch (not evaluated for every pair)
+            asm volatile("" ::: "memory");
+            if (amb) {
+                // opaque copies: the FP64 lens coordinates are formed here,
+                // not hoisted out of the loop into four long-lived registers
+                float xa = L.xa, ya = L.ya;
+                asm volatile("" : "+v"(xa), "+v"(ya));
+                const double ddx = (double)xa - (double)q.x, ddy = (double)ya - (double)q.y;
+                below = fma(ddx, ddx, ddy * ddy) < et.ed;
+            }
+        }
+        return below;
+    } else {
+        return d2 < et.ed;
+    }
+}
+
 // Edge pass: the segment straddles one edge (between window bins wk and
-// wk+1).  Per pair only the share of bin wk+1 is accumulated next to the
-// segment totals; bin wk gets total - share (W, K and kappa's G from the exact
-// segment sums, pair counts from wave ballots).  GUARD: the chunk and segment
+// wk+1).  Every sum is taken per side of the edge, directly: a side derived
+// as total - other side would carry the other side's rounding, which is
+// large next to a side that holds few of the segment's pairs.  The two sides
+// are two exec-masked branches (empty asm statements keep them from being
+// turned into selects), so each pair issues one side's instructions.  Pair
+// counts come from wave ballots.  GUARD: the chunk and segment
 // boxes overlap, so a pair may sit at zero separation (its per-pair values are
 // forced finite); TRACK0: shear with x_min == 0, where such a pair is in bin 0
 // (reference: NaN).  MZ: all m == 0, so K == W.
 template <typename T, int KIND, bool MZ, bool GUARD, bool TRACK0, bool SPH>
 __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
-                                          const SrcRec<T> *__restrict__ r, const Seg<T> &sg, uint32_t count,
-                                          int n_bin)
+                                          const SrcRec<T> *__restrict__ r, uint32_t count, int n_bin)
 {
     constexpr int WQ = win_q<MZ>();
     const EdgeTest<T> et(A, w.wk + 1);
-    T sT = 0, sG1 = 0, sG2 = 0, hT = 0, hG1 = 0, hG2 = 0, hW = 0, hK = 0, sZ = 0;
+    T lT = 0, hT = 0, lW = 0, hW = 0, lK = 0, hK = 0, lG1 = 0, lG2 = 0, hG1 = 0, hG2 = 0, sZ = 0;
     uint32_t n_hi = 0;   // pairs at or above the edge (wave total, scalar)
     stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
         const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
         const T d2 = g.d2;
-        bool below;
-        if constexpr (sizeof(T) == 4) {
-            below = d2 < et.lo;
-            // pairs within the band are re-decided in FP64; the branch is
-            // taken by the whole wave only when some lane needs it
-            const bool amb = !below && d2 <= et.hi;
-            if (__ballot(amb)) {
-                // keeps this rare path a branch (not evaluated for every pair)
-                asm volatile("" ::: "memory");
-                if (amb) {
-                    // opaque copies: the FP64 lens coordinates are formed here,
-                    // not hoisted out of the loop into four long-lived registers
-                    float xa = L.xa, ya = L.ya;
-                    asm volatile("" : "+v"(xa), "+v"(ya));
-                    const double ddx = (double)xa - (double)q.x, ddy = (double)ya - (double)q.y;
-                    below = fma(ddx, ddx, ddy * ddy) < et.ed;
-                }
-            }
-        } else {
-            below = d2 < et.ed;
-        }
+        const bool below = edge_below<T>(et, d2, L, q);
         // the count of the upper bin is a wave total from the lane mask
         n_hi += __popcll(__builtin_amdgcn_ballot_w64(!below) & w.valid);
         T d, inv, u, v;
         pair_proj<T, SPH, GUARD>(L, g, d, inv, u, v);
-        // selects rather than products with a 0/1 mask
-        const T wm = below ? (T)0 : q.w;
-        sT = fma(d, q.w, sT);
-        hT = fma(d, wm, hT);
-        hW += wm;
-        if (!MZ) hK += below ? (T)0 : q.wk;
+        // the shear terms before the branches (both sides use them)
+        T t1 = 0, t2 = 0;
         if (KIND == SHEAR) {
-            if (TRACK0) sZ += (d2 == (T)0 && below) ? (T)1 : (T)0;
-            const T t1 = fma(q.s1w, u, q.s2w2 * v);
-            const T t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v));
-            const T invm = below ? (T)0 : inv;
-            sG1 = fma(t1, inv, sG1);
-            sG2 = fma(t2, inv, sG2);
-            hG1 = fma(t1, invm, hG1);
-            hG2 = fma(t2, invm, hG2);
+            t1 = fma(q.s1w, u, q.s2w2 * v) * inv;
+            t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)) * inv;
+        }
+        if (below) {
+            lT = fma(d, q.w, lT);
+            lW += q.w;
+            if (!MZ) lK += q.wk;
+            if (KIND == SHEAR) {
+                lG1 += t1;
+                lG2 += t2;
+                if (TRACK0) sZ += (d2 == (T)0) ? (T)1 : (T)0;
+            } else {
+                lG1 += q.s1w;
+                lG2 += q.s2w;
+            }
+            asm volatile("");
         } else {
-            hG1 += below ? (T)0 : q.s1w;
-            hG2 += below ? (T)0 : q.s2w;
+            hT = fma(d, q.w, hT);
+            hW += q.w;
+            if (!MZ) hK += q.wk;
+            if (KIND == SHEAR) {
+                hG1 += t1;
+                hG2 += t2;
+            } else {
+                hG1 += q.s1w;
+                hG2 += q.s2w;
+            }
+            asm volatile("");
         }
     });
     const int lane = L.lane;
-    const double dhT = (double)hT, dhG1 = (double)hG1, dhG2 = (double)hG2, dhW = (double)hW, dhK = (double)hK;
-    wacc(w, QT, lane) += (double)sT - dhT;
-    wacc(w, WQ + QT, lane) += dhT;
-    if (KIND == SHEAR) {
-        wacc(w, QG1, lane) += (double)sG1 - dhG1;
-        wacc(w, QG2, lane) += (double)sG2 - dhG2;
-    } else {
-        wacc(w, QG1, lane) += sg.S1 - dhG1;
-        wacc(w, QG2, lane) += sg.S2 - dhG2;
-    }
-    wacc(w, WQ + QG1, lane) += dhG1;
-    wacc(w, WQ + QG2, lane) += dhG2;
-    wacc(w, QW, lane) += sg.W - dhW;
-    wacc(w, WQ + QW, lane) += dhW;
+    wacc(w, QT, lane) += (double)lT;
+    wacc(w, WQ + QT, lane) += (double)hT;
+    wacc(w, QG1, lane) += (double)lG1;
+    wacc(w, QG2, lane) += (double)lG2;
+    wacc(w, WQ + QG1, lane) += (double)hG1;
+    wacc(w, WQ + QG2, lane) += (double)hG2;
+    wacc(w, QW, lane) += (double)lW;
+    wacc(w, WQ + QW, lane) += (double)hW;
     if (!MZ) {
-        wacc(w, QK, lane) += sg.K - dhK;
-        wacc(w, WQ + QK, lane) += dhK;
+        wacc(w, QK, lane) += (double)lK;
+        wacc(w, WQ + QK, lane) += (double)hK;
     }
     if (KIND == SHEAR && TRACK0) wacc(w, 2 * WQ, lane) += (double)sZ;
     const double nhi = (double)n_hi;
     tab_count(w, lane, w.wk, n_bin, (double)count * (double)w.n_valid - nhi);
     tab_count(w, lane, w.wk + 1, n_bin, nhi);
     w.used0 = w.used1 = true;
+    w.edge = true;
+}
+
+// Edge pass at a boundary of the binned range: the segment straddles x_min
+// (UPPER: of the window bins only wk+1 = 0 is in range) or x_max (!UPPER:
+// only wk = n_bin - 1).  Only the in-range side is summed, directly; the
+// values of out-of-range pairs, zero separations among them, are selected
+// away.  Not used where a zero separation is in range (x_min == 0).  The
+// kernel uses it at x_max, where the edge pass would give bin n_bin - 1 as
+// the segment's G total minus the pairs beyond x_max, whose rounding can be
+// large next to a bin that holds few of the segment's pairs.
+template <typename T, int KIND, bool MZ, bool UPPER, bool SPH>
+__device__ __forceinline__ void edge_side_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
+                                               const SrcRec<T> *__restrict__ r, uint32_t count, int n_bin)
+{
+    constexpr int WQ = win_q<MZ>();
+    const EdgeTest<T> et(A, w.wk + 1);
+    T sT = 0, sG1 = 0, sG2 = 0, sW = 0, sK = 0;
+    uint32_t n_in = 0;   // in-range pairs (wave total, scalar)
+    stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
+        const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+        const bool below = edge_below<T>(et, g.d2, L, q);
+        const bool in = UPPER ? !below : below;
+        n_in += __popcll(__builtin_amdgcn_ballot_w64(in) & w.valid);
+        T d, inv, u, v;
+        pair_proj<T, SPH, false>(L, g, d, inv, u, v);
+        // an exec-masked branch (kept one by the empty asm): out-of-range
+        // lanes, with their non-finite values at zero separation, add nothing
+        if (in) {
+            sT = fma(d, q.w, sT);
+            sW += q.w;
+            if (!MZ) sK += q.wk;
+            if (KIND == SHEAR) {
+                sG1 = fma(fma(q.s1w, u, q.s2w2 * v), inv, sG1);
+                sG2 = fma(fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)), inv, sG2);
+            } else {
+                sG1 += q.s1w;
+                sG2 += q.s2w;
+            }
+            asm volatile("");
+        }
+    });
+    const int lane = L.lane;
+    constexpr int o = UPPER ? WQ : 0;
+    wacc(w, o + QT, lane) += (double)sT;
+    wacc(w, o + QG1, lane) += (double)sG1;
+    wacc(w, o + QG2, lane) += (double)sG2;
+    wacc(w, o + QW, lane) += (double)sW;
+    if (!MZ) wacc(w, o + QK, lane) += (double)sK;
+    tab_count(w, lane, UPPER ? w.wk + 1 : w.wk, n_bin, (double)n_in);
+    if (UPPER) w.used1 = true;
+    else w.used0 = true;
     w.edge = true;
 }
 
@@ -463,6 +531,8 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
         // a segment spanning many bins is walked once per window: a source
         // none of whose pairs fall in this window adds nothing (whole wave)
         if (!__builtin_amdgcn_ballot_w64(in0 || in1)) continue;
+        // 0/1 masks rather than branches: the window's two bins are usually
+        // both present in a wave, and masked FMAs measured faster here
         const T m = in0 ? (T)1 : (T)0, m2 = in1 ? (T)1 : (T)0;
         n0 += __popcll(__builtin_amdgcn_ballot_w64(m != (T)0) & w.valid);
         n1 += __popcll(__builtin_amdgcn_ballot_w64(m2 != (T)0) & w.valid);
@@ -665,12 +735,17 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                     cls = 1;
                     // ---- edge: one bin edge inside the distance range ----
                     win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, kmin);
-                    if (lo2 > (T)0) {
-                        edge_pass<T, KIND, MZ, false, false, SPH>(w, L, A, r, sg, cnt, n_bin);
+                    if (kmax >= n_bin && kmin >= 0 && !(A.xmin_zero && lo2 == (T)0)) {
+                        // x_max inside the range: only bin n_bin - 1 counts (the
+                        // edge pass would derive it as total - the pairs beyond
+                        // x_max; at x_min it sums bin 0 directly already)
+                        edge_side_pass<T, KIND, MZ, false, SPH>(w, L, A, r, cnt, n_bin);
+                    } else if (lo2 > (T)0) {
+                        edge_pass<T, KIND, MZ, false, false, SPH>(w, L, A, r, cnt, n_bin);
                     } else if (KIND == SHEAR && A.xmin_zero && kmin == 0) {
-                        edge_pass<T, KIND, MZ, true, true, SPH>(w, L, A, r, sg, cnt, n_bin);
+                        edge_pass<T, KIND, MZ, true, true, SPH>(w, L, A, r, cnt, n_bin);
                     } else {
-                        edge_pass<T, KIND, MZ, true, false, SPH>(w, L, A, r, sg, cnt, n_bin);
+                        edge_pass<T, KIND, MZ, true, false, SPH>(w, L, A, r, cnt, n_bin);
                     }
                 } else {
                     cls = 2;

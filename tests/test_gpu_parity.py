@@ -75,12 +75,22 @@ def check(tp, orc, kind, lens, src, bins, *, precision="f32", marks=False, patch
             assert np.max(np.abs(getattr(g, gk) - o[ok]) / np.maximum(o[ak], 1e-300)) <= tol, gk
         del scale
     if marks:
-        # marks come back as (n_bin, n_A) views; compare against the oracle's (n_A, n_bin)
+        # marks come back as (n_bin, n_A) views; compare against the oracle's
+        # (n_A, n_bin).  W and K marks: relative to the lens' row; the signed
+        # xi marks: relative to a bound of their sum of |terms|, the oracle's
+        # kappa marks of |val| against |s1| + |s2| (|e1 c + e2 s| <= |e1| + |e2|)
+        s1, s2 = ("e1", "e2") if kind == SHEAR else ("kappa_E", "kappa_B")
+        o_abs = orc.correlate(KAPPA, dict(x=lens["x"], y=lens["y"], val=np.abs(lens["val"]), w=lens["w"]),
+                              dict(x=src["x"], y=src["y"], kappa_E=np.abs(src[s1]) + np.abs(src[s2]),
+                                   kappa_B=np.zeros(len(src["x"])), w=src["w"]), bins.bin_edges, marks=True)
         for gk, ok in (("xi_E_marks", "xi_e"), ("xi_B_marks", "xi_b"), ("K_marks", "k_marks"),
                        ("w_marks", "w_marks")):
-            gm = np.asarray(getattr(g, gk)).T
+            gm = np.asarray(getattr(g, gk), np.float64).T
             om = o[ok]
-            scale = np.abs(om).sum(axis=1, keepdims=True) + 1e-300
+            if ok in ("xi_e", "xi_b"):
+                scale = np.abs(o_abs["xi_e"]) + 1e-300
+            else:
+                scale = np.abs(om).sum(axis=1, keepdims=True) + 1e-300
             err = np.max(np.abs(gm - om) / scale)
             assert err <= 10 * tol, (gk, err)
         np.testing.assert_allclose(g.marks_weight, o["marks_weight"], rtol=10 * tol, atol=1e-300)
@@ -172,6 +182,21 @@ def test_shear_marks_m_zero_cancelling_weights(tp, orc, ctx):
         assert g.w_marks[0, i] == 0 and g.K_marks[0, i] == 0
         np.testing.assert_allclose(g.xi_E_marks[0, i], o["xi_e"][i, 0], rtol=1e-6)
         np.testing.assert_allclose(g.xi_B_marks[0, i], o["xi_b"][i, 0], rtol=1e-6)
+
+
+@pytest.mark.parametrize("kind", [KAPPA, SHEAR])
+@pytest.mark.parametrize("n_bin,spacing,x_min", [(63, 1, 0.3), (1, 1, 2.0), (63, 0, 0.0)])
+def test_bin_count_extremes(tp, orc, ctx, kind, n_bin, spacing, x_min):
+    """The largest bin count (TPCF_MAX_BIN = 63: every lane of the per-bin
+    table but one, the two-bin window at the last bin) and a single bin, with
+    marks and patches."""
+    lens, src = uniform_case(91 + n_bin + spacing + kind, 2500, 3000, 60.0, kind, z=kind == SHEAR, zero_w=True)
+    r = _rng(92)
+    pa = r.integers(0, 3, len(lens["x"])).astype(np.int32)
+    pb = r.integers(0, 2, len(src["x"])).astype(np.int32)
+    bins = tp.create_bins(x_min, 25.0, n_bin, spacing)
+    g, o = check(tp, orc, kind, lens, src, bins, marks=True, patch_A=pa, patch_B=pb, n_pa=3, n_pb=2, ctx=ctx)
+    assert np.count_nonzero(g.n_pairs) == n_bin
 
 
 @pytest.mark.parametrize("kind", [KAPPA, SHEAR])
