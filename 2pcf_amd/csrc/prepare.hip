@@ -232,25 +232,44 @@ __global__ __launch_bounds__(kBlk) void k_lens_keys(int64_t n, const T *__restri
                                                     uint32_t *__restrict__ idx, uint32_t *__restrict__ pcount,
                                                     int32_t *__restrict__ err)
 {
+    // per-patch counts: a block histogram in LDS when the patches fit (labels
+    // in input order are scattered, so per-lane global atomics would meet on
+    // n_pa addresses), else wave-aggregated global atomics
+    constexpr int kHist = 1024;
+    __shared__ uint32_t hist[kHist];
+    const bool use_lds = n_pa <= kHist;
+    if (use_lds) {
+        for (int b = threadIdx.x; b < n_pa; b += blockDim.x) hist[b] = 0u;
+        __syncthreads();
+    }
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t pa = patch ? patch[i] : 0;
-    const bool bad = pa < 0 || pa >= n_pa;
+    const bool live = i < n;
+    int32_t pa = (live && patch) ? patch[i] : 0;
+    const bool bad = live && (pa < 0 || pa >= n_pa);
     wave_flag(err, 2, bad);
     if (bad) pa = 0;
-    const bool drop = (w[i] == (T)0) || (z && z[i] < (T)0);
-    uint64_t key;
-    if (drop) {
-        key = (uint64_t)n_pa << 32;
-    } else {
-        double fx = ((double)x[i] - lx0) * qs, fy = ((double)y[i] - ly0) * qs;
-        uint32_t qx = (uint32_t)fmin(fmax(fx, 0.0), 65535.0);
-        uint32_t qy = (uint32_t)fmin(fmax(fy, 0.0), 65535.0);
-        key = ((uint64_t)pa << 32) | (uint64_t)hilbert16(qx, qy);
+    const bool drop = !live || (w[i] == (T)0) || (z && z[i] < (T)0);
+    if (live) {
+        uint64_t key;
+        if (drop) {
+            key = (uint64_t)n_pa << 32;
+        } else {
+            double fx = ((double)x[i] - lx0) * qs, fy = ((double)y[i] - ly0) * qs;
+            uint32_t qx = (uint32_t)fmin(fmax(fx, 0.0), 65535.0);
+            uint32_t qy = (uint32_t)fmin(fmax(fy, 0.0), 65535.0);
+            key = ((uint64_t)pa << 32) | (uint64_t)hilbert16(qx, qy);
+        }
+        keys[i] = key;
+        idx[i] = (uint32_t)i;
     }
-    wave_count(pcount, (uint32_t)pa, !drop);
-    keys[i] = key;
-    idx[i] = (uint32_t)i;
+    if (use_lds) {
+        if (!drop) atomicAdd(&hist[pa], 1u);
+        __syncthreads();
+        for (int b = threadIdx.x; b < n_pa; b += blockDim.x)
+            if (hist[b]) atomicAdd(&pcount[b], hist[b]);
+    } else {
+        wave_count(pcount, (uint32_t)pa, !drop);
+    }
 }
 
 template <typename T>
