@@ -132,7 +132,8 @@ __device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_
 // q = Theta, G1, G2, W, K of bin wk (q) and bin wk+1 (WQ + q), plus the
 // count of in-bin zero-separation pairs (shear with x_min == 0).  Pair counts
 // are wave totals (ballot + popcount) and go straight to the per-bin sums,
-// q = Theta, G1, G2, W, K, N, kept in registers by the lane of the bin.  Only
+// q = Theta, G1, G2, W, K, N, kept in registers by the lane of the bin (K
+// only when some m != 0: otherwise it equals W and is not kept).  Only
 // the window sums use LDS (5.6 KB per wave), which leaves room for 7
 // workgroups of 4 waves per CU.
 constexpr int kQ = 6;
@@ -310,7 +311,7 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
             w.t[QG1] += sG1;
             w.t[QG2] += sG2;
             w.t[QW] += sW;
-            w.t[QK] += sK;
+            if (!MZ) w.t[QK] += sK;   // MZ: K == W, not kept
         }
         // A lens' chunk may be split over several waves: add atomically.  The
         // atomics are what a marks pass costs, so a lane whose four terms are
@@ -724,7 +725,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                         w.t[QN] += (double)cnt * (double)ch.count;
                         if (!MARKS) {
                             w.t[QW] += ch.wsum * sg.W;
-                            w.t[QK] += ch.wsum * sg.K;
+                            if (!MZ) w.t[QK] += ch.wsum * sg.K;
                             if (KIND == KAPPA) {
                                 w.t[QG1] += ch.asum * sg.S1;
                                 w.t[QG2] += ch.asum * sg.S2;
@@ -774,7 +775,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
             unsafeAtomicAdd(&C.pp[o * kNQ + 1], w.t[QG1]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 2], w.t[QG2]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 3], w.t[QW]);
-            unsafeAtomicAdd(&C.pp[o * kNQ + 4], w.t[QK]);
+            unsafeAtomicAdd(&C.pp[o * kNQ + 4], MZ ? w.t[QW] : w.t[QK]);
             atomicAdd(&C.ppn[o], (unsigned long long)w.t[QN]);
         }
         for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
