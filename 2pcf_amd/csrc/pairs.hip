@@ -202,14 +202,40 @@ __device__ __forceinline__ PairGeo<T, SPH> pair_geo(const LaneData<T> &L, const 
     return g;
 }
 
-// Separation angle from the squared chord: 2 asin(c / 2).  Below c = 0.07
-// (4 degrees) the series of asin(x)/x to x^12 is exact to rounding.
+// sqrt(d2) for d2 > 0 from the hardware reciprocal square root and two
+// Newton-Raphson steps r <- r (3/2 - d2 r^2 / 2), each of which squares the
+// seed's relative error (within 2 ulp of the rounded square root).
+__device__ __forceinline__ double sqrt_pos(double d2)
+{
+    double r = __builtin_amdgcn_rsq(d2);
+    const double h = 0.5 * d2;
+    r = r * fma(-h * r, r, 1.5);
+    r = r * fma(-h * r, r, 1.5);
+    return d2 * r;
+}
+
+// 1/s for s > 0 from the hardware reciprocal and two Newton-Raphson steps
+// r <- r + r (1 - s r).
+__device__ __forceinline__ double rcp_pos(double s)
+{
+    double r = __builtin_amdgcn_rcp(s);
+    r = fma(r, fma(-s, r, 1.0), r);
+    r = fma(r, fma(-s, r, 1.0), r);
+    return r;
+}
+
+// Separation angle from the squared chord: 2 asin(c / 2).  Below
+// x^2 = (c/2)^2 = 0.025 (18 degrees) the series of asin(x)/x to x^16 is
+// exact to rounding (the first omitted term, 12155/1245184 x^18, is below
+// 4e-17 of the sum there).
 __device__ __forceinline__ double sph_angle(double d2)
 {
-    const double x = 0.5 * sqrt(d2);
+    const double x = d2 > 0.0 ? 0.5 * sqrt_pos(d2) : 0.0;
     const double x2 = x * x;
-    if (x2 < 1.3e-3) {
-        double t = 231.0 / 13312.0;
+    if (x2 < 0.025) {
+        double t = 6435.0 / 557056.0;
+        t = fma(t, x2, 143.0 / 10240.0);
+        t = fma(t, x2, 231.0 / 13312.0);
         t = fma(t, x2, 63.0 / 2816.0);
         t = fma(t, x2, 35.0 / 1152.0);
         t = fma(t, x2, 5.0 / 112.0);
@@ -238,7 +264,7 @@ __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T,
         const T E = fma(L.fe0, g.a, fma(L.fe1, g.b, L.fe2 * g.c));
         const T N = fma(L.fn0, g.a, fma(L.fn1, g.b, L.fn2 * g.c));
         const T s2 = fma(E, E, N * N);
-        inv = (s2 > (T)0) ? (T)1 / s2 : (T)0;
+        inv = (s2 > (T)0) ? rcp_pos(s2) : (T)0;
         u = fma(E, E, -(N * N));
         v = E * N;
         d = sph_angle(g.d2);
