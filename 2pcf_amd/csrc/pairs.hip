@@ -30,6 +30,30 @@ void fill_u64(hipStream_t s, unsigned long long *p, int64_t n);
 void fill_f64(hipStream_t s, double *p, int64_t n);
 
 
+// 1/sqrt(d2) for d2 > 0 from the hardware reciprocal square root and two
+// Newton-Raphson steps r <- r (3/2 - d2 r^2 / 2), each of which squares the
+// seed's relative error (within 2 ulp of the rounded value).
+__device__ __forceinline__ double rsq_pos(double d2)
+{
+    double r = __builtin_amdgcn_rsq(d2);
+    const double h = 0.5 * d2;
+    r = r * fma(-h * r, r, 1.5);
+    r = r * fma(-h * r, r, 1.5);
+    return r;
+}
+
+__device__ __forceinline__ double sqrt_pos(double d2) { return d2 * rsq_pos(d2); }
+
+// 1/s for s > 0 from the hardware reciprocal and two Newton-Raphson steps
+// r <- r + r (1 - s r).
+__device__ __forceinline__ double rcp_pos(double s)
+{
+    double r = __builtin_amdgcn_rcp(s);
+    r = fma(r, fma(-s, r, 1.0), r);
+    r = fma(r, fma(-s, r, 1.0), r);
+    return r;
+}
+
 template <typename T>
 struct Geo;
 
@@ -47,12 +71,15 @@ struct Geo<float> {
 
 template <>
 struct Geo<double> {
+    // separation and 1/d^2 from d^2 > 0 with one refined reciprocal square
+    // root (the IEEE-rounded sqrt and division sequences cost twice as much)
     static __device__ inline void dinv(double d2, double &d, double &inv)
     {
-        d = sqrt(d2);
-        inv = 1.0 / d2;
+        const double r = rsq_pos(d2);
+        d = d2 * r;
+        inv = r * r;
     }
-    static __device__ inline double dist(double d2) { return sqrt(d2); }
+    static __device__ inline double dist(double d2) { return d2 > 0.0 ? sqrt_pos(d2) : 0.0; }
 };
 
 // d^2 < edge test for the squared edge with index e.  FP32: decided in FP32
@@ -200,28 +227,6 @@ __device__ __forceinline__ PairGeo<T, SPH> pair_geo(const LaneData<T> &L, const 
         g.d2 = fma(g.a, g.a, fma(g.b, g.b, g.c * g.c));
     }
     return g;
-}
-
-// sqrt(d2) for d2 > 0 from the hardware reciprocal square root and two
-// Newton-Raphson steps r <- r (3/2 - d2 r^2 / 2), each of which squares the
-// seed's relative error (within 2 ulp of the rounded square root).
-__device__ __forceinline__ double sqrt_pos(double d2)
-{
-    double r = __builtin_amdgcn_rsq(d2);
-    const double h = 0.5 * d2;
-    r = r * fma(-h * r, r, 1.5);
-    r = r * fma(-h * r, r, 1.5);
-    return d2 * r;
-}
-
-// 1/s for s > 0 from the hardware reciprocal and two Newton-Raphson steps
-// r <- r + r (1 - s r).
-__device__ __forceinline__ double rcp_pos(double s)
-{
-    double r = __builtin_amdgcn_rcp(s);
-    r = fma(r, fma(-s, r, 1.0), r);
-    r = fma(r, fma(-s, r, 1.0), r);
-    return r;
 }
 
 // Separation angle from the squared chord: 2 asin(c / 2).  Below
