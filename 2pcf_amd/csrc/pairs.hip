@@ -717,8 +717,12 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                 if (kmin == kmax && !zc && lo2 > (T)0) {
                     cls = 0;
                     // ---- fast: the whole segment is in bin kmin ----
-                    if (!(w.open && pb == w.pb && (kmin == w.wk || kmin == w.wk + 1)))
-                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, kmin);
+                    if (!(w.open && pb == w.pb && (kmin == w.wk || kmin == w.wk + 1))) {
+                        // a walk moving to lower bins keeps the bin below in
+                        // the new window, one moving up the bin above
+                        const int k0 = (w.open && pb == w.pb && kmin < w.wk) ? kmin - 1 : kmin;
+                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, k0);
+                    }
                     T sT = 0, sG1 = 0, sG2 = 0;
                     if (KIND == SHEAR) {
                         stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
