@@ -67,10 +67,9 @@ __global__ __launch_bounds__(256) void k_adj_fwd(int64_t n_lens, int32_t n_bin, 
          li += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t o = lidx[li];
         const double wi = (double)w[o], ai = (double)val[o] * wi;
-        const double *m4 = mk + ((size_t)li * n_bin + b) * 4;
-        g = fma(ai, m4[0], g);
-        x = fma(ai, m4[1], x);
-        k = fma(wi, kind == SHEAR ? m4[2] : m4[3], k);
+        g = fma(ai, mk[mark_at(n_lens, b, 0, li)], g);
+        x = fma(ai, mk[mark_at(n_lens, b, 1, li)], x);
+        k = fma(wi, mk[mark_at(n_lens, b, kind == SHEAR ? 2 : 3, li)], k);
     }
     sm[0][threadIdx.x] = g;
     sm[1][threadIdx.x] = x;
@@ -85,7 +84,8 @@ __global__ __launch_bounds__(256) void k_adj_fwd(int64_t n_lens, int32_t n_bin, 
         for (int q = 0; q < 3; ++q) unsafeAtomicAdd(&out[3 * b + q], sm[q][0]);
 }
 
-// Lens side: marks[li][b] = (P or S^E, Q or S^B, kap, Om) of sorted lens li.
+// Lens side: the marks of sorted lens li in bin b (mark_at planes) are
+// (P or S^E, Q or S^B, kap, Om).
 template <typename T>
 __global__ void k_adj_lens(int64_t n_lens, int32_t n_bin, const uint32_t *__restrict__ lidx,
                            const double *__restrict__ mk, const T *__restrict__ val, const T *__restrict__ w,
@@ -96,15 +96,15 @@ __global__ void k_adj_lens(int64_t n_lens, int32_t n_bin, const uint32_t *__rest
     const uint32_t o = lidx[li];
     double s = 0.0, sk = 0.0;
     for (int b = 0; b < n_bin; ++b) {
-        const double *m4 = mk + ((size_t)li * n_bin + b) * 4;
-        s += c.at[b] * m4[0] + c.ax[b] * m4[1];
-        sk += c.be[b] * m4[2];
+        s += c.at[b] * mk[mark_at(n_lens, b, 0, li)] + c.ax[b] * mk[mark_at(n_lens, b, 1, li)];
+        sk += c.be[b] * mk[mark_at(n_lens, b, 2, li)];
     }
     if (dval) dval[o] = (double)w[o] * s;
     if (dw) dw[o] = (double)val[o] * s - sk;
 }
 
-// Source side (roles swapped): marks[li][b] = (C or A, S', ., Om) of the
+// Source side (roles swapped): the marks of sorted source li in bin b are
+// (C or A, S', ., Om) of the
 // sorted source li.
 template <typename T, int KIND>
 __global__ void k_adj_src(int64_t n_src, int32_t n_bin, const uint32_t *__restrict__ lidx,
@@ -119,8 +119,8 @@ __global__ void k_adj_src(int64_t n_src, int32_t n_bin, const uint32_t *__restri
     const double opm = (KIND == SHEAR && m) ? 1.0 + (double)m[o] : 1.0;
     double a1 = 0.0, a2 = 0.0, aw = 0.0, bo = 0.0;
     for (int b = 0; b < n_bin; ++b) {
-        const double *m4 = mk + ((size_t)li * n_bin + b) * 4;
-        const double P = m4[0], Q = m4[1], Om = m4[3];
+        const double P = mk[mark_at(n_src, b, 0, li)], Q = mk[mark_at(n_src, b, 1, li)],
+                     Om = mk[mark_at(n_src, b, 3, li)];
         if (KIND == SHEAR) {
             a1 += c.at[b] * P + c.ax[b] * Q;
             a2 += -c.at[b] * Q + c.ax[b] * P;
@@ -409,7 +409,7 @@ struct tpcf_tape {
     tpcf_bins bins{};
     int64_t n_a = 0, n_b = 0;
     const void *in[12] = {};   // x_A, y_A, z_A, val_A, w_A, x_B, y_B, z_B, s1_B, s2_B, w_B, m_B
-    double *mk = nullptr;      // [n_lens][n_bin][4]
+    double *mk = nullptr;      // marks planes [n_bin][4][n_lens]
     uint32_t *lidx = nullptr;  // [n_lens]
     int64_t n_lens = 0;
     std::vector<double> fwd;   // [n_bin][3]

@@ -114,35 +114,48 @@ struct MarksOut {
     R *q[5];
 };
 
-// One thread per (lens, bin): a lens' row of n_bin reals is written by
-// consecutive lanes (the lenses themselves land in the caller's order, i.e.
-// scattered); the bin-0 thread also sums the row for marks_weight.
+// The marks planes [n_bin][4][n_lens] (sorted lens order) to the caller's
+// rows [n_A][n_bin] per component, through an LDS tile of 64 lenses x n_bin
+// bins: coalesced reads along the lenses, each lens' row written by
+// consecutive threads (the rows themselves land in the caller's order, i.e.
+// scattered).  marks_weight is the row sum of the W plane.
+constexpr int kMarksTile = 64;
+
 template <typename R>
-__global__ void k_marks_out(int64_t n_lens, int32_t n_bin, const uint32_t *__restrict__ lidx,
-                            const double *__restrict__ mk, MarksOut<R> out)
+__global__ __launch_bounds__(256) void k_marks_out(int64_t n_lens, int32_t n_bin, const uint32_t *__restrict__ lidx,
+                                                   const double *__restrict__ mk, MarksOut<R> out)
 {
-    const int64_t n = n_lens * n_bin;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = t / n_bin;
-        const int b = (int)(t - i * n_bin);
-        const size_t o = lidx[i];
-        const double *m4 = mk + (size_t)t * 4;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (out.q[q]) out.q[q][o * n_bin + b] = (R)m4[q];
-        if (b == 0 && out.q[4]) {
-            double mw = 0.0;
-            for (int c = 0; c < n_bin; ++c) mw += m4[4 * c + 3];
-            out.q[4][o] = (R)mw;
+    __shared__ double tile[TPCF_MAX_BIN][kMarksTile + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * kMarksTile;
+    const int nl = (int)min<int64_t>(kMarksTile, n_lens - i0);
+    for (int q = 0; q < 4; ++q) {
+        const bool want_w = (q == 3 && out.q[4] != nullptr);
+        if (!out.q[q] && !want_w) continue;   // the same for the whole block
+        for (int e = threadIdx.x; e < n_bin * kMarksTile; e += blockDim.x) {
+            const int b = e / kMarksTile, i = e % kMarksTile;
+            tile[b][i] = (i < nl) ? mk[mark_at(n_lens, b, q, i0 + i)] : 0.0;
         }
+        __syncthreads();
+        if (out.q[q])
+            for (int e = threadIdx.x; e < nl * n_bin; e += blockDim.x) {
+                const int i = e / n_bin, b = e - i * n_bin;
+                out.q[q][(size_t)lidx[i0 + i] * n_bin + b] = (R)tile[b][i];
+            }
+        if (want_w && (int)threadIdx.x < nl) {
+            double s = 0.0;
+            for (int b = 0; b < n_bin; ++b) s += tile[b][threadIdx.x];
+            out.q[4][lidx[i0 + threadIdx.x]] = (R)s;
+        }
+        __syncthreads();
     }
 }
 
-// marks [n][4]: K (component 2) = W (component 3)
-__global__ void k_marks_k_from_w(int64_t n, double *__restrict__ mk)
+// The K plane of every bin (blockIdx.y) = its W plane.
+__global__ void k_marks_k_from_w(int64_t n_lens, double *__restrict__ mk)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        mk[4 * i + 2] = mk[4 * i + 3];
+    const int b = blockIdx.y;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_lens; i += (int64_t)gridDim.x * blockDim.x)
+        mk[mark_at(n_lens, b, 2, i)] = mk[mark_at(n_lens, b, 3, i)];
 }
 
 template <typename R>
@@ -165,8 +178,8 @@ int marks_to(tpcf_ctx *ctx, hipStream_t s, int64_t n_lens, int32_t n_bin, int64_
                            out.q[q], n);
     }
     if (n_lens > 0)
-        hipLaunchKernelGGL(k_marks_out<R>, dim3((unsigned)std::min<int64_t>(1 << 20, (n_lens * n_bin + 255) / 256)),
-                           dim3(256), 0, s, n_lens, n_bin, lidx, mk, out);
+        hipLaunchKernelGGL(k_marks_out<R>, dim3((unsigned)((n_lens + kMarksTile - 1) / kMarksTile)), dim3(256), 0, s,
+                           n_lens, n_bin, lidx, mk, out);
     TPCF_HIP(ctx, hipGetLastError());
     return TPCF_OK;
 }
@@ -315,9 +328,9 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
         TPCF_TRY(run_pairs<T>(ctx, a, p, pp, ppn, marks));
         if (a.marks && p.n_lens > 0 && (a.kind == KAPPA || p.m_zero)) {
             // the pair kernel leaves K marks to this copy when K == W
-            const int64_t nm = p.n_lens * n_bin;
-            hipLaunchKernelGGL(k_marks_k_from_w, dim3((unsigned)std::min<int64_t>(8192, (nm + 255) / 256)),
-                               dim3(256), 0, s, nm, marks);
+            hipLaunchKernelGGL(k_marks_k_from_w,
+                               dim3((unsigned)std::min<int64_t>(4096, (p.n_lens + 255) / 256), (unsigned)n_bin),
+                               dim3(256), 0, s, p.n_lens, marks);
             TPCF_HIP(ctx, hipGetLastError());
         }
         if (p.n_items == 0) {

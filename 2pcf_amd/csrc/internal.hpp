@@ -307,6 +307,14 @@ struct PairArgs {
 };
 
 // One catalogue's prepared state, as the pair kernels read it.
+// Per-lens marks on the device: component planes [n_bin][4][n_lens] over the
+// sorted lenses (q = xi_E or P, xi_B or Q, K, W), so the lenses of one wave,
+// which are consecutive, add to consecutive addresses.
+__host__ __device__ inline size_t mark_at(int64_t n_lens, int bin, int q, int64_t li)
+{
+    return ((size_t)bin * 4 + (size_t)q) * (size_t)n_lens + (size_t)li;
+}
+
 template <typename T>
 struct CatArgs {
     Grid g;
@@ -321,7 +329,8 @@ struct CatArgs {
     const T *sz;
     double *pp;                     // [slot][P_A][P_B][n_bin][5]
     unsigned long long *ppn;        // [slot][P_A][P_B][n_bin]
-    double *marks;                  // [n_lens][n_bin][4] or NULL
+    double *marks;                  // [n_bin][4][n_lens] (mark_at) or NULL
+    int64_t marks_n;                // n_lens, the stride of the marks planes
     const double *sph_lp, *sph_lf;  // spherical geometry (else NULL)
     const double2 *sph_segp, *sph_chp;
 };

@@ -173,6 +173,7 @@ struct Win {
     int pb, wk;
     bool open, used0, used1, edge;
     double *acc;   // this wave's [2 WQ + 1][kWave] block in LDS
+    int64_t mstride;   // n_lens: the stride of the marks planes
     double t[6];   // per-bin sums Theta, G1, G2, W, K, N of bin `lane`, in registers (lane b owns bin b)
     uint64_t valid;   // lanes holding a lens
     uint32_t n_valid;
@@ -355,11 +356,12 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
             const double m0 = gs * daa * aG1 + nan_or_0, m1 = gs * daa * aG2 + nan_or_0;
             const double m2 = dwa * aK, m3 = dwa * aW;
             if (m0 != 0.0 || m1 != 0.0 || m2 != 0.0 || m3 != 0.0) {   // NaN != 0
-                double *mk = marks + ((size_t)L.li * n_bin + bin) * 4;
+                const int64_t n = w.mstride;
+                double *mk = marks + mark_at(n, bin, 0, L.li);
                 unsafeAtomicAdd(&mk[0], m0);
-                unsafeAtomicAdd(&mk[1], m1);
-                if (!MZ) unsafeAtomicAdd(&mk[2], m2);
-                unsafeAtomicAdd(&mk[3], m3);
+                unsafeAtomicAdd(&mk[n], m1);
+                if (!MZ) unsafeAtomicAdd(&mk[2 * n], m2);
+                unsafeAtomicAdd(&mk[3 * n], m3);
             }
         }
     }
@@ -651,6 +653,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
     w.wk = -3;
     w.valid = __ballot(L.valid);
     w.n_valid = ch.count;
+    w.mstride = C.marks_n;
     for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
 
     uint64_t cseg0 = 0, cseg1 = 0, cseg2 = 0, cpair0 = 0, cpair1 = 0, cpair2 = 0;   // class counters
@@ -848,8 +851,8 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     const T *__restrict__ lz, const T *__restrict__ la, const T *__restrict__ lw, const Seg<T> *__restrict__ segs,
     const uint32_t *__restrict__ seg_first, const int4 *__restrict__ pbox, const SrcRec<T> *__restrict__ srec,
     const T *__restrict__ sz, const T *__restrict__ e2lane, const Item *__restrict__ items, double *__restrict__ pp,
-    unsigned long long *__restrict__ ppn, double *__restrict__ marks, unsigned long long *__restrict__ probe,
-    const double *__restrict__ sph_lp, const double *__restrict__ sph_lf, const double2 *__restrict__ sph_segp,
+    unsigned long long *__restrict__ ppn, double *__restrict__ marks, const int64_t marks_n,
+    unsigned long long *__restrict__ probe, const double *__restrict__ sph_lp, const double *__restrict__ sph_lf, const double2 *__restrict__ sph_segp,
     const double2 *__restrict__ sph_chp)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -875,6 +878,7 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     C.pp = pp;
     C.ppn = ppn;
     C.marks = marks;
+    C.marks_n = marks_n;
     C.sph_lp = sph_lp;
     C.sph_lf = sph_lf;
     C.sph_segp = sph_segp;
@@ -940,6 +944,7 @@ CatArgs<T> cat_args(const Prepared<T> &p, int32_t n_pa, int32_t n_pb, double *pp
     C.pp = pp;
     C.ppn = ppn;
     C.marks = marks;
+    C.marks_n = p.n_lens;
     C.sph_lp = p.sph_lp;
     C.sph_lf = p.sph_lf;
     C.sph_segp = p.sph_segp;
@@ -981,7 +986,7 @@ int launch_pairs(tpcf_ctx *ctx, hipStream_t s, int kind, bool m_zero, bool marks
 #define TPCF_LAUNCH(K, M, Z, S)                                                                                   \
     hipLaunchKernelGGL((k_pairs<T, K, M, Z, S>), grid, block, 0, s, A, C.g, C.n_pa, C.n_pb, C.margin, C.chunks,     \
                        C.lx, C.ly, C.lz, C.la, C.lw, C.segs, C.seg_first, C.pbox, C.srec, C.sz, e2lane, items, C.pp, \
-                       C.ppn, C.marks, probe, C.sph_lp, C.sph_lf, C.sph_segp, C.sph_chp)
+                       C.ppn, C.marks, C.marks_n, probe, C.sph_lp, C.sph_lf, C.sph_segp, C.sph_chp)
     if (C.sph_lp) {
         // spherical geometry: FP64 only
         if constexpr (sizeof(T) == 8) {
