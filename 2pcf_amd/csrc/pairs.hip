@@ -832,10 +832,11 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
 // latency-bound: on the 20M x 20M shear workload 5 waves take 6.73 s, 6 take
 // 5.92 s, 7 take 5.60 s and 8 take 5.35 s.  FP32 with all m == 0 (MZ) keeps
 // 18 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
-// few cold-path spills); otherwise 22.5 KB allows 7 (72 VGPRs).  FP64 keeps 4
-// waves (no spills).
+// few cold-path spills); otherwise 22.5 KB allows 7 (72 VGPRs).  FP64 with
+// MZ runs 6 waves (80 VGPRs, some spills): on the FP64 1M x 2M shear workload
+// 4 waves take 208 ms, 5 take 187 ms and 6 take 182 ms; otherwise 4 waves.
 template <typename T, bool MZ>
-constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : 4; }
+constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : (MZ ? 6 : 4); }
 // the spherical geometry (FP64) holds seven more lane values: 3 waves
 template <typename T, bool MZ, bool SPH>
 constexpr int pair_waves() { return SPH ? 3 : pair_waves_per_simd<T, MZ>(); }
