@@ -837,9 +837,10 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
 // 4 waves take 208 ms, 5 take 187 ms and 6 take 182 ms; otherwise 4 waves.
 template <typename T, bool MZ>
 constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : (MZ ? 6 : 4); }
-// the spherical geometry (FP64) holds seven more lane values: 3 waves
+// the spherical geometry (FP64) holds seven more lane values: 4 waves with
+// spills (the 20M x 20M sphere pair kernel takes 19.5 s, 20.0 s at 3 waves)
 template <typename T, bool MZ, bool SPH>
-constexpr int pair_waves() { return SPH ? 3 : pair_waves_per_simd<T, MZ>(); }
+constexpr int pair_waves() { return SPH ? 4 : pair_waves_per_simd<T, MZ>(); }
 
 // One catalogue: one item per wave, items dealt so that consecutive items
 // (neighbouring chunks) run on one XCD and share its L2.  The catalogue's
