@@ -87,3 +87,49 @@ def test_fullsize_marks_contract_to_bin_sums(tp, ctx, wl):
         # the 1024-lens sample predicts the full pair count to a few per cent
         per_lens = float(g.n_pairs.sum()) / n
         assert abs(per_lens - ppl) <= 0.1 * ppl, (per_lens, ppl)
+
+
+def test_fullsize_adjoint_record_matches_plain_forward(tp):
+    """The differentiable form at its full size (5M kappa points x 5M shear
+    sources uniform on 1200' x 1200', 20 log bins 1'-200'): the forward the
+    recorded adjoint contracts from its per-lens sums equals the plain
+    correlation's per-bin outputs (the same FP32 per-lens partial sums, added
+    in another order: within 1e-9 of each bin's sum of |terms| / K, bounded by
+    a kappa correlation of |val| against |s1| + |s2|), for kappa-kappa and
+    shear-scalar."""
+    import torch
+    synth = importlib.import_module("2pcf_amd.synth")
+    wl = synth.config("sk_5Mx5M_uniform_f32")
+    dev = torch.device("cuda:0")
+    n = len(wl.lens["x"])
+    bins = tp.create_bins(wl.x_min, wl.x_max, wl.n_bin, tp.LOG)
+    ctx = tp.Context(0)
+    kap = dict(x=wl.lens["x"], y=wl.lens["y"], val=wl.lens["val"], w=wl.lens["w"])
+    cases = ((1, kap, dict(x=wl.lens["x"], y=wl.lens["y"], kappa_E=wl.lens["val"],
+                           kappa_B=np.ascontiguousarray(wl.src["e2"] * 0.1), w=wl.lens["w"])),
+             (SHEAR, dict(kap, z=wl.lens["z"]), wl.src))
+    for kind, lens, src in cases:
+        dl = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in lens.items()}
+        ds = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in src.items()}
+        torch.cuda.synchronize()
+        pl = {k: v.data_ptr() for k, v in dl.items()}
+        ps = {k: v.data_ptr() for k, v in ds.items()}
+        g = tp.correlate(kind, pl, ps, bins, real_bytes=8, device_inputs=True, ctx=ctx, n_A=n, n_B=n)
+        g1, g2, tape = tp.adjoint_record(kind, pl, ps, bins, ctx=ctx, n_A=n, n_B=n)
+        tape.close()
+        s1, s2 = ("e1", "e2") if kind == SHEAR else ("kappa_E", "kappa_B")
+        va = dl["val"].abs()
+        sa = ds[s1].abs() + ds[s2].abs()
+        zb = torch.zeros_like(sa)
+        torch.cuda.synchronize()
+        bound = tp.correlate(1, dict(x=pl["x"], y=pl["y"], val=va.data_ptr(), w=pl["w"]),
+                             dict(x=ps["x"], y=ps["y"], kappa_E=sa.data_ptr(), kappa_B=zb.data_ptr(), w=ps["w"]),
+                             bins, real_bytes=8, device_inputs=True, ctx=ctx, n_A=n, n_B=n)
+        scale = bound.sum_gt / g.sum_k
+        print("kind", kind, "pairs", int(g.n_pairs.sum()), flush=True)
+        assert np.all(g.n_pairs > 0)
+        for have, want in ((g1, g.g_t), (g2, g.g_x)):
+            err = np.abs(have - want) / scale
+            print("max error / (sum |terms| / K):", float(err.max()), flush=True)
+            assert np.all(err <= 1e-9), kind
+    ctx.close()
