@@ -18,7 +18,7 @@ constexpr int kWavesPerBlock = 4;  // pair kernel: 4 independent waves per workg
 constexpr int kChunk = 64;         // lenses per chunk = one lens per lane
 constexpr int kSlots = 8;          // per-XCD copies of the patch-pair accumulators
 constexpr int kNQ = 5;             // accumulated FP64 quantities: Theta, G1, G2, W, K
-constexpr uint32_t kSegMax = 1024; // longest source segment (bounds per-segment FP32 sums)
+constexpr uint32_t kSegMax = 256;  // longest source segment (bounds per-segment FP32 sums)
 constexpr int kRecPad = 4;         // readable records past the last source (prefetch slack)
 
 enum Kind { SHEAR = 0, KAPPA = 1 };

@@ -687,12 +687,13 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                 const uint32_t cnt = __builtin_amdgcn_readfirstlane(sg.count);
                 const SrcRec<T> *__restrict__ r = C.srec + __builtin_amdgcn_readfirstlane(sg.start);
                 // Warm L2 with the segment's records: lane l reads one value of
-                // record 4 l (one 128-B line per lane, 256 records; segments are
-                // never empty).  The scalar loads of the passes wait for every
-                // load in flight, so their latency is exposed once per two
-                // records; with the lines on their way to L2 it is the L2
-                // latency rather than HBM's (2 % on the 20M x 20M workload).
-                // The value is only consumed after the pass.
+                // record 4 l (FP32: one 128-B line per lane, 256 records, a
+                // whole segment; segments are never empty).  The scalar loads
+                // of the passes wait for every load in flight, so their latency
+                // is exposed once per two records; with the lines on their way
+                // to L2 it is the L2 latency rather than HBM's (2 % on the
+                // 20M x 20M workload).  The value is only consumed after the
+                // pass.
                 const uint32_t pfi = min((uint32_t)(4 * lane), cnt - 1);
                 const float pfv = reinterpret_cast<const float *>(r + pfi)[0];
                 // squared distance range between the chunk box and the segment box
