@@ -283,6 +283,22 @@ def test_empty_inputs(tp, ctx):
     assert g.n_pairs.sum() == 0 and np.all(np.isnan(g.mean_theta)) and np.all(g.g_t == 0)
 
 
+@pytest.mark.parametrize("marks_device", [False, True])
+def test_all_lenses_masked_with_marks(tp, ctx, marks_device):
+    """Every lens weight 0 (no lens is used, src/tpcf.f90:285): no pairs, NaN
+    mean_theta, g = 0, and the marks outputs and marks_weight all zero --
+    written over whatever the output arrays held."""
+    lens, src = uniform_case(95, 500, 700, 30.0, SHEAR)
+    lens["w"][:] = 0
+    bins = tp.create_bins(1.0, 10.0, 6, tp.LOG)
+    g = tp.correlate(SHEAR, lens, src, bins, marks=True, ctx=ctx, marks_device=marks_device)
+    assert g.n_pairs.sum() == 0 and np.all(np.isnan(g.mean_theta)) and np.all(g.g_t == 0)
+    for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks", "marks_weight"):
+        v = getattr(g, k)
+        v = v.cpu().numpy() if marks_device else np.asarray(v)
+        assert np.all(v == 0), k
+
+
 def test_error_paths(tp, ctx):
     bins = tp.create_bins(1.0, 10.0, 5, tp.LOG)
     lens, src = uniform_case(81, 100, 100, 20.0, KAPPA)
