@@ -624,6 +624,10 @@ class Batch:
     def run(self) -> list:
         """One run of every catalogue: a list of result dicts (per-bin
         outputs, raw sums, per-patch-pair sums when asked for)."""
+        if not self._h:
+            raise TpcfError(N.ERR_ARG, "the batch is closed")
+        if not self.ctx.handle:
+            raise TpcfError(N.ERR_ARG, "the batch's context is closed")
         rc = N.lib().tpcf_batch_run(self._h)
         if rc:
             raise TpcfError(rc, self.ctx.last_error())

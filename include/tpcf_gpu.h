@@ -291,13 +291,15 @@ int tpcf_tape_destroy(tpcf_tape *tape);
  * src/main.f90:678-695).  tpcf_batch_create_* prepares every catalogue once
  * (cells, segments, chunks: positions and masks w_A == 0, z < 0) and, with
  * TPCF_BATCH_AUTOTUNE, first times candidate cell sizes and work-item sizes on
- * the largest catalogues.  Each tpcf_batch_run re-reads the values (val, w,
+ * a sample of up to 12 catalogues of each kind spread over the batch's sizes.
+ * Each tpcf_batch_run re-reads the values (val, w,
  * z, s1, s2, m) from the same device arrays, runs every catalogue and writes
  * each catalogue's outputs (per-bin outputs, raw sums, per-patch-pair sums;
  * marks are not supported).  The first run captures the work into one HIP
  * graph (catalogues on parallel branches); later runs replay it.  Device
  * arrays only (TPCF_DEVICE_INPUTS); they must stay valid while the batch
- * exists.  m_B may be NULL (m == 0). */
+ * exists.  m_B may be NULL (m == 0).  The context must stay valid while the
+ * batch is created and run; tpcf_batch_destroy does not use it. */
 typedef struct tpcf_catalog {
     int32_t kind;                  /* enum tpcf_kind */
     int32_t n_patch_A, n_patch_B;
