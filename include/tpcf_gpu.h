@@ -336,7 +336,8 @@ int tpcf_batch_destroy(tpcf_batch *batch);
  * Replaces the resampling loop of survey_marked_point_block_bootstrap
  * (src/tpcf.f90:1165-1191; survey_marked_point_bootstrap :1230-1262 is the
  * same with one object per block) and covariance_matrix (src/utils.f90:745-764)
- * as the reference's main program applies it to the resampled g_t and g_x (src/main.f90:524-537).
+ * as the reference's main program applies it to the resampled g_t and g_x
+ * (src/main.f90:524-537).
  *
  * Blocks are lens patches: g_t, g_x, k hold each block's per-bin sums of the
  * G_t, G_x and K marks over its lenses (the per-patch-pair sums of a
