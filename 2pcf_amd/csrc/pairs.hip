@@ -11,8 +11,8 @@
 //   fast  -- every pair lies in one bin: no per-pair binning; W, K and the
 //            pair count come from the segment sums (and G for kappa);
 //   edge  -- pairs straddle one bin edge: per-pair FP32 test against the
-//            squared edge, FP64 re-test for pairs within the band; only the
-//            upper bin's share is summed per pair;
+//            squared edge, FP64 re-test for pairs within the band; each
+//            side's sums are taken directly, in exec-masked branches;
 //   multi -- several edges (or a z test): windows of two bins, bounded tests.
 // Each segment's pairs are summed per lane in the input precision (at most
 // kSegMax terms), then added to per-lane FP64 sums for a window of two
