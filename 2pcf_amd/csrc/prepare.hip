@@ -292,59 +292,71 @@ __global__ __launch_bounds__(kBlk) void k_lens_gather(int64_t n, const uint32_t 
 
 // Greedy cut of one block of sorted lenses (same patch, <= kLensBlock) into
 // chunks of at most 64 lenses whose bounding box side stays <= dmax (a jump of
-// the Hilbert order out of and back into a patch starts a new chunk).
+// the Hilbert order out of and back into a patch starts a new chunk): a chunk
+// starts at the first lens not yet taken and takes the lenses that follow
+// while its box stays within dmax.  One wave per block.  The wave loads the
+// next kChunk lenses, one per lane, and forms prefix minima and maxima of x
+// and y across the lanes (exact, so these are the boxes a lens-by-lens walk
+// grows); the first lane whose prefix box is too wide, or that lies past the
+// block, ends the chunk (ballot).  !WRITE counts each block's chunks; WRITE
+// stores their lens range, patch and box, and k_chunk_sums then adds the
+// value-dependent fields.
 template <typename T, bool WRITE>
 __global__ __launch_bounds__(kBlk) void k_chunk_blocks(int64_t n_blocks, const uint4 *__restrict__ blocks, T dmax,
                                                        const T *__restrict__ lx, const T *__restrict__ ly,
-                                                       const T *__restrict__ lz, const T *__restrict__ lw,
-                                                       const T *__restrict__ la, uint32_t *__restrict__ nch,
-                                                       const uint32_t *__restrict__ ch_off,
+                                                       uint32_t *__restrict__ nch, const uint32_t *__restrict__ ch_off,
                                                        Chunk<T> *__restrict__ chunks)
 {
-    const int64_t bi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    static_assert(kChunk == kWave, "one lens per lane");
+    const int64_t bi = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;   // wave-uniform
+    const int lane = (int)(threadIdx.x % kWave);
     if (bi >= n_blocks) return;
     const uint4 blk = blocks[bi];   // start, count, pa
     const uint32_t end = blk.x + blk.y;
-    uint32_t i = blk.x, c = 0;
-    while (i < end) {
-        const uint32_t s = i;
-        T bx0 = lx[i], bx1 = bx0, by0 = ly[i], by1 = by0;
-        ++i;
-        while (i < end && i - s < (uint32_t)kChunk) {
-            const T nx0 = lx[i] < bx0 ? lx[i] : bx0, nx1 = lx[i] > bx1 ? lx[i] : bx1;
-            const T ny0 = ly[i] < by0 ? ly[i] : by0, ny1 = ly[i] > by1 ? ly[i] : by1;
-            if (nx1 - nx0 > dmax || ny1 - ny0 > dmax) break;
-            bx0 = nx0; bx1 = nx1; by0 = ny0; by1 = ny1;
-            ++i;
-        }
-        if (WRITE) {
-            Chunk<T> ch;
-            ch.start = s;
-            ch.count = i - s;
-            ch.pa = blk.z;
-            ch.pad = 0;
-            ch.bx0 = bx0; ch.bx1 = bx1; ch.by0 = by0; ch.by1 = by1;
-            T zmax = lz[s];
-            double ws = 0, as = 0;
-            for (uint32_t q = s; q < i; ++q) {
-                zmax = fmax(zmax, lz[q]);
-                ws += (double)lw[q];
-                as += (double)la[q];
+    uint32_t s = blk.x, c = 0;
+    while (s < end) {
+        const uint32_t i = s + (uint32_t)lane;
+        const bool in = i < end;
+        T x0 = lx[in ? i : s], y0 = ly[in ? i : s];
+        T x1 = x0, y1 = y0;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const T ax0 = __shfl_up(x0, o), ax1 = __shfl_up(x1, o);
+            const T ay0 = __shfl_up(y0, o), ay1 = __shfl_up(y1, o);
+            if (lane >= o) {
+                x0 = ax0 < x0 ? ax0 : x0;
+                x1 = ax1 > x1 ? ax1 : x1;
+                y0 = ay0 < y0 ? ay0 : y0;
+                y1 = ay1 > y1 ? ay1 : y1;
             }
-            ch.zmax = zmax;
-            ch.zpad = (T)0;
-            ch.wsum = ws;
-            ch.asum = as;
-            chunks[ch_off[bi] + c] = ch;
         }
+        const uint64_t bad = __builtin_amdgcn_ballot_w64(!in || x1 - x0 > dmax || y1 - y0 > dmax);
+        // lane 0 is in the block and its box is a point, so n >= 1
+        const int n = bad ? __builtin_ctzll(bad) : kWave;
+        if (WRITE) {
+            const T bx0 = __shfl(x0, n - 1), bx1 = __shfl(x1, n - 1);
+            const T by0 = __shfl(y0, n - 1), by1 = __shfl(y1, n - 1);
+            if (lane == 0) {
+                Chunk<T> ch;
+                ch.start = s;
+                ch.count = (uint32_t)n;
+                ch.pa = blk.z;
+                ch.pad = 0;
+                ch.bx0 = bx0; ch.bx1 = bx1; ch.by0 = by0; ch.by1 = by1;
+                ch.zmax = ch.zpad = (T)0;
+                ch.wsum = ch.asum = 0.0;
+                chunks[ch_off[bi] + c] = ch;
+            }
+        }
+        s += (uint32_t)n;
         ++c;
     }
-    if (!WRITE) nch[bi] = c;
+    if (!WRITE && lane == 0) nch[bi] = c;
 }
 
-// Value-dependent fields of existing chunks (z maximum, sums of w and val w),
-// one thread per chunk, in the same order as k_chunk_blocks<T, true>: used
-// when new values arrive on an unchanged structure.
+// Value-dependent fields of chunks (z maximum, sums of w and val w), one
+// thread per chunk: after k_chunk_blocks<T, true>, and again when new values
+// arrive on an unchanged structure.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_chunk_sums(int64_t n_chunks, const T *__restrict__ lz,
                                                      const T *__restrict__ lw, const T *__restrict__ la,
@@ -590,8 +602,8 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     if (nl > 0) {
         hipLaunchKernelGGL(k_lens_gather<T>, dim3(blocks_for(nl)), dim3(kBlk), 0, s, nl, lidx2, a.xa, a.ya,
                            shear ? a.za : nullptr, a.vala, a.wa, lx, ly, lz, la, lw, (T)a.zsign);
-        hipLaunchKernelGGL((k_chunk_blocks<T, false>), dim3(blocks_for(nblk)), dim3(kBlk), 0, s, nblk, dblk, dmax, lx,
-                           ly, lz, lw, la, dnch, nullptr, nullptr);
+        hipLaunchKernelGGL((k_chunk_blocks<T, false>), dim3(blocks_for(nblk * kWave)), dim3(kBlk), 0, s, nblk, dblk,
+                           dmax, lx, ly, dnch, nullptr, nullptr);
         TPCF_HIP(ctx, hipMemcpyAsync(hnch.data(), dnch, sizeof(uint32_t) * nblk, hipMemcpyDeviceToHost, s));
         TPCF_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         TPCF_HIP(ctx, hipStreamSynchronize(s));
@@ -607,8 +619,9 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     p.margin = 1e-9 * (std::fabs(g.x0) + std::fabs(g.y0) + g.cell * (g.nx + g.ny) + x_max) + 1e-12;
     Item *items;
     if (nch > 0) {
-        hipLaunchKernelGGL((k_chunk_blocks<T, true>), dim3(blocks_for(nblk)), dim3(kBlk), 0, s, nblk, dblk, dmax, lx,
-                           ly, lz, lw, la, nullptr, dnch + nblk, chunks);
+        hipLaunchKernelGGL((k_chunk_blocks<T, true>), dim3(blocks_for(nblk * kWave)), dim3(kBlk), 0, s, nblk, dblk,
+                           dmax, lx, ly, nullptr, dnch + nblk, chunks);
+        hipLaunchKernelGGL(k_chunk_sums<T>, dim3(blocks_for(nch)), dim3(kBlk), 0, s, nch, lz, lw, la, chunks);
         // split chunks that meet many sources into several work items
         float *dwork;
         TPCF_TRY(ensure(ctx, tpcf_ctx::B_WORK, sizeof(float) * nch, (void **)&dwork));

@@ -15,7 +15,8 @@ with FP64 re-binning of pairs within 1e-5 of an edge and FP64 sums.
 
 The roofline line is for the dominant kernel (the pair kernel): achieved
 TFLOP/s = in-bin pairs x FLOPs/pair / pair-kernel time (HIP events on the
-library stream), against the MI355X FP32 vector peak.  The CPU baseline is
+library stream), against the MI355X FP32 vector peak (FP64 vector peak for
+FP64 inputs and the spherical geometry).  The CPU baseline is
 the reference's own correlation code (kd-tree, FP32, OpenMP; built from its
 sources into oracle/_ref/ by oracle/build_ref.sh), or where that is absent a C
 port of the same algorithm, timed on a lens subsample against all sources.
@@ -406,7 +407,9 @@ def main() -> int:
     per_step_pairs = pairs / args.steps
     k_ms = float(np.mean(kernel_ms))
     fpp = (FLOPS_PER_PAIR_SPH if wl.spherical else FLOPS_PER_PAIR)[wl.kind]
-    peak = FP64_PEAK_TFLOPS if wl.spherical else FP32_PEAK_TFLOPS
+    # FP64 arithmetic (FP64 inputs, or the sphere's FP64 geometry) is priced
+    # against the FP64 vector peak
+    peak = FP64_PEAK_TFLOPS if (wl.spherical or prec == "f64") else FP32_PEAK_TFLOPS
     achieved = per_step_pairs * fpp / (k_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
