@@ -229,26 +229,43 @@ __device__ __forceinline__ PairGeo<T, SPH> pair_geo(const LaneData<T> &L, const 
     return g;
 }
 
+// asin(x)/x = sum_k C(2k,k) x^2k / (4^k (2k+1)) to x^12, for x^2 = x2 <
+// 0.025: the first omitted term, 143/10240 x^14, is below 1e-16 of the sum
+// for x2 < 9.5e-3 and below 8.6e-14 at 0.025.
+__device__ __forceinline__ double asin_ratio(double x2)
+{
+    double t = 231.0 / 13312.0;
+    t = fma(t, x2, 63.0 / 2816.0);
+    t = fma(t, x2, 35.0 / 1152.0);
+    t = fma(t, x2, 5.0 / 112.0);
+    t = fma(t, x2, 3.0 / 40.0);
+    t = fma(t, x2, 1.0 / 6.0);
+    return fma(t, x2, 1.0);
+}
+
+// 2 asin(x) for x^2 = x2 in [0.025, 1] (18 to 180 degrees): the half-angle
+// step asin(x) = 2 asin(x'), x'^2 = x^2 / (2 (1 + sqrt(1 - x^2))), taken until
+// x'^2 < 0.025 (at most four times: sin^2(pi/32) < 0.025), then the series.
+// It needs a few registers, where an inlined library asin raised the pair
+// loops' register peak and made them spill.
+__device__ __noinline__ double sph_angle_wide(double x2)
+{
+    double scale = 2.0;
+    for (int k = 0; k < 4 && x2 >= 0.025; ++k) {
+        const double r = 1.0 - x2;
+        const double s = r > 0.0 ? sqrt_pos(r) : 0.0;
+        x2 = x2 * rcp_pos(2.0 + 2.0 * s);
+        scale += scale;
+    }
+    return scale * sqrt_pos(x2) * asin_ratio(x2);
+}
+
 // Separation angle from the squared chord c^2 = d2: 2 asin(x) with x = c/2,
-// = c asin(x)/x.  Below x^2 = d2/4 = 0.025 (18 degrees) the series of
-// asin(x)/x = sum_k C(2k,k) x^2k / (4^k (2k+1)) to x^12: the first omitted
-// term, 143/10240 x^14, is below 1e-16 of the sum under 11 degrees and below
-// 8.6e-14 at 18 degrees.
+// = c asin(x)/x; the series below x^2 = d2/4 = 0.025 (18 degrees).
 __device__ __forceinline__ double sph_angle(double d2)
 {
-    const double c = d2 > 0.0 ? sqrt_pos(d2) : 0.0;
-    const double x2 = 0.25 * d2;
-    if (x2 < 0.025) {
-        double t = 231.0 / 13312.0;
-        t = fma(t, x2, 63.0 / 2816.0);
-        t = fma(t, x2, 35.0 / 1152.0);
-        t = fma(t, x2, 5.0 / 112.0);
-        t = fma(t, x2, 3.0 / 40.0);
-        t = fma(t, x2, 1.0 / 6.0);
-        t = fma(t, x2, 1.0);
-        return c * t;
-    }
-    return 2.0 * asin(fmin(0.5 * c, 1.0));
+    if (d2 < 0.1) return d2 > 0.0 ? sqrt_pos(d2) * asin_ratio(0.25 * d2) : 0.0;
+    return sph_angle_wide(fmin(0.25 * d2, 1.0));
 }
 
 // Separation, 1/d^2 (or 1/sin^2 d) and the projection terms of one pair; d

@@ -38,7 +38,9 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_PAIR = {0: 33, 1: 17}     # shear-scalar, kappa-kappa (algorithmic count)
 FP32_PEAK_TFLOPS = 157.3            # MI355X published FP32 vector peak
 # spherical geometry (FP64): chord 3 + 5, bin 3, east / north 10, 1/sin^2 3 + 1,
-# cos / sin 2phi 4, projections 6, angle (sqrt + series) 16, weights and sums 11
+# cos / sin 2phi 4, projections 6, angle 16 (a square root and asin, counted
+# as a 9-term series and held fixed as the kernel's series changes), weights
+# and sums 11
 FLOPS_PER_PAIR_SPH = {0: 62, 1: 38}
 FP64_PEAK_TFLOPS = 78.6             # MI355X published FP64 vector peak
 METRIC = "pairs/sec on 1 MI355X vs host-core ref; FP32+FP64-edge vs FP64 at ≤1e-6 rel"

@@ -11,8 +11,10 @@ bin's scale (Theta, W, K relative; G_t, G_x against the sum of |terms|).
 Cases: a field around longitude pi with z masks, m, zero lens weights,
 patches and marks (FP64 inputs); a cap around the north pole that also
 crosses longitude 0 (FP32 inputs, so the FP32 entry point widens them); a
-kappa-kappa auto-correlation across longitude 0; the Python shear_2pcf
-wrapper with spherical_coords; the adjoint refusing spherical coordinates.
+kappa-kappa auto-correlation across longitude 0; bins from 5 to 150 degrees
+over the whole sphere (separations above 18 degrees take the half-angle path
+of the angle, not the series alone); the Python shear_2pcf wrapper with
+spherical_coords; the adjoint refusing spherical coordinates.
 """
 import numpy as np
 import pytest
@@ -121,6 +123,18 @@ def test_sphere_kappa_auto_across_meridian(tp, orc, ctx):
     lens = dict(x=lon, y=lat, val=kap, w=w)
     src = dict(x=lon, y=lat, kappa_E=kap, kappa_B=r.normal(0, 0.02, n), w=w)
     check(tp, orc, ctx, KAPPA, lens, src, np.radians(0.3), np.radians(6.0), 12, "f64")
+
+
+def test_sphere_wide_angles_all_sky(tp, orc, ctx):
+    # uniform over the sphere: most pairs lie between 18 and 150 degrees
+    lens, src = field(35, 1200, 1500, SHEAR, lambda r, n: r.uniform(0, 2 * np.pi, n),
+                      lambda r, n: np.arcsin(r.uniform(-1.0, 1.0, n)))
+    g, o = check(tp, orc, ctx, SHEAR, lens, src, np.radians(5.0), np.radians(150.0), 8, "f64", patches=3,
+                 marks=True)
+    # the mean angle of each bin lies inside the bin
+    edges = np.asarray(tp.create_bins(np.radians(5.0), np.radians(150.0), 8, tp.LOG).bin_edges)
+    mt = g.sum_theta / g.sum_w
+    assert np.all((mt > edges[:-1]) & (mt < edges[1:]))
 
 
 def test_sphere_python_wrapper(tp, orc):
