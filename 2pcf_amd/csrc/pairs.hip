@@ -850,11 +850,13 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
 // 18 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
 // few cold-path spills); otherwise 22.5 KB allows 7 (72 VGPRs).  FP64 with
 // MZ runs 6 waves (80 VGPRs, some spills): on the FP64 1M x 2M shear workload
-// 4 waves take 208 ms, 5 take 187 ms and 6 take 182 ms; otherwise 4 waves.
+// 4 waves take 208 ms, 5 take 187 ms and 6 take 182 ms (7: the same as 6,
+// with twice the spills).  FP64 with some m != 0 runs 5 waves (96 VGPRs): on
+// that workload with m drawn from N(0, 0.05), 203 ms at 4 waves, 185 ms at 5.
 template <typename T, bool MZ>
-constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : (MZ ? 6 : 4); }
-// the spherical geometry (FP64) holds seven more lane values: 4 waves with
-// spills (the 20M x 20M sphere pair kernel takes 19.5 s, 20.0 s at 3 waves)
+constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : (MZ ? 6 : 5); }
+// the spherical geometry (FP64) holds seven more lane values: 4 waves, up to
+// 128 VGPRs (3 waves were slower on the 20M x 20M sphere workload)
 template <typename T, bool MZ, bool SPH>
 constexpr int pair_waves() { return SPH ? 4 : pair_waves_per_simd<T, MZ>(); }
 
