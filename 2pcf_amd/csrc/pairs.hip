@@ -702,15 +702,22 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                 const uint32_t cnt = __builtin_amdgcn_readfirstlane(sg.count);
                 const SrcRec<T> *__restrict__ r = C.srec + __builtin_amdgcn_readfirstlane(sg.start);
                 // Warm L2 with the segment's records: lane l reads one value of
-                // record 4 l (FP32: one 128-B line per lane, 256 records, a
-                // whole segment; segments are never empty).  The scalar loads
-                // of the passes wait for every load in flight, so their latency
-                // is exposed once per two records; with the lines on their way
-                // to L2 it is the L2 latency rather than HBM's (2 % on the
-                // 20M x 20M workload).  The value is only consumed after the
-                // pass.
-                const uint32_t pfi = min((uint32_t)(4 * lane), cnt - 1);
+                // each 128-B line it covers (FP32, 32-B records: record 4 l, so
+                // 64 lanes cover 256 records, a whole segment; FP64, 64-B
+                // records: records 2 l and 2 l + 128; segments are never
+                // empty).  The scalar loads of the passes wait for every load
+                // in flight, so their latency is exposed once per two records;
+                // with the lines on their way to L2 it is the L2 latency
+                // rather than HBM's (2 % on the 20M x 20M workload).  The
+                // values are only consumed after the pass.
+                constexpr uint32_t kRecLine = 128 / sizeof(SrcRec<T>);
+                const uint32_t pfi = min(kRecLine * (uint32_t)lane, cnt - 1);
                 const float pfv = reinterpret_cast<const float *>(r + pfi)[0];
+                float pfv2 = 0.0f;
+                if constexpr (kRecLine * kWave < kSegMax) {
+                    const uint32_t pfj = min(kRecLine * (uint32_t)lane + kRecLine * kWave, cnt - 1);
+                    pfv2 = reinterpret_cast<const float *>(r + pfj)[0];
+                }
                 // squared distance range between the chunk box and the segment box
                 const T dxl = fmax((T)0, fmax(sg.x0 - ch.bx1, ch.bx0 - sg.x1));
                 const T dyl2 = fmax((T)0, fmax(sg.y0 - ch.by1, ch.by0 - sg.y1));
@@ -813,7 +820,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                         win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
                     }
                 }
-                asm volatile("" ::"v"(pfv));
+                asm volatile("" ::"v"(pfv), "v"(pfv2));
                 if (A.counters) {
                     if (cls == 0) { cseg0 += 1; cpair0 += (uint64_t)cnt * ch.count; }
                     else if (cls == 1) { cseg1 += 1; cpair1 += (uint64_t)cnt * ch.count; }
