@@ -16,11 +16,13 @@ MAX_BIN = 63
 LIN, LOG, IHS, SQRT = 0, 1, 2, 3            # src/utils.f90:7
 SHEAR, KAPPA = 0, 1
 
-OK, ERR_ARG, ERR_BINS, ERR_ALLOC, ERR_HIP, ERR_UNSUPPORTED, ERR_NONFINITE, ERR_NODEVICE, ERR_PATCH, ERR_GRID = range(10)
+(OK, ERR_ARG, ERR_BINS, ERR_ALLOC, ERR_HIP, ERR_UNSUPPORTED, ERR_NONFINITE, ERR_NODEVICE, ERR_PATCH, ERR_GRID,
+ ERR_MASK) = range(11)
 STATUS_NAMES = {
     OK: "TPCF_OK", ERR_ARG: "TPCF_ERR_ARG", ERR_BINS: "TPCF_ERR_BINS", ERR_ALLOC: "TPCF_ERR_ALLOC",
     ERR_HIP: "TPCF_ERR_HIP", ERR_UNSUPPORTED: "TPCF_ERR_UNSUPPORTED", ERR_NONFINITE: "TPCF_ERR_NONFINITE",
     ERR_NODEVICE: "TPCF_ERR_NODEVICE", ERR_PATCH: "TPCF_ERR_PATCH", ERR_GRID: "TPCF_ERR_GRID",
+    ERR_MASK: "TPCF_ERR_MASK",
 }
 
 FLAG_DEVICE_INPUTS = 1 << 0

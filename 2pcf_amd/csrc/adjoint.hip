@@ -297,7 +297,7 @@ int adj_check(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
               const T *wa, int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b,
               const T *wb, const T *mb, const tpcf_bins *bins, uint32_t flags)
 {
-    if (!bins || bins->n_bin < 1 || bins->n_bin > TPCF_MAX_BIN) return fail(ctx, TPCF_ERR_BINS, "bad bins");
+    TPCF_TRY(check_bins(ctx, bins));
     if (flags & TPCF_SPHERICAL)
         return fail(ctx, TPCF_ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build");
     if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE))

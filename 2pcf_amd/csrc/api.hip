@@ -83,27 +83,35 @@ __global__ void k_reduce_pp(int32_t n_pa, int32_t n_pb, int32_t n_bin, const dou
     outn[d] = (long long)n;
 }
 
-// Per-bin totals over all patch pairs: one block per bin.
+// Per-bin totals over all patch pairs: one block per bin.  tot holds the
+// FP64 sums [n_bin][kNQ] followed by the pair counts [n_bin] (int64, summed
+// as integers).
 __global__ void k_bin_totals(int64_t n_pp, int32_t n_bin, const double *__restrict__ ppo,
                              const long long *__restrict__ ppno, double *__restrict__ tot)
 {
     const int bin = blockIdx.x;
     const int64_t total = n_pp * n_bin;
-    __shared__ double sh[kNQ + 1][256];
-    double acc[kNQ + 1] = {0, 0, 0, 0, 0, 0};
+    __shared__ double sh[kNQ][256];
+    __shared__ unsigned long long shn[256];
+    double acc[kNQ] = {0, 0, 0, 0, 0};
+    unsigned long long n = 0;
     for (int64_t r = threadIdx.x; r < n_pp; r += blockDim.x) {
         const size_t d = (size_t)r * n_bin + bin;
         for (int q = 0; q < kNQ; ++q) acc[q] += ppo[(size_t)q * total + d];
-        acc[kNQ] += (double)ppno[d];
+        n += (unsigned long long)ppno[d];
     }
-    for (int q = 0; q <= kNQ; ++q) sh[q][threadIdx.x] = acc[q];
+    for (int q = 0; q < kNQ; ++q) sh[q][threadIdx.x] = acc[q];
+    shn[threadIdx.x] = n;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int q = 0; q <= kNQ; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+        if ((int)threadIdx.x < s) {
+            for (int q = 0; q < kNQ; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+            shn[threadIdx.x] += shn[threadIdx.x + s];
+        }
         __syncthreads();
     }
-    if (threadIdx.x <= kNQ) tot[bin * (kNQ + 1) + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < kNQ) tot[bin * kNQ + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x == 0) reinterpret_cast<long long *>(tot + (size_t)n_bin * kNQ)[bin] = (long long)shn[0];
 }
 
 // Per-lens marks from sorted order back to the caller's order (columns of
@@ -191,6 +199,8 @@ void put_real(void *dst, int rb, int i, double v)
     else static_cast<float *>(dst)[i] = (float)v;
 }
 
+}  // namespace
+
 int check_bins(tpcf_ctx *ctx, const tpcf_bins *b)
 {
     if (!b) return fail(ctx, TPCF_ERR_ARG, "bins is NULL");
@@ -201,6 +211,8 @@ int check_bins(tpcf_ctx *ctx, const tpcf_bins *b)
             return fail(ctx, TPCF_ERR_BINS, "bin edges must be finite, non-negative and increasing");
     return TPCF_OK;
 }
+
+namespace {
 
 template <typename T>
 int upload(tpcf_ctx *ctx, int which, const T *src, int64_t n, const T **dst)
@@ -409,9 +421,10 @@ int run_call(tpcf_ctx *ctx, CallArgs<T> &a, uint32_t flags, tpcf_out *out, doubl
     // ---- per-bin outputs and normalisation (src/tpcf.f90:369-376 / :536-543) ----
     int64_t npairs = 0;
     for (int b = 0; b < n_bin; ++b) {
-        const double *t = &tot[(size_t)b * (kNQ + 1)];
+        const double *t = &tot[(size_t)b * kNQ];
         const double th = t[0], gt = t[1], gx = t[2], w = t[3], k = t[4];
-        const int64_t n = (int64_t)std::llround(t[5]);
+        int64_t n;
+        std::memcpy(&n, &tot[(size_t)n_bin * kNQ + b], sizeof n);   // the int64 counts follow the sums
         npairs += n;
         put_real(out->theta, rb, b, (double)bins->bin_centers[b]);
         put_real(out->mean_theta, rb, b, th / w);   // NaN for an empty bin, as the reference

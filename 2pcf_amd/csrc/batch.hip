@@ -38,10 +38,11 @@ struct BatchCat {
     unsigned long long *ppn = nullptr;
     double *ppo = nullptr;        // reduced [5][P_B][P_A][n_bin]
     long long *ppno = nullptr;    // reduced [P_B][P_A][n_bin]
-    double *dtot = nullptr;       // [n_bin][6]
+    double *dtot = nullptr;       // [n_bin][5] FP64 sums, then [n_bin] int64 counts
     double *htot = nullptr;       // pinned copy of dtot
     double *hpp = nullptr;        // pinned copies of ppo / ppno (when asked for)
     long long *hppn = nullptr;
+    int32_t *herr = nullptr;      // pinned copy of the preparation's error flags (mask breaches)
     int64_t pp_total = 0;
     bool want_pp = false;
 };
@@ -106,6 +107,7 @@ struct Batch final : tpcf_batch {
             if (c->htot) (void)hipHostFree(c->htot);
             if (c->hpp) (void)hipHostFree(c->hpp);
             if (c->hppn) (void)hipHostFree(c->hppn);
+            if (c->herr) (void)hipHostFree(c->herr);
             free_sub(c->sub);
         }
         for (int i = 0; i < kBranches; ++i) {
@@ -128,6 +130,7 @@ struct Batch final : tpcf_batch {
         TPCF_TRY(launch_pairs<T>(sc, s, c.args.kind, c.p.m_zero, false, c.A, C, e2lane, c.p.items, nullptr));
         TPCF_TRY(launch_reduce(sc, s, c.args.n_pa, c.args.n_pb, bins.n_bin, c.pp, c.ppn, c.ppo, c.ppno, c.dtot));
         TPCF_HIP(sc, hipMemcpyAsync(c.htot, c.dtot, sizeof(double) * bins.n_bin * (kNQ + 1), hipMemcpyDeviceToHost, s));
+        TPCF_HIP(sc, hipMemcpyAsync(c.herr, c.p.err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         if (c.want_pp) {
             TPCF_HIP(sc, hipMemcpyAsync(c.hpp, c.ppo, sizeof(double) * kNQ * c.pp_total, hipMemcpyDeviceToHost, s));
             TPCF_HIP(sc, hipMemcpyAsync(c.hppn, c.ppno, sizeof(long long) * c.pp_total, hipMemcpyDeviceToHost, s));
@@ -168,10 +171,11 @@ struct Batch final : tpcf_batch {
             if (!out) continue;
             const int rb = out->real_bytes == 8 ? 8 : 4;
             for (int b = 0; b < n_bin; ++b) {
-                const double *t = c.args.n_a && c.args.n_b ? &c.htot[(size_t)b * (kNQ + 1)] : nullptr;
+                const double *t = c.args.n_a && c.args.n_b ? &c.htot[(size_t)b * kNQ] : nullptr;
                 const double th = t ? t[0] : 0, gt = t ? t[1] : 0, gx = t ? t[2] : 0, w = t ? t[3] : 0,
                              k = t ? t[4] : 0;
-                const int64_t n = t ? (int64_t)std::llround(t[5]) : 0;
+                int64_t n = 0;   // the int64 counts follow the sums
+                if (t) std::memcpy(&n, &c.htot[(size_t)n_bin * kNQ + b], sizeof n);
                 total += n;
                 auto put = [&](void *dst, double v) {
                     if (!dst) return;
@@ -228,6 +232,16 @@ struct Batch final : tpcf_batch {
             TPCF_TRY(enqueue_all());
         }
         TPCF_HIP(ctx, hipStreamSynchronize(s));
+        // the masks are fixed at planning: a kept object that the new values
+        // mask would still contribute, where the reference skips it per call
+        for (size_t i = 0; i < cats.size(); ++i) {
+            const BatchCat<T> &c = *cats[i];
+            if (c.herr && (*c.herr & (8 | 16)))
+                return fail(ctx, TPCF_ERR_MASK,
+                            "catalogue " + std::to_string(i) + ": new values mask " +
+                                ((*c.herr & 16) ? "a lens (w == 0 or z < 0)" : "a source (z < 0)") +
+                                " that the batch was planned with; plan a new batch");
+        }
         write_outputs();
         stats.ms_run = now_ms() - t0;
         stats.runs += 1;
@@ -296,6 +310,8 @@ int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_catalog &d, BatchC
     if (keep_host) {
         TPCF_HIP(ctx, hipHostMalloc((void **)&c.htot, sizeof(double) * bins->n_bin * (kNQ + 1), hipHostMallocDefault));
         std::memset(c.htot, 0, sizeof(double) * bins->n_bin * (kNQ + 1));
+        TPCF_HIP(ctx, hipHostMalloc((void **)&c.herr, sizeof(int32_t), hipHostMallocDefault));
+        *c.herr = 0;
         if (c.want_pp) {
             TPCF_HIP(ctx, hipHostMalloc((void **)&c.hpp, sizeof(double) * kNQ * c.pp_total, hipHostMallocDefault));
             TPCF_HIP(ctx, hipHostMalloc((void **)&c.hppn, sizeof(long long) * c.pp_total, hipHostMallocDefault));
@@ -363,7 +379,7 @@ int batch_create(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const t
     *out = nullptr;
     ctx->err.clear();
     if (n_cat < 0 || (n_cat > 0 && !cats)) return fail(ctx, TPCF_ERR_ARG, "bad catalogue list");
-    if (!bins || bins->n_bin < 1 || bins->n_bin > TPCF_MAX_BIN) return fail(ctx, TPCF_ERR_BINS, "bad bins");
+    TPCF_TRY(check_bins(ctx, bins));
     if (!(flags & TPCF_DEVICE_INPUTS))
         return fail(ctx, TPCF_ERR_ARG, "a batch reads device arrays: TPCF_DEVICE_INPUTS is required");
     if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE | TPCF_BATCH_AUTOTUNE |

@@ -210,6 +210,9 @@ namespace tpcf {
 
 int fail(tpcf_ctx *ctx, int code, const std::string &msg);
 int ensure(tpcf_ctx *ctx, int which, size_t bytes, void **ptr);
+// TPCF_ERR_BINS unless 1 <= n_bin <= TPCF_MAX_BIN and the edges are finite,
+// non-negative and increasing (what the pair kernel's bin search assumes).
+int check_bins(tpcf_ctx *ctx, const tpcf_bins *b);
 
 #define TPCF_HIP(ctx, call)                                                              \
     do {                                                                                 \
@@ -288,7 +291,9 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p);
 // Re-reads the values (val, w, z, s1, s2, m) of the same catalogues into the
 // prepared structure on stream s: source records and segment sums, sorted
 // lens values and chunk sums.  Kernel launches only (graph-capturable).  The
-// positions and the masks (w_A == 0, z < 0) are those of the preparation.
+// positions and the masks (w_A == 0, z < 0) are those of the preparation: a
+// kept source whose new z is < 0 sets bit 8 of p.err, a kept lens whose new w
+// is 0 (or, for shear, z < 0) sets bit 16 (p.err is cleared first).
 template <typename T>
 int refresh_values(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, hipStream_t s);
 
@@ -346,7 +351,8 @@ template <typename T>
 int launch_pairs(tpcf_ctx *ctx, hipStream_t s, int kind, bool m_zero, bool marks, const PairArgs<T> &A,
                  const CatArgs<T> &C, const T *e2lane, const Item *items, unsigned long long *probe);
 // Per-patch-pair and per-bin reduction of the accumulators (kernels on s):
-// ppo [5][P_B][P_A][n_bin], ppno [P_B][P_A][n_bin], tot [n_bin][6].
+// ppo [5][P_B][P_A][n_bin], ppno [P_B][P_A][n_bin], tot [n_bin][5] FP64 sums
+// followed by [n_bin] int64 pair counts.
 int launch_reduce(tpcf_ctx *ctx, hipStream_t s, int32_t n_pa, int32_t n_pb, int32_t n_bin, const double *pp,
                   const unsigned long long *ppn, double *ppo, long long *ppno, double *tot);
 void fill_f64(hipStream_t s, double *p, int64_t n);

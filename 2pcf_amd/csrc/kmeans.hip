@@ -20,8 +20,12 @@
 // offered here (TPCF_ERR_UNSUPPORTED).
 //
 // One thread per point, the centroids (FP64, k <= 1024) in LDS; per-block
-// partial sums of w x, w y, w per cluster in LDS, added to global FP64 sums
-// once per block.  A round is two launches and one 8-byte read-back.
+// partial sums of w (x - x0), w (y - y0), w per cluster in LDS, added to global
+// sums once per block.  The sums are 64-bit fixed point (each term rounded to
+// a multiple of 2^-62 of the largest possible total), so every add is an
+// integer add: the totals, the centroids and hence the labels do not depend
+// on the order in which the blocks and lanes add, and a run is reproducible
+// bit for bit.  A round is two launches and one 8-byte read-back.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -47,15 +51,29 @@ __global__ __launch_bounds__(kKmBlk) void k_km_init(int64_t n, int32_t k, int32_
     lab[i] = l < k ? l : k - 1;
 }
 
+// Fixed-point frame of the centroid sums: term q of a point is rounded to an
+// integer multiple of 1/scale[q] after subtracting the origin (x0, y0), with
+// scale[q] = 2^62 / (n max|w| extent): no partial sum can leave the int64
+// range, and each term carries an absolute error of at most 0.5 / scale[q].
+struct KmFix {
+    double x0, y0;
+    double scale[3];   // w (x - x0), w (y - y0), w
+};
+
+__device__ __forceinline__ unsigned long long km_fix(double v, double scale)
+{
+    return (unsigned long long)__double2ll_rn(v * scale);   // two's complement: integer adds are exact
+}
+
 template <typename T, bool SPH>
 __global__ __launch_bounds__(kKmBlk) void k_km_assign(int64_t n, int32_t k, const T *__restrict__ x,
                                                       const T *__restrict__ y, const T *__restrict__ w,
-                                                      const double *__restrict__ cen, double init_best,
-                                                      int32_t *__restrict__ lab, double *__restrict__ sums,
+                                                      const double *__restrict__ cen, double init_best, KmFix fx,
+                                                      int32_t *__restrict__ lab, unsigned long long *__restrict__ sums,
                                                       unsigned long long *__restrict__ moves)
 {
     __shared__ double c0[kKmMax], c1[kKmMax], c2[kKmMax];
-    __shared__ double s[3 * kKmMax];
+    __shared__ unsigned long long s[3 * kKmMax];
     __shared__ unsigned int blk_moves;
     for (int j = threadIdx.x; j < k; j += kKmBlk) {
         const double a = cen[2 * j], b = cen[2 * j + 1];
@@ -70,7 +88,7 @@ __global__ __launch_bounds__(kKmBlk) void k_km_assign(int64_t n, int32_t k, cons
             c0[j] = a;
             c1[j] = b;
         }
-        s[3 * j] = s[3 * j + 1] = s[3 * j + 2] = 0.0;
+        s[3 * j] = s[3 * j + 1] = s[3 * j + 2] = 0ull;
     }
     if (threadIdx.x == 0) blk_moves = 0;
     __syncthreads();
@@ -106,43 +124,49 @@ __global__ __launch_bounds__(kKmBlk) void k_km_assign(int64_t n, int32_t k, cons
             lab[i] = bi;
             atomicAdd(&blk_moves, 1u);
         }
-        atomicAdd(&s[3 * l], px * pw);
-        atomicAdd(&s[3 * l + 1], py * pw);
-        atomicAdd(&s[3 * l + 2], pw);
+        atomicAdd(&s[3 * l], km_fix((px - fx.x0) * pw, fx.scale[0]));
+        atomicAdd(&s[3 * l + 1], km_fix((py - fx.y0) * pw, fx.scale[1]));
+        atomicAdd(&s[3 * l + 2], km_fix(pw, fx.scale[2]));
     }
     __syncthreads();
     for (int j = threadIdx.x; j < k; j += kKmBlk)
-        if (s[3 * j + 2] != 0.0 || s[3 * j] != 0.0) {
-            unsafeAtomicAdd(&sums[3 * j], s[3 * j]);
-            unsafeAtomicAdd(&sums[3 * j + 1], s[3 * j + 1]);
-            unsafeAtomicAdd(&sums[3 * j + 2], s[3 * j + 2]);
+        if (s[3 * j + 2] != 0ull || s[3 * j] != 0ull || s[3 * j + 1] != 0ull) {
+            atomicAdd(&sums[3 * j], s[3 * j]);
+            atomicAdd(&sums[3 * j + 1], s[3 * j + 1]);
+            atomicAdd(&sums[3 * j + 2], s[3 * j + 2]);
         }
     if (threadIdx.x == 0 && blk_moves) atomicAdd(moves, (unsigned long long)blk_moves);
 }
 
-__global__ void k_km_centroids(int32_t k, const double *__restrict__ sums, double *__restrict__ cen,
-                               double *__restrict__ cw)
+__global__ void k_km_centroids(int32_t k, const unsigned long long *__restrict__ sums, KmFix fx,
+                               double *__restrict__ cen, double *__restrict__ cw)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= k) return;
-    const double sw = sums[3 * j + 2];
+    const double sw = (double)(long long)sums[3 * j + 2] / fx.scale[2];
     cw[j] = sw;
-    cen[2 * j] = sums[3 * j] / sw;        // 0/0 = NaN for an empty cluster, as the reference
-    cen[2 * j + 1] = sums[3 * j + 1] / sw;
+    // 0/0 = NaN for an empty cluster, as the reference
+    cen[2 * j] = fx.x0 + ((double)(long long)sums[3 * j] / fx.scale[0]) / sw;
+    cen[2 * j + 1] = fx.y0 + ((double)(long long)sums[3 * j + 1] / fx.scale[1]) / sw;
 }
 
+// Per block: min and max of x and y, and max |w| (out[5 blockIdx.x + q]).
 template <typename T>
-__global__ void k_km_bbox(int64_t n, const T *__restrict__ x, const T *__restrict__ y, double *__restrict__ out)
+__global__ void k_km_bbox(int64_t n, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ w,
+                          double *__restrict__ out)
 {
-    __shared__ double sm[4][kKmBlk];
-    double a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY;
+    __shared__ double sm[5][kKmBlk];
+    double a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY, e = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kKmBlk + threadIdx.x; i < n; i += (int64_t)gridDim.x * kKmBlk) {
         a = fmin(a, (double)x[i]);
         b = fmax(b, (double)x[i]);
         c = fmin(c, (double)y[i]);
         d = fmax(d, (double)y[i]);
+        const double aw = fabs((double)w[i]);
+        e = (aw > e || aw != aw) ? aw : e;   // a NaN weight propagates
     }
     sm[0][threadIdx.x] = a; sm[1][threadIdx.x] = b; sm[2][threadIdx.x] = c; sm[3][threadIdx.x] = d;
+    sm[4][threadIdx.x] = e;
     __syncthreads();
     for (int st = kKmBlk / 2; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
@@ -150,11 +174,13 @@ __global__ void k_km_bbox(int64_t n, const T *__restrict__ x, const T *__restric
             sm[1][threadIdx.x] = fmax(sm[1][threadIdx.x], sm[1][threadIdx.x + st]);
             sm[2][threadIdx.x] = fmin(sm[2][threadIdx.x], sm[2][threadIdx.x + st]);
             sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + st]);
+            const double o = sm[4][threadIdx.x + st];
+            if (o > sm[4][threadIdx.x] || o != o) sm[4][threadIdx.x] = o;
         }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; ++q) out[4 * blockIdx.x + q] = sm[q][0];
+        for (int q = 0; q < 5; ++q) out[5 * blockIdx.x + q] = sm[q][0];
 }
 
 // fixed initial centroids, src/kmeans.f90:85-110 (in FP64)
@@ -200,7 +226,7 @@ int kmeans(tpcf_ctx *ctx, int64_t n, const T *x, const T *y, const T *w, const t
     const unsigned nb = (unsigned)((n + kKmBlk - 1) / kKmBlk);
     const unsigned nbb = std::min<unsigned>(nb, 1024);
     const size_t in_bytes = dev ? 0 : sizeof(T) * 3 * (size_t)n;
-    const size_t bytes = in_bytes + sizeof(int32_t) * n + sizeof(double) * (3 * k + 2 * k + k + 4 * nbb) + 64;
+    const size_t bytes = in_bytes + sizeof(int32_t) * n + sizeof(double) * (3 * k + 2 * k + k + 5 * nbb) + 64;
     char *wk;
     TPCF_TRY(ensure(ctx, tpcf_ctx::B_KMEANS, bytes, (void **)&wk));
     const T *dx = x, *dy = y, *dw = w;
@@ -214,22 +240,35 @@ int kmeans(tpcf_ctx *ctx, int64_t n, const T *x, const T *y, const T *w, const t
         p += (in_bytes + 15) & ~(size_t)15;
     }
     int32_t *lab = reinterpret_cast<int32_t *>(p);
-    double *sums = reinterpret_cast<double *>(p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
-    double *cen = sums + 3 * k, *cw = cen + 2 * k, *bb = cw + k;
-    unsigned long long *moves = reinterpret_cast<unsigned long long *>(bb + 4 * nbb);
+    unsigned long long *sums = reinterpret_cast<unsigned long long *>(p + ((sizeof(int32_t) * n + 15) & ~(size_t)15));
+    double *cen = reinterpret_cast<double *>(sums + 3 * k), *cw = cen + 2 * k, *bb = cw + k;
+    unsigned long long *moves = reinterpret_cast<unsigned long long *>(bb + 5 * nbb);
 
     // bounding box: the starting grid and the search bound 10 x diameter (src/kmeans.f90:62-66, :160)
-    hipLaunchKernelGGL(k_km_bbox<T>, dim3(nbb), dim3(kKmBlk), 0, s, n, dx, dy, bb);
-    std::vector<double> hb(4 * nbb);
+    hipLaunchKernelGGL(k_km_bbox<T>, dim3(nbb), dim3(kKmBlk), 0, s, n, dx, dy, dw, bb);
+    std::vector<double> hb(5 * nbb);
     TPCF_HIP(ctx, hipMemcpyAsync(hb.data(), bb, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, s));
     TPCF_HIP(ctx, hipStreamSynchronize(s));
-    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY, wmax = 0.0;
     for (unsigned b = 0; b < nbb; ++b) {
-        xmin = std::min(xmin, hb[4 * b]); xmax = std::max(xmax, hb[4 * b + 1]);
-        ymin = std::min(ymin, hb[4 * b + 2]); ymax = std::max(ymax, hb[4 * b + 3]);
+        xmin = std::min(xmin, hb[5 * b]); xmax = std::max(xmax, hb[5 * b + 1]);
+        ymin = std::min(ymin, hb[5 * b + 2]); ymax = std::max(ymax, hb[5 * b + 3]);
+        wmax = (hb[5 * b + 4] > wmax || std::isnan(hb[5 * b + 4])) ? hb[5 * b + 4] : wmax;
     }
     if (!std::isfinite(xmin) || !std::isfinite(xmax) || !std::isfinite(ymin) || !std::isfinite(ymax))
         return fail(ctx, TPCF_ERR_NONFINITE, "non-finite coordinate");
+    if (!std::isfinite(wmax)) return fail(ctx, TPCF_ERR_NONFINITE, "non-finite weight");
+    // the fixed-point frame: |sum| <= n max|w| extent < 2^62 for every partial sum
+    KmFix fx;
+    fx.x0 = xmin;
+    fx.y0 = ymin;
+    {
+        const double wm = wmax > 0.0 ? wmax : 1.0, top = std::ldexp(1.0, 62);
+        const double ex = xmax > xmin ? xmax - xmin : 1.0, ey = ymax > ymin ? ymax - ymin : 1.0;
+        fx.scale[0] = top / ((double)n * wm * ex);
+        fx.scale[1] = top / ((double)n * wm * ey);
+        fx.scale[2] = top / ((double)n * wm);
+    }
     double diam = std::sqrt((xmax - xmin) * (xmax - xmin) + (ymax - ymin) * (ymax - ymin)) * 10.0;
     // compared as squared distances (chords on the sphere: 10 diameters exceed any chord)
     const double init_best = sph ? 5.0 : diam * diam;
@@ -247,15 +286,15 @@ int kmeans(tpcf_ctx *ctx, int64_t n, const T *x, const T *y, const T *w, const t
     unsigned long long hm = 0;
     while (it <= o->max_iterations) {
         ++it;
-        fill_f64(s, sums, 3 * k);
+        fill_u64(s, sums, 3 * k);
         fill_u64(s, moves, 1);
         if (sph)
             hipLaunchKernelGGL((k_km_assign<T, true>), dim3(nb), dim3(kKmBlk), 0, s, n, k, dx, dy, dw, cen, init_best,
-                               lab, sums, moves);
+                               fx, lab, sums, moves);
         else
             hipLaunchKernelGGL((k_km_assign<T, false>), dim3(nb), dim3(kKmBlk), 0, s, n, k, dx, dy, dw, cen, init_best,
-                               lab, sums, moves);
-        hipLaunchKernelGGL(k_km_centroids, dim3((k + 255) / 256), dim3(256), 0, s, k, sums, cen, cw);
+                               fx, lab, sums, moves);
+        hipLaunchKernelGGL(k_km_centroids, dim3((k + 255) / 256), dim3(256), 0, s, k, sums, fx, cen, cw);
         TPCF_HIP(ctx, hipMemcpyAsync(&hm, moves, sizeof hm, hipMemcpyDeviceToHost, s));
         TPCF_HIP(ctx, hipStreamSynchronize(s));
         if (hm == 0 || (double)hm / (double)n < o->reassignment_tol) break;

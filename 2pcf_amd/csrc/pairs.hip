@@ -158,30 +158,31 @@ __device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_
 // q = Theta, G1, G2, W, K of bin wk (q) and bin wk+1 (WQ + q), plus the
 // count of in-bin zero-separation pairs (shear with x_min == 0).  Pair counts
 // are wave totals (ballot + popcount) and go straight to the per-bin sums,
-// q = Theta, G1, G2, W, K, N, kept in registers by the lane of the bin (K
-// only when some m != 0: otherwise it equals W and is not kept).  Only
+// q = Theta, G1, G2, W, K kept in FP64 registers and the pair count N in a
+// u64 register, by the lane of the bin (K only when some m != 0: otherwise it
+// equals W and is not kept).  Only
 // the window sums use LDS (5.6 KB per wave), which leaves room for 7
 // workgroups of 4 waves per CU.
-constexpr int kQ = 6;
 // window quantities per bin: Theta, G1, G2, W (+ K unless all m == 0)
 template <bool MZ>
 constexpr int win_q() { return MZ ? 4 : 5; }
-enum { QT = 0, QG1 = 1, QG2 = 2, QW = 3, QK = 4, QN = 5 };
+enum { QT = 0, QG1 = 1, QG2 = 2, QW = 3, QK = 4 };
 
 struct Win {
     int pb, wk;
     bool open, used0, used1, edge;
     double *acc;   // this wave's [2 WQ + 1][kWave] block in LDS
     int64_t mstride;   // n_lens: the stride of the marks planes
-    double t[6];   // per-bin sums Theta, G1, G2, W, K, N of bin `lane`, in registers (lane b owns bin b)
+    double t[kNQ];   // per-bin sums Theta, G1, G2, W, K of bin `lane`, in registers (lane b owns bin b)
+    uint64_t tn;     // pair count of bin `lane` (integer: exact at any count)
     uint64_t valid;   // lanes holding a lens
     uint32_t n_valid;
 };
 
 // Adds a wave-total pair count to the table entry of bin `bin` (if real).
-__device__ __forceinline__ void tab_count(Win &w, int lane, int bin, int n_bin, double n)
+__device__ __forceinline__ void tab_count(Win &w, int lane, int bin, int n_bin, uint64_t n)
 {
-    if (bin >= 0 && bin < n_bin && lane == bin) w.t[QN] += n;
+    if (bin >= 0 && bin < n_bin && lane == bin) w.tn += n;
 }
 
 __device__ __forceinline__ double &wacc(const Win &w, int q, int lane) { return w.acc[q * kWave + lane]; }
@@ -492,9 +493,8 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         wacc(w, WQ + QK, lane) += (double)hK;
     }
     if (KIND == SHEAR && TRACK0) wacc(w, 2 * WQ, lane) += (double)sZ;
-    const double nhi = (double)n_hi;
-    tab_count(w, lane, w.wk, n_bin, (double)count * (double)w.n_valid - nhi);
-    tab_count(w, lane, w.wk + 1, n_bin, nhi);
+    tab_count(w, lane, w.wk, n_bin, (uint64_t)count * w.n_valid - n_hi);
+    tab_count(w, lane, w.wk + 1, n_bin, n_hi);
     w.used0 = w.used1 = true;
     w.edge = true;
 }
@@ -545,7 +545,7 @@ __device__ __forceinline__ void edge_side_pass(Win &w, const LaneData<T> &L, con
     wacc(w, o + QG2, lane) += (double)sG2;
     wacc(w, o + QW, lane) += (double)sW;
     if (!MZ) wacc(w, o + QK, lane) += (double)sK;
-    tab_count(w, lane, UPPER ? w.wk + 1 : w.wk, n_bin, (double)n_in);
+    tab_count(w, lane, UPPER ? w.wk + 1 : w.wk, n_bin, n_in);
     if (UPPER) w.used1 = true;
     else w.used0 = true;
     w.edge = true;
@@ -623,8 +623,8 @@ __device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const Pai
         wacc(w, WQ + QK, lane) += (double)sK1;
     }
     if (KIND == SHEAR) wacc(w, 2 * WQ, lane) += (double)sZ;
-    tab_count(w, lane, w.wk, n_bin, (double)n0);
-    tab_count(w, lane, w.wk + 1, n_bin, (double)n1);
+    tab_count(w, lane, w.wk, n_bin, n0);
+    tab_count(w, lane, w.wk + 1, n_bin, n1);
     w.used0 = w.used1 = true;
     w.edge = true;
 }
@@ -669,7 +669,8 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
     w.valid = __ballot(L.valid);
     w.n_valid = ch.count;
     w.mstride = C.marks_n;
-    for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
+    for (int q = 0; q < kNQ; ++q) w.t[q] = 0.0;
+    w.tn = 0;
 
     uint64_t cseg0 = 0, cseg1 = 0, cseg2 = 0, cpair0 = 0, cpair1 = 0, cpair2 = 0;   // class counters
     const double xm = A.x_max, mg = C.margin;
@@ -783,7 +784,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                     }
                     // pair count (and, without marks, W, K and kappa's G) from the segment sums
                     if (lane == kmin) {
-                        w.t[QN] += (double)cnt * (double)ch.count;
+                        w.tn += (uint64_t)cnt * ch.count;
                         if (!MARKS) {
                             w.t[QW] += ch.wsum * sg.W;
                             if (!MZ) w.t[QK] += ch.wsum * sg.K;
@@ -813,7 +814,9 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                     cls = 2;
                     // ---- multi: two-bin windows over [kmin, kmax], bounded tests ----
                     const T *zr = (KIND == SHEAR) ? C.sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
-                    for (int w0 = kmin; w0 <= kmax; w0 += 2) {
+                    // a window starting at n_bin would hold no in-range bin
+                    const int wlast = min(kmax, n_bin - 1);
+                    for (int w0 = kmin; w0 <= wlast; w0 += 2) {
                         win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, w0);
                         if (zc) win_pass<T, KIND, MZ, true, SPH>(w, L, A, r, zr, cnt, n_bin);
                         else win_pass<T, KIND, MZ, false, SPH>(w, L, A, r, zr, cnt, n_bin);
@@ -830,16 +833,17 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
         }
         win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
         // hand the patch's per-bin sums to the patch-pair accumulators
-        if (lane < n_bin && w.t[QN] > 0.0) {
+        if (lane < n_bin && w.tn > 0) {
             const size_t o = (((size_t)slot * C.n_pa + ch.pa) * C.n_pb + pb) * n_bin + lane;
             unsafeAtomicAdd(&C.pp[o * kNQ + 0], w.t[QT]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 1], w.t[QG1]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 2], w.t[QG2]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 3], w.t[QW]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 4], MZ ? w.t[QW] : w.t[QK]);
-            atomicAdd(&C.ppn[o], (unsigned long long)w.t[QN]);
+            atomicAdd(&C.ppn[o], (unsigned long long)w.tn);
         }
-        for (int q = 0; q < kQ; ++q) w.t[q] = 0.0;
+        for (int q = 0; q < kNQ; ++q) w.t[q] = 0.0;
+        w.tn = 0;
     }
     if (A.counters && lane == 0) {
         atomicAdd(&probe[0], (unsigned long long)cseg0);

@@ -144,6 +144,9 @@ __global__ __launch_bounds__(kBlk) void k_src_gather(int64_t n, int kind, const 
     if (i >= n) return;
     const uint32_t j = order[i];
     wave_flag(flags, 4, kind == SHEAR && m && m[j] != (T)0);
+    // a kept source that its values now mask (z < 0, src/tpcf.f90:306-309):
+    // only possible when refresh_values re-reads changed values
+    wave_flag(flags, 8, z && z[j] < (T)0);
     SrcRec<T> r;
     r.x = x[j];
     r.y = y[j];
@@ -278,11 +281,14 @@ __global__ __launch_bounds__(kBlk) void k_lens_gather(int64_t n, const uint32_t 
                                                       const T *__restrict__ z, const T *__restrict__ val,
                                                       const T *__restrict__ w, T *__restrict__ lx, T *__restrict__ ly,
                                                       T *__restrict__ lz, T *__restrict__ la, T *__restrict__ lw,
-                                                      T zsign)
+                                                      T zsign, int32_t *__restrict__ flags)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = order[i];
+    // a kept lens that its values now mask (w == 0, src/tpcf.f90:285 / :474;
+    // z < 0 for shear): only possible when refresh_values re-reads changed values
+    wave_flag(flags, 16, w[j] == (T)0 || (z && z[j] < (T)0));
     lx[i] = x[j];
     ly[i] = y[j];
     lz[i] = z ? zsign * z[j] : (T)0;
@@ -601,7 +607,7 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     std::vector<uint32_t> hnch(nblk), hchoff(nblk);
     if (nl > 0) {
         hipLaunchKernelGGL(k_lens_gather<T>, dim3(blocks_for(nl)), dim3(kBlk), 0, s, nl, lidx2, a.xa, a.ya,
-                           shear ? a.za : nullptr, a.vala, a.wa, lx, ly, lz, la, lw, (T)a.zsign);
+                           shear ? a.za : nullptr, a.vala, a.wa, lx, ly, lz, la, lw, (T)a.zsign, err);
         hipLaunchKernelGGL((k_chunk_blocks<T, false>), dim3(blocks_for(nblk * kWave)), dim3(kBlk), 0, s, nblk, dblk,
                            dmax, lx, ly, dnch, nullptr, nullptr);
         TPCF_HIP(ctx, hipMemcpyAsync(hnch.data(), dnch, sizeof(uint32_t) * nblk, hipMemcpyDeviceToHost, s));
@@ -663,6 +669,7 @@ template <typename T>
 int refresh_values(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, hipStream_t s)
 {
     const bool shear = (a.kind == SHEAR);
+    launch_fill<int32_t>(s, p.err, 0, 4);
     if (p.n_src > 0)
         hipLaunchKernelGGL(k_src_gather<T>, dim3(blocks_for(p.n_src)), dim3(kBlk), 0, s, p.n_src, a.kind, p.sidx, a.xb,
                            a.yb, a.zb, a.s1b, a.s2b, a.wb, a.mb, const_cast<SrcRec<T> *>(p.srec),
@@ -673,7 +680,7 @@ int refresh_values(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, hi
     if (p.n_lens > 0) {
         hipLaunchKernelGGL(k_lens_gather<T>, dim3(blocks_for(p.n_lens)), dim3(kBlk), 0, s, p.n_lens, p.lidx, a.xa,
                            a.ya, shear ? a.za : nullptr, a.vala, a.wa, const_cast<T *>(p.lx), const_cast<T *>(p.ly),
-                           const_cast<T *>(p.lz), const_cast<T *>(p.la), const_cast<T *>(p.lw), (T)a.zsign);
+                           const_cast<T *>(p.lz), const_cast<T *>(p.la), const_cast<T *>(p.lw), (T)a.zsign, p.err);
         if (p.n_chunks > 0)
             hipLaunchKernelGGL(k_chunk_sums<T>, dim3(blocks_for(p.n_chunks)), dim3(kBlk), 0, s, p.n_chunks, p.lz,
                                p.lw, p.la, const_cast<Chunk<T> *>(p.chunks));

@@ -26,7 +26,8 @@ module tpcf_gpu_c
     integer(c_int32_t), parameter :: TPCF_LIN = 0, TPCF_LOG = 1, TPCF_IHS = 2, TPCF_SQRT = 3
     integer(c_int32_t), parameter :: TPCF_OK = 0, TPCF_ERR_ARG = 1, TPCF_ERR_BINS = 2, TPCF_ERR_ALLOC = 3, &
                                      TPCF_ERR_HIP = 4, TPCF_ERR_UNSUPPORTED = 5, TPCF_ERR_NONFINITE = 6, &
-                                     TPCF_ERR_NODEVICE = 7, TPCF_ERR_PATCH = 8, TPCF_ERR_GRID = 9
+                                     TPCF_ERR_NODEVICE = 7, TPCF_ERR_PATCH = 8, TPCF_ERR_GRID = 9, &
+                                     TPCF_ERR_MASK = 10
     integer(c_int32_t), parameter :: TPCF_DEVICE_INPUTS = 1, TPCF_SPHERICAL = 2, TPCF_VERBOSE = 4, &
                                      TPCF_NO_KD_TREE = 8, TPCF_DEVICE_MARKS = 64
     integer(c_int32_t), parameter :: TPCF_OPT_CELL_SIZE = 1, TPCF_OPT_COUNTERS = 2, TPCF_OPT_CELL_TARGET = 3, &

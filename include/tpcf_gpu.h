@@ -47,7 +47,9 @@ enum tpcf_status {
     TPCF_ERR_NONFINITE = 6,    /* a coordinate is NaN or Inf */
     TPCF_ERR_NODEVICE = 7,     /* no usable GPU */
     TPCF_ERR_PATCH = 8,        /* patch label outside [0, n_patch) */
-    TPCF_ERR_GRID = 9          /* catalogue extent too large for the cell grid */
+    TPCF_ERR_GRID = 9,         /* catalogue extent too large for the cell grid */
+    TPCF_ERR_MASK = 10         /* tpcf_batch_run: new values mask an object the batch was planned with
+                                  (a kept lens now has w == 0 or z < 0, a kept source z < 0): plan a new batch */
 };
 
 /* Flags of the correlation calls. */

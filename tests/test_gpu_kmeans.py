@@ -2,10 +2,12 @@
 restatement of partition_data (src/kmeans.f90:12-263), itself pinned against
 the reference's own code.
 
-The GPU compares squared distances and sums the centroids with atomics in no
-fixed order, so a point at an exact tie between two centroids could go either
-way: labels must agree for >= 99.99 % of the points (all of them in these
-cases so far), centroids within 1e-9 of the field size, rounds within one.
+The GPU sums the centroids in 64-bit fixed point (integer atomics), so its
+result does not depend on the order of the adds: two runs agree bit for bit.
+Its arithmetic is not the oracle's, so a point at an exact tie between two
+centroids could still go either way: labels must agree for >= 99.99 % of the
+points (all of them in these cases so far), centroids within 1e-9 of the
+field size, rounds within one.
 Cases: clustered and uniform flat catalogues in FP64 and FP32, given starting
 centroids, spherical coordinates across longitude 0; the weight-equalised
 form and k outside [2, 1024] are refused.
@@ -76,3 +78,15 @@ def test_kmeans_refusals(tp, ctx):
         tp.kmeans(x, y, w, 1, ctx=ctx)
     with pytest.raises(tp.TpcfError):
         tp.kmeans(x, y, w, 2000, ctx=ctx)
+
+
+def test_kmeans_reproducible(tp, ctx):
+    """Two runs on the same input give identical labels, centroids and
+    weights (no order-dependent floating-point adds)."""
+    x, y, w = blobs(11, 300000, k=40)
+    a = tp.kmeans(x, y, w, 64, ctx=ctx)
+    b = tp.kmeans(x, y, w, 64, ctx=ctx)
+    np.testing.assert_array_equal(a["labels"], b["labels"])
+    np.testing.assert_array_equal(a["centroids"], b["centroids"])
+    np.testing.assert_array_equal(a["cluster_weights"], b["cluster_weights"])
+    assert a["iterations"] == b["iterations"]
