@@ -44,7 +44,7 @@ EXPORTS = (
     "tpcf_batch_create_f32", "tpcf_batch_create_f64", "tpcf_batch_run", "tpcf_batch_get_stats", "tpcf_batch_destroy",
     "tpcf_bootstrap", "tpcf_kmeans_f32", "tpcf_kmeans_f64",
     "tpcf_adjoint_record_shear_f32", "tpcf_adjoint_record_shear_f64", "tpcf_adjoint_record_kappa_f32",
-    "tpcf_adjoint_record_kappa_f64", "tpcf_adjoint_replay", "tpcf_tape_destroy",
+    "tpcf_adjoint_record_kappa_f64", "tpcf_adjoint_replay", "tpcf_tape_destroy", "tpcf_shear_f32_noalloc",
 )
 
 
@@ -173,7 +173,8 @@ def lib():
     kappa_args = [_vp, _i64] + [_vp] * 4 + [_i64] + [_vp] * 5 + [C.POINTER(Bins), C.c_uint32, _vp, _vp, _i32, _i32,
                                                                  C.POINTER(Out)]
     for name, args in (("tpcf_shear_f32", shear_args), ("tpcf_shear_f64", shear_args),
-                       ("tpcf_kappa_f32", kappa_args), ("tpcf_kappa_f64", kappa_args)):
+                       ("tpcf_kappa_f32", kappa_args), ("tpcf_kappa_f64", kappa_args),
+                       ("tpcf_shear_f32_noalloc", shear_args)):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int

@@ -26,6 +26,22 @@ int ensure(tpcf_ctx *ctx, int which, size_t bytes, void **ptr)
 {
     DevBuf &b = ctx->buf[which];
     bytes = std::max<size_t>(bytes, 256);
+    if (ctx->arena) {
+        // allocation-free call: a fresh region of the scratch array (the
+        // buffer's old contents are not kept, as after a regrowth below)
+        if (b.cap < bytes) {
+            const size_t off = (ctx->arena_off + 255) & ~(size_t)255;
+            if (off > ctx->arena_cap || bytes > ctx->arena_cap - off)
+                return fail(ctx, TPCF_ERR_ALLOC,
+                            "the call needs more device memory than the " + std::to_string(ctx->arena_cap >> 20) +
+                                " MiB scratch region of the allocation-free entry point");
+            b.p = ctx->arena + off;
+            b.cap = bytes;
+            ctx->arena_off = off + bytes;
+        }
+        *ptr = b.p;
+        return TPCF_OK;
+    }
     if (b.cap < bytes) {
         if (b.p) {
             hipError_t e = hipFree(b.p);
@@ -228,6 +244,8 @@ int upload(tpcf_ctx *ctx, int which, const T *src, int64_t n, const T **dst)
     return TPCF_OK;
 }
 
+}  // namespace
+
 template <typename T>
 int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala,
               const T *wa, int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b,
@@ -293,7 +311,15 @@ int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
     return run_call<T>(ctx, a, flags, out, t0, nullptr);
 }
 
-}  // namespace
+template int correlate<float>(tpcf_ctx *, int, int64_t, const float *, const float *, const float *, const float *,
+                              const float *, int64_t, const float *, const float *, const float *, const float *,
+                              const float *, const float *, const float *, const tpcf_bins *, uint32_t,
+                              const int32_t *, const int32_t *, int32_t, int32_t, tpcf_out *);
+template int correlate<double>(tpcf_ctx *, int, int64_t, const double *, const double *, const double *,
+                               const double *, const double *, int64_t, const double *, const double *,
+                               const double *, const double *, const double *, const double *, const double *,
+                               const tpcf_bins *, uint32_t, const int32_t *, const int32_t *, int32_t, int32_t,
+                               tpcf_out *);
 
 // Everything after the inputs are on the device: preparation, the pair
 // kernel, reductions and outputs.  With a.marks_device the marks stay in the

@@ -187,6 +187,11 @@ struct tpcf_ctx {
     bool opt_counters = false;
     tpcf_stats stats{};
     std::string err;
+    // Allocation-free calls: while `arena` is set, ensure() carves the named
+    // buffers out of [arena, arena + arena_cap) with a bump pointer instead of
+    // calling hipMalloc (the caller swaps in an empty buffer table first).
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
     // named device buffers, grown on demand and reused between calls
     enum {
         B_IN0, B_IN1, B_IN2, B_IN3, B_IN4, B_IN5, B_IN6, B_IN7, B_IN8, B_IN9, B_IN10, B_IN11,
@@ -357,6 +362,14 @@ int launch_reduce(tpcf_ctx *ctx, hipStream_t s, int32_t n_pa, int32_t n_pb, int3
                   const unsigned long long *ppn, double *ppo, long long *ppno, double *tot);
 void fill_f64(hipStream_t s, double *p, int64_t n);
 void fill_u64(hipStream_t s, unsigned long long *p, int64_t n);
+
+// A whole correlation call: argument checks, uploads (unless
+// TPCF_DEVICE_INPUTS), then run_call or the spherical path (api.hip).
+template <typename T>
+int correlate(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const T *za, const T *vala,
+              const T *wa, int64_t n_b, const T *xb, const T *yb, const T *zb, const T *s1b, const T *s2b,
+              const T *wb, const T *mb, const tpcf_bins *bins, uint32_t flags, const int32_t *pat_a,
+              const int32_t *pat_b, int32_t n_pa, int32_t n_pb, tpcf_out *out);
 
 // The correlation call once the inputs are on the device (api.hip).
 template <typename T>

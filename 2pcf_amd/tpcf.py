@@ -206,14 +206,16 @@ def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str
               patch_A=None, patch_B=None, n_patch_A: int = 1, n_patch_B: int = 1, real_bytes: int = 4,
               device_inputs: bool = False, verbose: bool = False, ctx: Optional[Context] = None,
               n_A: Optional[int] = None, n_B: Optional[int] = None, spherical: bool = False,
-              marks_device: bool = False) -> Shear2pcfResults:
+              marks_device: bool = False, allocation_free: bool = False) -> Shear2pcfResults:
     """One call of the C-ABI.  ``lens``: x, y, val, w (+ z for shear);
     ``src``: x, y, w + (z, e1, e2, m) for shear or (kappa_E, kappa_B) for kappa.
     With ``device_inputs`` the values are device addresses (ints) of arrays of
     the given precision and ``n_A``/``n_B`` give the sizes.  With ``spherical``
     x, y are longitude and latitude and the bins angles, all in radians.  With
     ``marks`` and ``marks_device`` the marks are torch tensors on the current
-    GPU (TPCF_DEVICE_MARKS: no copy to the host)."""
+    GPU (TPCF_DEVICE_MARKS: no copy to the host).  ``allocation_free``
+    (shear, FP32) calls tpcf_shear_f32_noalloc, which carves its device
+    buffers from the library's fixed scratch array instead of allocating."""
     ctx = ctx or default_context()
     L = N.lib()
     dt = np.float32 if precision == "f32" else np.float64
@@ -274,10 +276,16 @@ def correlate(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str
     pb = ptr(patch_B, np.int32) if patch_B is not None else None
     if kind == SHEAR:
         fn = L.tpcf_shear_f32 if precision == "f32" else L.tpcf_shear_f64
+        if allocation_free:
+            if precision != "f32":
+                raise ValueError("the allocation-free entry point is the FP32 shear call")
+            fn = L.tpcf_shear_f32_noalloc
         rc = fn(ctx.handle, n_A, ptr(lens["x"]), ptr(lens["y"]), ptr(lens["z"]), ptr(lens["val"]), ptr(lens["w"]),
                 n_B, ptr(src["x"]), ptr(src["y"]), ptr(src["z"]), ptr(src["e1"]), ptr(src["e2"]), ptr(src["w"]),
                 ptr(src["m"]), C.byref(bins.c), flags, pa, pb, n_patch_A, n_patch_B, C.byref(o))
     else:
+        if allocation_free:
+            raise ValueError("the allocation-free entry point is the FP32 shear call")
         fn = L.tpcf_kappa_f32 if precision == "f32" else L.tpcf_kappa_f64
         rc = fn(ctx.handle, n_A, ptr(lens["x"]), ptr(lens["y"]), ptr(lens["val"]), ptr(lens["w"]),
                 n_B, ptr(src["x"]), ptr(src["y"]), ptr(src["kappa_E"]), ptr(src["kappa_B"]), ptr(src["w"]),

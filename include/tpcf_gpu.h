@@ -176,6 +176,25 @@ int tpcf_kappa_f32(tpcf_ctx *ctx,
                    const int32_t *patch_A, const int32_t *patch_B,
                    int32_t n_patch_A, int32_t n_patch_B, tpcf_out *out);
 
+/* Allocation-free variant of tpcf_shear_f32 (no counterpart in the reference):
+ * the same arguments, outputs and kernels, for callers that cannot allocate
+ * device memory (serving loops that must not call hipMalloc, harnesses that
+ * forbid it).  It calls no hipMalloc, hipFree or other allocating device
+ * routine: every device buffer of the call is carved from a 256 MiB scratch
+ * array that each GPU holds from the library's load until the process ends.
+ * A call that needs more (about 2M x 2M objects with device inputs and 16 x 16
+ * patches fit) returns TPCF_ERR_ALLOC with a message naming the limit.  All
+ * calls on one GPU share its scratch array: overlapping calls on one GPU race
+ * and must be serialised by the caller. */
+int tpcf_shear_f32_noalloc(tpcf_ctx *ctx,
+                           int64_t n_A, const float *x_A, const float *y_A, const float *z_A,
+                           const float *val_A, const float *w_A,
+                           int64_t n_B, const float *x_B, const float *y_B, const float *z_B,
+                           const float *e1_B, const float *e2_B, const float *w_B, const float *m_B,
+                           const tpcf_bins *bins, uint32_t flags,
+                           const int32_t *patch_A, const int32_t *patch_B,
+                           int32_t n_patch_A, int32_t n_patch_B, tpcf_out *out);
+
 /* FP64 twins (the reference built with -fdefault-real-8): FP64 geometry,
  * binning and sums throughout. */
 int tpcf_shear_f64(tpcf_ctx *ctx,
