@@ -217,6 +217,23 @@ void put_real(void *dst, int rb, int i, double v)
 
 }  // namespace
 
+int launch_marks_out(tpcf_ctx *ctx, hipStream_t s, int64_t n_lens, int32_t n_bin, int64_t n_a, const uint32_t *lidx,
+                     double *mk, void *const out[5], int rb, bool k_from_w)
+{
+    if (k_from_w && n_lens > 0) {
+        hipLaunchKernelGGL(k_marks_k_from_w, dim3((unsigned)std::min<int64_t>(4096, (n_lens + 255) / 256), (unsigned)n_bin),
+                           dim3(256), 0, s, n_lens, mk);
+        TPCF_HIP(ctx, hipGetLastError());
+    }
+    if (rb == 8)
+        return marks_to<double>(ctx, s, n_lens, n_bin, n_a, lidx, mk,
+                                MarksOut<double>{{(double *)out[0], (double *)out[1], (double *)out[2],
+                                                  (double *)out[3], (double *)out[4]}});
+    return marks_to<float>(ctx, s, n_lens, n_bin, n_a, lidx, mk,
+                           MarksOut<float>{{(float *)out[0], (float *)out[1], (float *)out[2], (float *)out[3],
+                                            (float *)out[4]}});
+}
+
 int check_bins(tpcf_ctx *ctx, const tpcf_bins *b)
 {
     if (!b) return fail(ctx, TPCF_ERR_ARG, "bins is NULL");

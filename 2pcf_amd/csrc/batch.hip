@@ -1,22 +1,29 @@
 // batch.hip -- many catalogues as one graph: the field loop of the reference
 // (src/main.f90:596-617, :639-660: one shear_2pcf / kappa_2pcf call per
-// field) for mock catalogues of uneven sizes.
+// field, each writing its marks, :704-707) for mock catalogues of uneven
+// sizes.
 //
 // tpcf_batch_create_* prepares every catalogue once (sorting into cells,
 // segment, chunk and item tables: the positions and masks), optionally after
-// an autotuning pass that times candidate cell sizes and work-item sizes on
-// the largest catalogues.  tpcf_batch_run then, per catalogue, re-reads the
+// an autotuning pass that times candidate cell sizes and work-item sizes on a
+// sample of the catalogues.  tpcf_batch_run then, per catalogue, re-reads the
 // values (val, w, z, e1/e2 or kappa, m) from the caller's device arrays into
 // the prepared structure, zeroes the accumulators, runs the pair kernel,
+// writes the per-lens marks into the caller's device arrays (when asked for),
 // reduces and copies the per-bin sums to pinned host memory.  The first run
 // captures all of that into one HIP graph whose catalogues are spread over
 // kBranches parallel branches (so small catalogues run side by side); later
 // runs replay the graph.  Device inputs only: the graph reads the same device
 // pointers on every replay, so new values (a new mock realisation on the
 // same positions) are picked up by the next run.
+//
+// With TPCF_SPHERICAL every catalogue is planned on the sphere as a single
+// spherical call is (sphere.hip): FP64 projected positions and frames fixed
+// at planning, FP64 copies of FP32 values redone at the start of each run.
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -27,13 +34,15 @@ namespace {
 
 constexpr int kBranches = 8;
 
-template <typename T>
+// One catalogue.  U is the precision the catalogue is computed in: the input
+// precision on the flat sky, FP64 on the sphere.
+template <typename U>
 struct BatchCat {
     tpcf_catalog desc;
     tpcf_ctx *sub = nullptr;   // owns this catalogue's device buffers
-    CallArgs<T> args;
-    Prepared<T> p;
-    PairArgs<T> A;
+    CallArgs<U> args;
+    Prepared<U> p;
+    PairArgs<U> A;
     double *pp = nullptr;
     unsigned long long *ppn = nullptr;
     double *ppo = nullptr;        // reduced [5][P_B][P_A][n_bin]
@@ -45,6 +54,17 @@ struct BatchCat {
     int32_t *herr = nullptr;      // pinned copy of the preparation's error flags (mask breaches)
     int64_t pp_total = 0;
     bool want_pp = false;
+    // spherical catalogues: the chord bins (args.kbins points here) and the
+    // FP32 -> FP64 value copies each run repeats
+    bool sph = false;
+    tpcf_bins kb{};
+    std::vector<F64Copy> copies;
+    // per-lens marks: FP64 planes [n_bin][4][n_lens] and the caller's device
+    // outputs xi_E, xi_B, K, w marks and marks_weight of mrb bytes per element
+    bool marks = false;
+    double *mk = nullptr;
+    void *mout[5] = {};
+    int mrb = 4;
 };
 
 int sub_ctx(tpcf_ctx *parent, tpcf_ctx **out)
@@ -80,14 +100,16 @@ struct tpcf_batch {
 namespace tpcf {
 namespace {
 
-template <typename T>
+template <typename U>
 struct Batch final : tpcf_batch {
     tpcf_ctx *ctx = nullptr;   // used by create and run only: the destructor does not touch it
     int device = 0;
-    tpcf_bins bins{};
-    std::vector<std::unique_ptr<BatchCat<T>>> cats;
-    const T *e2lane = nullptr;
-    PairArgs<T> A0{};
+    tpcf_bins bins{};          // the caller's bins (the outputs' bin centres)
+    tpcf_bins kb{};            // the bins the pair kernel compares with (chord edges on the sphere)
+    bool sph = false;
+    std::vector<std::unique_ptr<BatchCat<U>>> cats;
+    const U *e2lane = nullptr;
+    PairArgs<U> A0{};
     hipStream_t br[kBranches] = {};
     hipEvent_t fork = nullptr, join[kBranches] = {};
     hipGraph_t graph = nullptr;
@@ -118,18 +140,33 @@ struct Batch final : tpcf_batch {
         if (e2own) (void)hipFree(e2own);
     }
 
-    // The work of one catalogue on stream s (kernel launches and one copy).
-    int enqueue(BatchCat<T> &c, hipStream_t s)
+    // The work of one catalogue on stream s (kernel launches and copies).
+    int enqueue(BatchCat<U> &c, hipStream_t s)
     {
         tpcf_ctx *sc = c.sub;
-        if (c.args.n_a == 0 || c.args.n_b == 0) return TPCF_OK;
-        TPCF_TRY(refresh_values<T>(sc, c.args, c.p, s));
+        const int n_bin = bins.n_bin;
+        if (c.args.n_a == 0 || c.args.n_b == 0) {
+            // no pairs: the marks outputs of the lenses are zero
+            if (c.marks && c.args.n_a > 0)
+                TPCF_TRY(launch_marks_out(sc, s, 0, n_bin, c.args.n_a, nullptr, nullptr, c.mout, c.mrb, false));
+            return TPCF_OK;
+        }
+        for (const F64Copy &f : c.copies) launch_to_f64(s, f.src, f.dst, f.n);
+        TPCF_TRY(refresh_values<U>(sc, c.args, c.p, s));
+        if constexpr (sizeof(U) == 8) {
+            // the re-gathered records hold 2 e1 w where the sphere keeps p
+            if (c.sph) sph_records_p(s, c.p, c.args.sph_pb);
+        }
         fill_f64(s, c.pp, kNQ * kSlots * c.pp_total);
         fill_u64(s, c.ppn, kSlots * c.pp_total);
-        const CatArgs<T> C = cat_args<T>(c.p, c.args.n_pa, c.args.n_pb, c.pp, c.ppn, nullptr);
-        TPCF_TRY(launch_pairs<T>(sc, s, c.args.kind, c.p.m_zero, false, c.A, C, e2lane, c.p.items, nullptr));
-        TPCF_TRY(launch_reduce(sc, s, c.args.n_pa, c.args.n_pb, bins.n_bin, c.pp, c.ppn, c.ppo, c.ppno, c.dtot));
-        TPCF_HIP(sc, hipMemcpyAsync(c.htot, c.dtot, sizeof(double) * bins.n_bin * (kNQ + 1), hipMemcpyDeviceToHost, s));
+        if (c.marks) fill_f64(s, c.mk, 4 * c.p.n_lens * n_bin);
+        const CatArgs<U> C = cat_args<U>(c.p, c.args.n_pa, c.args.n_pb, c.pp, c.ppn, c.marks ? c.mk : nullptr);
+        TPCF_TRY(launch_pairs<U>(sc, s, c.args.kind, c.p.m_zero, c.marks, c.A, C, e2lane, c.p.items, nullptr));
+        if (c.marks)
+            TPCF_TRY(launch_marks_out(sc, s, c.p.n_lens, n_bin, c.args.n_a, c.p.lidx, c.mk, c.mout, c.mrb,
+                                      c.args.kind == KAPPA || c.p.m_zero));
+        TPCF_TRY(launch_reduce(sc, s, c.args.n_pa, c.args.n_pb, n_bin, c.pp, c.ppn, c.ppo, c.ppno, c.dtot));
+        TPCF_HIP(sc, hipMemcpyAsync(c.htot, c.dtot, sizeof(double) * n_bin * (kNQ + 1), hipMemcpyDeviceToHost, s));
         TPCF_HIP(sc, hipMemcpyAsync(c.herr, c.p.err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         if (c.want_pp) {
             TPCF_HIP(sc, hipMemcpyAsync(c.hpp, c.ppo, sizeof(double) * kNQ * c.pp_total, hipMemcpyDeviceToHost, s));
@@ -166,7 +203,7 @@ struct Batch final : tpcf_batch {
         const int n_bin = bins.n_bin;
         int64_t total = 0;
         for (auto &cp : cats) {
-            BatchCat<T> &c = *cp;
+            BatchCat<U> &c = *cp;
             tpcf_out *out = c.desc.out;
             if (!out) continue;
             const int rb = out->real_bytes == 8 ? 8 : 4;
@@ -235,7 +272,7 @@ struct Batch final : tpcf_batch {
         // the masks are fixed at planning: a kept object that the new values
         // mask would still contribute, where the reference skips it per call
         for (size_t i = 0; i < cats.size(); ++i) {
-            const BatchCat<T> &c = *cats[i];
+            const BatchCat<U> &c = *cats[i];
             if (c.herr && (*c.herr & (8 | 16)))
                 return fail(ctx, TPCF_ERR_MASK,
                             "catalogue " + std::to_string(i) + ": new values mask " +
@@ -249,8 +286,11 @@ struct Batch final : tpcf_batch {
     }
 };
 
-template <typename T>
-int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_catalog &d, BatchCat<T> &c, bool keep_host)
+// Plans one catalogue whose arrays have precision T into c (precision U:
+// T on the flat sky, FP64 with sph).  kb: the batch's chord bins (sph only).
+template <typename T, typename U>
+int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_bins *kb, bool sph, const tpcf_catalog &d,
+             BatchCat<U> &c, bool keep_host)
 {
     c.desc = d;
     const bool shear = (d.kind == SHEAR);
@@ -261,8 +301,10 @@ int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_catalog &d, BatchC
         return fail(ctx, TPCF_ERR_ARG, "a required lens array is NULL");
     if (d.n_B > 0 && (!d.x_B || !d.y_B || !d.s1_B || !d.s2_B || !d.w_B || (shear && !d.z_B)))
         return fail(ctx, TPCF_ERR_ARG, "a required source array is NULL");
+    if (d.out && d.out->real_bytes != 4 && d.out->real_bytes != 8)
+        return fail(ctx, TPCF_ERR_ARG, "out->real_bytes must be 4 or 8");
     TPCF_TRY(sub_ctx(ctx, &c.sub));
-    CallArgs<T> &a = c.args;
+    CallArgs<T> a;
     a.kind = d.kind;
     a.n_a = d.n_A;
     a.n_b = d.n_B;
@@ -279,12 +321,37 @@ int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_catalog &d, BatchC
     a.bins = bins;
     a.marks = false;
     a.verbose = false;
+    if (d.out) {
+        void *m[5] = {d.out->xi_E_marks, d.out->xi_B_marks, d.out->K_marks, d.out->w_marks, d.out->marks_weight};
+        for (int q = 0; q < 5; ++q) {
+            c.mout[q] = m[q];
+            c.marks = c.marks || m[q] != nullptr;
+        }
+        c.mrb = d.out->real_bytes;
+    }
+    a.marks = c.marks;
+    tpcf_ctx *sc = c.sub;
+    if (sph) {
+        if constexpr (sizeof(U) == 8) {
+            c.sph = true;
+            c.kb = *kb;
+            // positions projected once; values copied to FP64 now and at each run
+            TPCF_TRY(sph_setup<T>(sc, a, &c.kb, c.args, &c.copies));
+        } else {
+            return fail(ctx, TPCF_ERR_ARG, "spherical catalogues are planned in FP64");
+        }
+    } else {
+        if constexpr (std::is_same<T, U>::value) c.args = a;
+        else return fail(ctx, TPCF_ERR_ARG, "flat catalogues are planned in their input precision");
+    }
     std::memset(&c.p, 0, sizeof c.p);
     c.pp_total = (int64_t)a.n_pa * a.n_pb * bins->n_bin;
     c.want_pp = d.out && (d.out->pp_n || d.out->pp_theta || d.out->pp_gt || d.out->pp_gx || d.out->pp_w || d.out->pp_k);
-    tpcf_ctx *sc = c.sub;
     if (a.n_a > 0 && a.n_b > 0) {
-        TPCF_TRY(prepare<T>(sc, a, c.p));
+        TPCF_TRY(prepare<U>(sc, c.args, c.p));
+        if constexpr (sizeof(U) == 8) {
+            if (c.sph) TPCF_TRY(sph_finish(sc, c.args, c.p));
+        }
         // m values are re-read on every run: the m == 0 kernel only when no m is given
         if (shear && a.mb) c.p.m_zero = false;
         const int64_t nblk = (c.p.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
@@ -292,6 +359,9 @@ int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_catalog &d, BatchC
         c.A.per_xcd = (nblk + kSlots - 1) / kSlots;
         TPCF_TRY(ensure(sc, tpcf_ctx::B_PP, sizeof(double) * kNQ * kSlots * c.pp_total, (void **)&c.pp));
         TPCF_TRY(ensure(sc, tpcf_ctx::B_PPN, sizeof(unsigned long long) * kSlots * c.pp_total, (void **)&c.ppn));
+        if (c.marks)
+            TPCF_TRY(ensure(sc, tpcf_ctx::B_MARKS, sizeof(double) * 4 * std::max<int64_t>(c.p.n_lens, 1) * bins->n_bin,
+                            (void **)&c.mk));
         void *vb;
         TPCF_TRY(ensure(sc, tpcf_ctx::B_OUT,
                         sizeof(double) * (kNQ + 1) * c.pp_total + sizeof(double) * bins->n_bin * (kNQ + 1) + 64, &vb));
@@ -300,7 +370,8 @@ int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_catalog &d, BatchC
         c.dtot = reinterpret_cast<double *>(c.ppno + c.pp_total);
         // the sort's scratch is not needed any more
         for (int bidx : {(int)tpcf_ctx::B_SKEY, (int)tpcf_ctx::B_SKEY2, (int)tpcf_ctx::B_SIDX, (int)tpcf_ctx::B_LKEY,
-                         (int)tpcf_ctx::B_LKEY2, (int)tpcf_ctx::B_LIDX, (int)tpcf_ctx::B_TMP, (int)tpcf_ctx::B_WORK}) {
+                         (int)tpcf_ctx::B_LKEY2, (int)tpcf_ctx::B_LIDX, (int)tpcf_ctx::B_TMP, (int)tpcf_ctx::B_WORK,
+                         (int)tpcf_ctx::B_SPH_PART}) {
             DevBuf &b = sc->buf[bidx];
             if (b.p) (void)hipFree(b.p);
             b.p = nullptr;
@@ -322,32 +393,37 @@ int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_catalog &d, BatchC
     return TPCF_OK;
 }
 
-// Times one eager run of the pair work of the given catalogues (ms).
 // The batch's pair arguments (bins, edges, tests) with one catalogue's work
 // items: plan_cat fills only the item count and layout.
-template <typename T>
-PairArgs<T> with_items(const PairArgs<T> &A0, const PairArgs<T> &planned)
+template <typename U>
+PairArgs<U> with_items(const PairArgs<U> &A0, const PairArgs<U> &planned)
 {
-    PairArgs<T> A = A0;
+    PairArgs<U> A = A0;
     A.n_items = planned.n_items;
     A.per_xcd = planned.per_xcd;
     return A;
 }
 
-template <typename T>
-int time_plan(tpcf_ctx *ctx, const tpcf_bins *bins, const std::vector<const tpcf_catalog *> &sample, double *ms)
+// Times one eager run of the pair work of the given catalogues (ms), without
+// their marks.
+template <typename T, typename U>
+int time_plan(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_bins *kb, bool sph,
+              const std::vector<const tpcf_catalog *> &sample, double *ms)
 {
-    Batch<T> b;
+    Batch<U> b;
     b.ctx = ctx;
     b.device = ctx->device;
     b.bins = *bins;
+    b.kb = *kb;
+    b.sph = sph;
     b.use_graph = false;
-    TPCF_TRY(make_pair_args<T>(ctx, bins, b.A0, &b.e2lane));
+    TPCF_TRY(make_pair_args<U>(ctx, &b.kb, b.A0, &b.e2lane));
     for (const tpcf_catalog *d : sample) {
-        auto c = std::make_unique<BatchCat<T>>();
-        TPCF_TRY(plan_cat<T>(ctx, &b.bins, *d, *c, true));
+        auto c = std::make_unique<BatchCat<U>>();
+        tpcf_catalog dd = *d;
+        dd.out = nullptr;
+        TPCF_TRY((plan_cat<T, U>(ctx, &b.bins, &b.kb, sph, dd, *c, true)));
         c->A = with_items(b.A0, c->A);
-        c->desc.out = nullptr;
         b.cats.push_back(std::move(c));
     }
     TPCF_HIP(ctx, hipEventCreate(&b.fork));
@@ -371,26 +447,20 @@ int time_plan(tpcf_ctx *ctx, const tpcf_bins *bins, const std::vector<const tpcf
     return TPCF_OK;
 }
 
-template <typename T>
-int batch_create(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const tpcf_bins *bins, uint32_t flags,
-                 tpcf_batch **out)
+template <typename T, typename U>
+int batch_build(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const tpcf_bins *bins, uint32_t flags,
+                tpcf_batch **out)
 {
-    if (!ctx || !out) return TPCF_ERR_ARG;
-    *out = nullptr;
-    ctx->err.clear();
-    if (n_cat < 0 || (n_cat > 0 && !cats)) return fail(ctx, TPCF_ERR_ARG, "bad catalogue list");
-    TPCF_TRY(check_bins(ctx, bins));
-    if (!(flags & TPCF_DEVICE_INPUTS))
-        return fail(ctx, TPCF_ERR_ARG, "a batch reads device arrays: TPCF_DEVICE_INPUTS is required");
-    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE | TPCF_BATCH_AUTOTUNE |
-                            TPCF_BATCH_NO_GRAPH))
-        return fail(ctx, TPCF_ERR_ARG, "unknown flag");
-    TPCF_HIP(ctx, hipSetDevice(ctx->device));
+    const bool sph = (flags & TPCF_SPHERICAL) != 0;
+    tpcf_bins kb = *bins;
+    if (sph) TPCF_TRY(sph_chord_bins(ctx, bins, kb));
     const double t0 = now_ms();
-    auto b = std::make_unique<Batch<T>>();
+    auto b = std::make_unique<Batch<U>>();
     b->ctx = ctx;
     b->device = ctx->device;
     b->bins = *bins;
+    b->kb = kb;
+    b->sph = sph;
     b->use_graph = !(flags & TPCF_BATCH_NO_GRAPH);
     const double saved_target = ctx->opt_cell_target, saved_split = ctx->opt_split;
     if ((flags & TPCF_BATCH_AUTOTUNE) && n_cat > 0) {
@@ -419,7 +489,7 @@ int batch_create(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const t
             ctx->opt_cell_target = cand[k][0];
             ctx->opt_split = cand[k][1];
             double ms = 0;
-            const int rc = time_plan<T>(ctx, bins, sample, &ms);
+            const int rc = time_plan<T, U>(ctx, bins, &kb, sph, sample, &ms);
             if (rc) {
                 ctx->opt_cell_target = saved_target;
                 ctx->opt_split = saved_split;
@@ -437,10 +507,10 @@ int batch_create(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const t
     }
     b->stats.cell_target = ctx->opt_cell_target;
     b->stats.split = ctx->opt_split;
-    int rc = make_pair_args<T>(ctx, bins, b->A0, &b->e2lane);
+    int rc = make_pair_args<U>(ctx, &b->kb, b->A0, &b->e2lane);
     for (int i = 0; i < n_cat && !rc; ++i) {
-        auto c = std::make_unique<BatchCat<T>>();
-        rc = plan_cat<T>(ctx, &b->bins, cats[i], *c, true);
+        auto c = std::make_unique<BatchCat<U>>();
+        rc = plan_cat<T, U>(ctx, &b->bins, &b->kb, sph, cats[i], *c, true);
         if (!rc) {
             c->A = with_items(b->A0, c->A);
             b->stats.n_items += c->p.n_items;
@@ -451,9 +521,9 @@ int batch_create(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const t
     ctx->opt_split = saved_split;
     if (rc) return rc;
     // a private copy of the squared edges: later calls on ctx reuse its table
-    TPCF_HIP(ctx, hipMalloc(&b->e2own, sizeof(T) * kWave));
-    TPCF_HIP(ctx, hipMemcpyAsync(b->e2own, b->e2lane, sizeof(T) * kWave, hipMemcpyDeviceToDevice, ctx->stream));
-    b->e2lane = static_cast<const T *>(b->e2own);
+    TPCF_HIP(ctx, hipMalloc(&b->e2own, sizeof(U) * kWave));
+    TPCF_HIP(ctx, hipMemcpyAsync(b->e2own, b->e2lane, sizeof(U) * kWave, hipMemcpyDeviceToDevice, ctx->stream));
+    b->e2lane = static_cast<const U *>(b->e2own);
     TPCF_HIP(ctx, hipEventCreate(&b->fork));
     for (int i = 0; i < kBranches; ++i) {
         TPCF_HIP(ctx, hipStreamCreate(&b->br[i]));
@@ -464,6 +534,25 @@ int batch_create(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const t
     b->stats.ms_plan = now_ms() - t0;
     *out = b.release();
     return TPCF_OK;
+}
+
+template <typename T>
+int batch_create(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const tpcf_bins *bins, uint32_t flags,
+                 tpcf_batch **out)
+{
+    if (!ctx || !out) return TPCF_ERR_ARG;
+    *out = nullptr;
+    ctx->err.clear();
+    if (n_cat < 0 || (n_cat > 0 && !cats)) return fail(ctx, TPCF_ERR_ARG, "bad catalogue list");
+    TPCF_TRY(check_bins(ctx, bins));
+    if (!(flags & TPCF_DEVICE_INPUTS))
+        return fail(ctx, TPCF_ERR_ARG, "a batch reads device arrays: TPCF_DEVICE_INPUTS is required");
+    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE | TPCF_BATCH_AUTOTUNE |
+                            TPCF_BATCH_NO_GRAPH | TPCF_SPHERICAL | TPCF_DEVICE_MARKS))
+        return fail(ctx, TPCF_ERR_ARG, "unknown flag");
+    TPCF_HIP(ctx, hipSetDevice(ctx->device));
+    if (flags & TPCF_SPHERICAL) return batch_build<T, double>(ctx, n_cat, cats, bins, flags, out);
+    return batch_build<T, T>(ctx, n_cat, cats, bins, flags, out);
 }
 
 }  // namespace

@@ -382,6 +382,32 @@ int sph_correlate(tpcf_ctx *ctx, const CallArgs<T> &a, uint32_t flags, tpcf_out 
 // After prepare: the sorted lens p and frames, the records' p and the
 // per-segment / per-chunk p ranges.
 int sph_finish(tpcf_ctx *ctx, const CallArgs<double> &a, Prepared<double> &p);
+// The chord edges 2 sin(e/2) of angle bins (what the spherical neighbour
+// search and pair kernel compare with).
+int sph_chord_bins(tpcf_ctx *ctx, const tpcf_bins *bins, tpcf_bins &kb);
+// One FP32 -> FP64 copy of an input array the spherical path made.
+struct F64Copy {
+    const float *src;
+    double *dst;
+    int64_t n;
+};
+// The front half of sph_correlate: the basis, the projected positions and
+// frames, and FP64 copies of the values (listed in *copies when given, so a
+// batch can repeat them on new values), into d with d.kbins = kb.
+template <typename T>
+int sph_setup(tpcf_ctx *ctx, const CallArgs<T> &a, const tpcf_bins *kb, CallArgs<double> &d,
+              std::vector<F64Copy> *copies);
+void launch_to_f64(hipStream_t s, const float *in, double *out, int64_t n);
+// The sources' p into record slot s1w2 (after a refresh_values re-gather).
+void sph_records_p(hipStream_t s, const Prepared<double> &p, const double *pb);
+
+// The per-lens marks of a pass (FP64 planes, sorted lens order) into the
+// caller's device arrays out[0..4] = xi_E, xi_B, K, w marks [n_a][n_bin] and
+// marks_weight [n_a] of rb bytes per element (any may be NULL): the K planes
+// are first copied from the W planes when k_from_w, every output is zeroed,
+// then the pair set's rows are written.  Kernel launches only.
+int launch_marks_out(tpcf_ctx *ctx, hipStream_t s, int64_t n_lens, int32_t n_bin, int64_t n_a, const uint32_t *lidx,
+                     double *mk, void *const out[5], int rb, bool k_from_w);
 
 double now_ms();
 

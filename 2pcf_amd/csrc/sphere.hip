@@ -216,25 +216,42 @@ int sph_basis(tpcf_ctx *ctx, int64_t n, const T *lon, const T *lat, Basis &B)
 
 }  // namespace
 
-template <typename T>
-int sph_correlate(tpcf_ctx *ctx, const CallArgs<T> &a, uint32_t flags, tpcf_out *out, double t0)
+int sph_chord_bins(tpcf_ctx *ctx, const tpcf_bins *bins, tpcf_bins &kb)
 {
-    hipStream_t s = ctx->stream;
-    const int64_t n_a = a.n_a, n_b = a.n_b;
-    const bool shear = (a.kind == SHEAR);
-    const tpcf_bins *bins = a.bins;
     const int n_bin = bins->n_bin;
     if (!(bins->edges[0] >= 0.0))
         return fail(ctx, TPCF_ERR_BINS, "spherical bins must be non-negative angles in radians");
-
     // chord edges 2 sin(e/2) for e in [0, pi]; angles past pi hold no pairs
-    tpcf_bins kb = *bins;
+    kb = *bins;
     const double pi = 3.14159265358979323846;
     for (int i = 0; i <= n_bin; ++i) kb.edges[i] = 2.0 * std::sin(0.5 * std::min(bins->edges[i], pi));
     for (int i = 1; i <= n_bin; ++i)   // keep them increasing past pi (those bins stay empty)
         if (!(kb.edges[i] > kb.edges[i - 1])) kb.edges[i] = std::nextafter(kb.edges[i - 1], INFINITY);
     kb.x_min = (float)kb.edges[0];
     kb.x_max = (float)kb.edges[n_bin];
+    return TPCF_OK;
+}
+
+void launch_to_f64(hipStream_t s, const float *in, double *out, int64_t n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_sph_f64<float>, dim3(sph_blocks(n)), dim3(kSphBlk), 0, s, n, in, out);
+}
+
+void sph_records_p(hipStream_t s, const Prepared<double> &p, const double *pb)
+{
+    if (p.n_src > 0)
+        hipLaunchKernelGGL(k_sph_src, dim3(sph_blocks(p.n_src)), dim3(kSphBlk), 0, s, p.n_src, p.sidx, pb,
+                           const_cast<SrcRec<double> *>(p.srec));
+}
+
+template <typename T>
+int sph_setup(tpcf_ctx *ctx, const CallArgs<T> &a, const tpcf_bins *kb, CallArgs<double> &d,
+              std::vector<F64Copy> *copies)
+{
+    hipStream_t s = ctx->stream;
+    const int64_t n_a = a.n_a, n_b = a.n_b;
+    const bool shear = (a.kind == SHEAR);
+    const tpcf_bins *bins = a.bins;
 
     Basis B;
     if (n_a > 0) TPCF_TRY(sph_basis<T>(ctx, n_a, a.xa, a.ya, B));
@@ -262,10 +279,11 @@ int sph_correlate(tpcf_ctx *ctx, const CallArgs<T> &a, uint32_t flags, tpcf_out 
             double *o = slot;
             slot += std::max<int64_t>(n, 1);
             if (n > 0) hipLaunchKernelGGL(k_sph_f64<T>, dim3(sph_blocks(n)), dim3(kSphBlk), 0, s, n, in, o);
+            if (copies) copies->push_back(F64Copy{reinterpret_cast<const float *>(in), o, n});
             return o;
         }
     };
-    CallArgs<double> d;
+    d = CallArgs<double>{};
     d.kind = a.kind;
     d.n_a = n_a;
     d.n_b = n_b;
@@ -274,7 +292,7 @@ int sph_correlate(tpcf_ctx *ctx, const CallArgs<T> &a, uint32_t flags, tpcf_out 
     d.patch_a = a.patch_a;
     d.patch_b = a.patch_b;
     d.bins = bins;
-    d.kbins = &kb;
+    d.kbins = kb;
     d.marks = a.marks;
     d.verbose = a.verbose;
     d.xa = qa;
@@ -294,6 +312,16 @@ int sph_correlate(tpcf_ctx *ctx, const CallArgs<T> &a, uint32_t flags, tpcf_out 
     d.sph_pb = pbv;
     d.sph_frame = fa;
     TPCF_HIP(ctx, hipGetLastError());
+    return TPCF_OK;
+}
+
+template <typename T>
+int sph_correlate(tpcf_ctx *ctx, const CallArgs<T> &a, uint32_t flags, tpcf_out *out, double t0)
+{
+    tpcf_bins kb;
+    TPCF_TRY(sph_chord_bins(ctx, a.bins, kb));
+    CallArgs<double> d;
+    TPCF_TRY(sph_setup<T>(ctx, a, &kb, d, nullptr));
     return run_call<double>(ctx, d, flags, out, t0, nullptr);
 }
 
@@ -326,6 +354,10 @@ int sph_finish(tpcf_ctx *ctx, const CallArgs<double> &a, Prepared<double> &p)
     return TPCF_OK;
 }
 
+template int sph_setup<float>(tpcf_ctx *, const CallArgs<float> &, const tpcf_bins *, CallArgs<double> &,
+                               std::vector<F64Copy> *);
+template int sph_setup<double>(tpcf_ctx *, const CallArgs<double> &, const tpcf_bins *, CallArgs<double> &,
+                                std::vector<F64Copy> *);
 template int sph_correlate<float>(tpcf_ctx *, const CallArgs<float> &, uint32_t, tpcf_out *, double);
 template int sph_correlate<double>(tpcf_ctx *, const CallArgs<double> &, uint32_t, tpcf_out *, double);
 

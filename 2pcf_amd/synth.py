@@ -134,7 +134,7 @@ def make_workload(name: str, n_a: int, n_b: int, side: float, kind: int, x_min: 
 # The benchmark configurations (sizes, patch sides in arcmin, bins, seeds).
 CONFIGS = {
     "kk_100k_auto": dict(n_a=100_000, n_b=100_000, side=600.0, kind=1, x_min=1.0, x_max=100.0, n_bin=20,
-                         n_patch=16, seed=1001, auto=True),
+                         n_patch=16, seed=1001, auto=True, precision=np.float64),
     "sk_1Mx2M_lognormal_f64": dict(n_a=1_000_000, n_b=2_000_000, side=1800.0, kind=0, x_min=0.5, x_max=250.0,
                                    n_bin=30, n_patch=64, seed=1002, clustered=True, npix=2048,
                                    precision=np.float64),
@@ -146,6 +146,59 @@ CONFIGS = {
     "sk_20Mx20M_sphere": dict(n_a=20_000_000, n_b=20_000_000, side=3000.0, kind=0, x_min=0.5, x_max=250.0,
                               n_bin=30, n_patch=128, seed=1003, clustered=True, npix=4096, spherical=True),
 }
+
+
+def mock_fields(n_fields: int = 256, device=None, seed: int = 1004, side: float = 600.0) -> list:
+    """The batched form's mock catalogues, generated on the GPU: per field a
+    kappa catalogue and a shear catalogue with sizes log-uniform in
+    [5e4, 2e6] (numpy PCG64(seed)), uniform on a side x side patch (torch
+    generator seeded with seed); kappa ~ N(0, 0.02) (kappa_B independent),
+    e1, e2 ~ N(0, 0.27), w ~ U[0.5, 1.5], lens z = 0, source z = 1, and 16
+    patches as 4 x 4 squares.  Per field two catalogues: kappa-kappa (the
+    kappa catalogue on both sides) and kappa x gamma_t.  Each entry: kind,
+    lens / src dicts of torch tensors, patch_A, patch_B (int32 tensors),
+    n_patch."""
+    import torch
+    dev = device or torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    q = side / 4.0
+
+    def uni(n, lo, hi):
+        return torch.rand(n, generator=gen, device=dev) * (hi - lo) + lo
+
+    def patch(x, y):
+        return (torch.clamp((x / q).int(), 0, 3) + 4 * torch.clamp((y / q).int(), 0, 3)).int()
+
+    cats = []
+    for _ in range(n_fields):
+        nk, ng = (int(np.exp(rng.uniform(np.log(5e4), np.log(2e6)))) for _ in range(2))
+        kx, ky = uni(nk, 0, side), uni(nk, 0, side)
+        kap = torch.randn(nk, generator=gen, device=dev) * 0.02
+        kw = uni(nk, 0.5, 1.5)
+        kz = torch.zeros(nk, device=dev)
+        gx, gy = uni(ng, 0, side), uni(ng, 0, side)
+        e1 = torch.randn(ng, generator=gen, device=dev) * 0.27
+        e2 = torch.randn(ng, generator=gen, device=dev) * 0.27
+        gw = uni(ng, 0.5, 1.5)
+        gz = torch.ones(ng, device=dev)
+        kb = torch.randn(nk, generator=gen, device=dev) * 0.02
+        pk, pg = patch(kx, ky), patch(gx, gy)
+        lens = dict(x=kx, y=ky, z=kz, val=kap, w=kw)
+        cats.append(dict(kind=1, lens=lens, src=dict(x=kx, y=ky, kappa_E=kap, kappa_B=kb, w=kw), patch_A=pk,
+                         patch_B=pk, n_patch=16))
+        cats.append(dict(kind=0, lens=lens, src=dict(x=gx, y=gy, z=gz, e1=e1, e2=e2, w=gw), patch_A=pk,
+                         patch_B=pg, n_patch=16))
+    return cats
+
+
+def batch_desc(cat: dict) -> dict:
+    """A catalogue of mock_fields as a batch descriptor (device addresses)."""
+    return dict(kind=cat["kind"], lens={k: v.data_ptr() for k, v in cat["lens"].items()},
+                src={k: v.data_ptr() for k, v in cat["src"].items()}, n_A=cat["lens"]["x"].numel(),
+                n_B=cat["src"]["x"].numel(), patch_A=cat["patch_A"].data_ptr(), patch_B=cat["patch_B"].data_ptr(),
+                n_patch_A=cat["n_patch"], n_patch_B=cat["n_patch"])
 
 
 def config(name: str, scale: float = 1.0) -> Workload:

@@ -580,10 +580,16 @@ class Batch:
     ``n_A``, ``n_B`` and optional ``patch_A``, ``patch_B`` (device int32
     addresses) with ``n_patch_A``, ``n_patch_B``.  The arrays must stay valid
     while the batch exists; ``run()`` re-reads their values each time (the
-    positions, w == 0 lenses and z < 0 masks are fixed at creation)."""
+    positions, w == 0 lenses and z < 0 masks are fixed at creation).  With
+    ``marks`` (True, or a collection of catalogue indices) each result (of
+    those catalogues) also holds the per-lens marks as torch tensors on the
+    GPU, (n_bin, n_A) views of [n_A][n_bin] FP64 arrays, rewritten by every
+    run; with ``spherical`` x, y are longitude and latitude and the bins
+    angles, all in radians."""
 
     def __init__(self, catalogs: list, bins: Binning, *, precision: str = "f32", autotune: bool = False,
-                 graph: bool = True, patches: bool = False, ctx: Optional[Context] = None):
+                 graph: bool = True, patches: bool = False, ctx: Optional[Context] = None, marks: bool = False,
+                 spherical: bool = False):
         self.ctx = ctx or default_context()
         self.bins = bins
         self._h = C.c_void_p()
@@ -620,10 +626,19 @@ class Batch:
                 for k in ("pp_theta", "pp_gt", "pp_gx", "pp_w", "pp_k"):
                     res[k] = np.zeros((pb, pa, nb), dtype=np.float64)
                     setattr(o, k, res[k].ctypes.data_as(C.c_void_p))
+            if marks is True or (marks and not isinstance(marks, bool) and i in marks):
+                import torch
+                dev = torch.device("cuda", self.ctx.device) if self.ctx.device >= 0 else torch.device("cuda")
+                for k in ("xi_E_marks", "xi_B_marks", "K_marks", "w_marks"):
+                    res[k] = torch.zeros((c.n_A, nb), dtype=torch.float64, device=dev)
+                    setattr(o, k, C.c_void_p(res[k].data_ptr()))
+                res["marks_weight"] = torch.zeros(c.n_A, dtype=torch.float64, device=dev)
+                o.marks_weight = C.c_void_p(res["marks_weight"].data_ptr())
+                torch.cuda.synchronize(dev)
             self._outs.append((o, res))
             c.out = C.pointer(o)
         flags = N.FLAG_DEVICE_INPUTS | (N.FLAG_BATCH_AUTOTUNE if autotune else 0) | \
-            (0 if graph else N.FLAG_BATCH_NO_GRAPH)
+            (0 if graph else N.FLAG_BATCH_NO_GRAPH) | (N.FLAG_SPHERICAL if spherical else 0)
         fn = N.lib().tpcf_batch_create_f32 if precision == "f32" else N.lib().tpcf_batch_create_f64
         rc = fn(self.ctx.handle, n, self._cats, C.byref(bins.c), flags, C.byref(self._h))
         if rc:
@@ -639,7 +654,18 @@ class Batch:
         rc = N.lib().tpcf_batch_run(self._h)
         if rc:
             raise TpcfError(rc, self.ctx.last_error())
-        return [{k: v.copy() for k, v in res.items()} for _, res in self._outs]
+        out = []
+        for _, res in self._outs:
+            r = {}
+            for k, v in res.items():
+                if k.endswith("marks"):
+                    r[k] = v.clone().T         # (n_bin, n_A) view of a copy of the [n_A][n_bin] array
+                elif k == "marks_weight":
+                    r[k] = v.clone()
+                else:
+                    r[k] = v.copy()
+            out.append(r)
+        return out
 
     @property
     def stats(self) -> dict:

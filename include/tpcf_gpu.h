@@ -315,12 +315,18 @@ int tpcf_tape_destroy(tpcf_tape *tape);
  * a sample of up to 12 catalogues of each kind spread over the batch's sizes.
  * Each tpcf_batch_run re-reads the values (val, w,
  * z, s1, s2, m) from the same device arrays, runs every catalogue and writes
- * each catalogue's outputs (per-bin outputs, raw sums, per-patch-pair sums;
- * marks are not supported).  The first run captures the work into one HIP
+ * each catalogue's outputs (per-bin outputs, raw sums, per-patch-pair sums,
+ * and the per-lens marks where out's marks pointers are not NULL: those are
+ * device arrays of out->real_bytes per element, written in place each run as
+ * with TPCF_DEVICE_MARKS; the reference writes every field's marks,
+ * src/main.f90:704-707).  The first run captures the work into one HIP
  * graph (catalogues on parallel branches); later runs replay it.  Device
  * arrays only (TPCF_DEVICE_INPUTS); they must stay valid while the batch
- * exists.  m_B may be NULL (m == 0).  The context must stay valid while the
- * batch is created and run; tpcf_batch_destroy does not use it. */
+ * exists.  m_B may be NULL (m == 0).  With TPCF_SPHERICAL every catalogue is
+ * on the sphere (x, y longitude and latitude, bins angles, in radians; FP64
+ * geometry as for a single call).  A run whose new values mask an object the
+ * batch was planned with returns TPCF_ERR_MASK.  The context must stay valid
+ * while the batch is created and run; tpcf_batch_destroy does not use it. */
 typedef struct tpcf_catalog {
     int32_t kind;                  /* enum tpcf_kind */
     int32_t n_patch_A, n_patch_B;
@@ -328,7 +334,7 @@ typedef struct tpcf_catalog {
     const void *x_A, *y_A, *z_A, *val_A, *w_A;          /* z_A: shear only */
     const void *x_B, *y_B, *z_B, *s1_B, *s2_B, *w_B, *m_B;   /* s1, s2: e1, e2 or kappa_E, kappa_B */
     const int32_t *patch_A, *patch_B;                   /* nullable */
-    tpcf_out *out;                 /* host outputs of this catalogue (marks pointers must be NULL) */
+    tpcf_out *out;                 /* outputs of this catalogue: host arrays, except the marks (device arrays) */
 } tpcf_catalog;
 
 typedef struct tpcf_batch_stats {
