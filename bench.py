@@ -37,13 +37,59 @@ sys.path.insert(0, ROOT)
 
 FLOPS_PER_PAIR = {0: 33, 1: 17}     # shear-scalar, kappa-kappa (algorithmic count)
 FP32_PEAK_TFLOPS = 157.3            # MI355X published FP32 vector peak
-# spherical geometry (FP64): chord 3 + 5, bin 3, east / north 10, 1/sin^2 3 + 1,
-# cos / sin 2phi 4, projections 6, angle 16 (a square root and asin, counted
-# as a 9-term series and held fixed as the kernel's series changes), weights
-# and sums 11
-FLOPS_PER_PAIR_SPH = {0: 62, 1: 38}
+# Spherical geometry, the reference's per-pair algorithm (distance_sphere,
+# angles_sphere, src/utils.f90:512-528, :618-639; src/tpcf.f90:313-345) with
+# the per-object sines and cosines of latitude hoisted, counted as the flat
+# path is (FMA 2; sqrt, rcp, log and each trigonometric function 1):
+# shear: lon difference 1, cos and sin of it 2, cos d 4, acos + bin 4, sin d 1,
+# 1/sin d 1, cos theta 2, sin theta 4 + 1, cos / sin 2phi 5, projections 6,
+# weights and sums 11 = 42; kappa: lon difference 1, its cos 1, cos d 4,
+# acos + bin 4, weights and sums 8 = 18.
+FLOPS_PER_PAIR_SPH = {0: 42, 1: 18}
 FP64_PEAK_TFLOPS = 78.6             # MI355X published FP64 vector peak
 METRIC = "pairs/sec on 1 MI355X vs host-core ref; FP32+FP64-edge vs FP64 at ≤1e-6 rel"
+# the sources the pair kernel's profile depends on (its code and the tables it walks)
+KERNEL_SOURCES = ("2pcf_amd/csrc/pairs.hip", "2pcf_amd/csrc/prepare.hip", "2pcf_amd/csrc/internal.hpp")
+
+
+def kernel_source_hash() -> str:
+    """sha256 (16 hex digits) over the pair kernel's sources: a profile's
+    traffic figure is reused only for the kernel it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for p in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, p), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def cpu_threads() -> int:
+    """Host threads of the CPU baseline: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS where it is set (a one-GPU box grants its GPU a
+    share of 16 host CPUs and sets OMP_NUM_THREADS to it)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, omp) if omp > 0 else n)
+
+
+def cpu_info() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "cpus_allowed": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def log(*a):
@@ -69,7 +115,7 @@ def cpu_baseline(wl, pairs_per_lens: float, budget_s: float, threads: int) -> di
         t0 = time.perf_counter()
         if use_ref:
             r = oracle.reference(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads,
-                                 spherical=wl.spherical)
+                                 spherical=wl.spherical, quiet=True)
             r["tree_seconds"] = 0.0
         else:
             r = oracle.ref_port(wl.kind, sub, wl.src, wl.x_min, wl.x_max, wl.n_bin, spacing, n_threads=threads)
@@ -91,14 +137,16 @@ def cpu_baseline(wl, pairs_per_lens: float, budget_s: float, threads: int) -> di
     fixed = t_fixed if use_ref else r["tree_seconds"]
     pair_time = total - fixed
     pairs = int(r["n_pairs"].sum())
-    return {"value": pairs / pair_time, "unit": "pairs/s", "cores": threads,
-            "kind": "reference" if use_ref else "port",
-            "sample": f"{n_s} random lenses of {n_a} against all {len(wl.src['x'])} sources; "
-                      f"{pairs} in-bin pairs in {pair_time:.2f} s; kd-tree build over the sources "
-                      f"{fixed:.2f} s reported apart",
-            "tree_build_s": fixed,
-            "code": ("reference tangential_shear_tpcf/scalar_kappa_tpcf (flang -O2 -fopenmp)" if use_ref
-                     else "C port of the reference algorithm (gcc -O2 -fopenmp)")}
+    out = {"value": pairs / pair_time, "unit": "pairs/s", "cores": threads,
+           "kind": "reference" if use_ref else "port",
+           "sample": f"{n_s} random lenses of {n_a} against all {len(wl.src['x'])} sources; "
+                     f"{pairs} in-bin pairs in {pair_time:.2f} s on {threads} threads; kd-tree build over the "
+                     f"sources {fixed:.2f} s reported apart",
+           "tree_build_s": fixed, "pairs": pairs, "seconds": pair_time,
+           "code": ("reference tangential_shear_tpcf/scalar_kappa_tpcf (flang -O2 -fopenmp)" if use_ref
+                    else "C port of the reference algorithm (gcc -O2 -fopenmp)")}
+    out.update(cpu_info())
+    return out
 
 
 def _have_ref() -> bool:
@@ -110,66 +158,71 @@ def _have_ref() -> bool:
 def profiled_traffic(workload: str):
     """HBM bytes per pair-kernel launch measured for this workload by
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (copy-calibrated), as recorded in
-    profiles/k_pairs_<workload>.md; None when no such profile exists."""
+    profiles/k_pairs_<workload>.md, when that profile was taken on the pair
+    kernel built now (its kernel_source_hash line equals kernel_source_hash());
+    else None."""
     path = os.path.join(ROOT, "profiles", f"k_pairs_{workload}.md")
     if not os.path.exists(path):
         return None, None
+    traffic = src_hash = None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
-    return None, None
+            traffic = float(line.split(":")[1])
+        elif line.startswith("kernel_source_hash:"):
+            src_hash = line.split(":")[1].strip()
+    if traffic is None or src_hash != kernel_source_hash():
+        return None, None
+    return traffic, os.path.relpath(path, ROOT)
+
+
+def blended_cpu_baseline(parts: list, budget_s: float, threads: int) -> dict:
+    """CPU baseline of a mixed workload: each part (a workload-like namespace
+    with the pair count the GPU step does for it) timed with cpu_baseline on
+    budget_s / len(parts) seconds, combined as the rate at which the host would
+    do the step's pairs: sum N / sum (N / rate)."""
+    res, t, n = [], 0.0, 0
+    for wl, n_pairs, pairs_per_lens in parts:
+        r = cpu_baseline(wl, pairs_per_lens, budget_s / len(parts), threads)
+        res.append(r)
+        t += n_pairs / r["value"]
+        n += n_pairs
+    out = dict(res[0])
+    out["value"] = n / t
+    out["sample"] = "; ".join(f"{'shear-scalar' if wl.kind == 0 else 'kappa-kappa'} part ({wl.name}): "
+                              f"{r['value']:.3g} pairs/s from {r['sample']}" for (wl, _, _), r in zip(parts, res))
+    out["blend"] = "rate at which the host does the step's pairs: sum N / sum (N / rate of its kind)"
+    for k in ("pairs", "seconds", "tree_build_s"):
+        out.pop(k, None)
+    return out
+
+
+def host_part(name, kind, lens, src, x_min, x_max, n_bin):
+    from types import SimpleNamespace
+    return SimpleNamespace(name=name, kind=kind, lens=lens, src=src, x_min=x_min, x_max=x_max, n_bin=n_bin,
+                           spherical=False)
 
 
 def bench_mocks(args) -> int:
-    """Batched form: 256 mock fields, each a kappa catalogue and a shear
-    catalogue with sizes log-uniform in [5e4, 2e6] on a 600'x600' patch,
-    kappa-kappa (auto) and kappa x gamma_t per field, 20 log bins 1'-100',
-    16 patches (4 x 4 squares) per field; one tpcf_batch of 512 catalogues
-    (graph captured on the first run, replayed after).  value = in-bin pairs
-    per second of one batch run."""
+    """Batched form: 256 mock fields (synth.mock_fields: per field a kappa
+    catalogue and a shear catalogue with sizes log-uniform in [5e4, 2e6] on a
+    600'x600' patch), kappa-kappa (auto) and kappa x gamma_t per field, 20 log
+    bins 1'-100', 16 patches (4 x 4 squares) per field; one tpcf_batch of 512
+    catalogues (graph captured on the first run, replayed after).  value =
+    in-bin pairs per second of one batch run.  The roofline counts 17 FLOPs per
+    kappa-kappa pair and 33 per shear pair over the batch run's wall time
+    (every kernel of the graph, on 8 parallel branches)."""
     import torch
     tp = importlib.import_module("2pcf_amd")
-    dev = torch.device("cuda:0")
-    rng = np.random.default_rng(1004)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(1004)
-    side, n_f = 600.0, int(256 * args.scale) or 1
-    keep, descs = [], []
-
-    def uni(n, lo, hi):
-        return torch.rand(n, generator=gen, device=dev) * (hi - lo) + lo
-
+    synth = importlib.import_module("2pcf_amd.synth")
+    n_f = int(256 * args.scale) or 1
     t0 = time.perf_counter()
-    for f in range(n_f):
-        nk, ng = (int(np.exp(rng.uniform(np.log(5e4), np.log(2e6)))) for _ in range(2))
-        kx, ky = uni(nk, 0, side), uni(nk, 0, side)
-        kap = torch.randn(nk, generator=gen, device=dev) * 0.02
-        kw = uni(nk, 0.5, 1.5)
-        kz = torch.zeros(nk, device=dev)
-        gx, gy = uni(ng, 0, side), uni(ng, 0, side)
-        e1 = torch.randn(ng, generator=gen, device=dev) * 0.27
-        e2 = torch.randn(ng, generator=gen, device=dev) * 0.27
-        gw = uni(ng, 0.5, 1.5)
-        gz = torch.ones(ng, device=dev)
-        kb = torch.randn(nk, generator=gen, device=dev) * 0.02
-        pk = (torch.clamp((kx / 150).int(), 0, 3) + 4 * torch.clamp((ky / 150).int(), 0, 3)).int()
-        pg = (torch.clamp((gx / 150).int(), 0, 3) + 4 * torch.clamp((gy / 150).int(), 0, 3)).int()
-        keep += [kx, ky, kap, kw, kz, gx, gy, e1, e2, gw, gz, kb, pk, pg]
-        lens = dict(x=kx.data_ptr(), y=ky.data_ptr(), z=kz.data_ptr(), val=kap.data_ptr(), w=kw.data_ptr())
-        descs.append(dict(kind=tp.KAPPA, lens=lens, n_A=nk, n_B=nk, patch_A=pk.data_ptr(), patch_B=pk.data_ptr(),
-                          n_patch_A=16, n_patch_B=16,
-                          src=dict(x=kx.data_ptr(), y=ky.data_ptr(), kappa_E=kap.data_ptr(), kappa_B=kb.data_ptr(),
-                                   w=kw.data_ptr())))
-        descs.append(dict(kind=tp.SHEAR, lens=lens, n_A=nk, n_B=ng, patch_A=pk.data_ptr(), patch_B=pg.data_ptr(),
-                          n_patch_A=16, n_patch_B=16,
-                          src=dict(x=gx.data_ptr(), y=gy.data_ptr(), z=gz.data_ptr(), e1=e1.data_ptr(),
-                                   e2=e2.data_ptr(), w=gw.data_ptr())))
+    cats = synth.mock_fields(n_f)
     torch.cuda.synchronize()
-    log(f"{len(descs)} catalogues generated on the device in {time.perf_counter() - t0:.1f} s")
+    log(f"{len(cats)} catalogues generated on the device in {time.perf_counter() - t0:.1f} s")
     bins = tp.create_bins(1.0, 100.0, 20, tp.LOG)
     ctx = tp.Context(0)
     t0 = time.perf_counter()
-    b = tp.Batch(descs, bins, autotune=not args.no_autotune, ctx=ctx)
+    b = tp.Batch([synth.batch_desc(c) for c in cats], bins, autotune=not args.no_autotune, ctx=ctx)
     st = b.stats
     log(f"batch planned in {time.perf_counter() - t0:.1f} s (autotune {st['autotuned']}: cell target "
         f"{st['cell_target']}, split {st['split']}, candidates {[round(t, 1) for t in st['tune_ms']]} ms)")
@@ -178,19 +231,46 @@ def bench_mocks(args) -> int:
         log(f"warmup {i}: {b.stats['ms_run']:.1f} ms ({b.stats['graph_nodes']} graph nodes)")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    pairs = 0
+    pairs, run_ms = 0, []
+    kinds = [c["kind"] for c in cats]
     for i in range(args.steps):
         out = b.run()
         pairs += int(sum(int(r["n_pairs"].sum()) for r in out))
+        run_ms.append(b.stats["ms_run"])
         log(f"step {i}: {b.stats['ms_run']:.1f} ms")
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    kk = int(sum(int(r["n_pairs"].sum()) for r, k in zip(out, kinds) if k == 1))
+    sk = int(sum(int(r["n_pairs"].sum()) for r, k in zip(out, kinds) if k == 0))
+    flops = kk * FLOPS_PER_PAIR[1] + sk * FLOPS_PER_PAIR[0]
+    run_s = float(np.mean(run_ms)) * 1e-3
+    achieved = flops / run_s / 1e12
     res = {"metric": "pairs/sec, batch of mock catalogues on 1 MI355X", "value": pairs / el, "unit": "pairs/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "mocks_256", "fields": n_f, "catalogues": len(descs),
-                      "pairs_per_step": pairs / args.steps, "cell_target": st["cell_target"], "split": st["split"],
-                      "graph_nodes": b.stats["graph_nodes"], "plan_ms": st["ms_plan"]}}
+           "config": {"workload": "mocks_256", "fields": n_f, "catalogues": len(cats),
+                      "pairs_per_step": pairs / args.steps, "pairs_kappa_kappa": kk, "pairs_shear_scalar": sk,
+                      "cell_target": st["cell_target"], "split": st["split"],
+                      "graph_nodes": b.stats["graph_nodes"], "plan_ms": st["ms_plan"]},
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_pairs (graph replay)",
+                        "time_base": "batch run wall time (all kernels of the graph)", "run_ms": run_s * 1e3,
+                        "flops_per_pair": {"kappa-kappa": FLOPS_PER_PAIR[1], "shear-scalar": FLOPS_PER_PAIR[0]}}}
+    if not args.no_cpu:
+        # the largest catalogue of each kind, lenses subsampled, against all its sources
+        parts = []
+        for kind, n_k in ((1, kk), (0, sk)):
+            i = max((j for j in range(len(cats)) if kinds[j] == kind),
+                    key=lambda j: cats[j]["lens"]["x"].numel() * cats[j]["src"]["x"].numel())
+            c = cats[i]
+            lens = {k: v.cpu().numpy() for k, v in c["lens"].items()}
+            src = {k: v.cpu().numpy() for k, v in c["src"].items()}
+            if kind == 0:
+                src["m"] = np.zeros(len(src["x"]), np.float32)
+            ppl = float(out[i]["n_pairs"].sum()) / len(lens["x"])
+            parts.append((host_part(f"mocks_256 catalogue {i}", kind, lens, src, 1.0, 100.0, 20), n_k, ppl))
+        res["cpu_baseline"] = blended_cpu_baseline(parts, args.cpu_seconds, cpu_threads())
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
     print(json.dumps(res), flush=True)
     b.close()
     ctx.close()
@@ -202,10 +282,14 @@ def bench_adjoint(args) -> int:
     1200'x1200' patch, 20 log bins 1'-200': forward xi_kk and gamma_t plus the
     adjoint of chi^2 (diagonal C) with respect to every kappa, e1, e2 and w.
     value = in-bin pairs of the two forward correlations per second of
-    forward + backward."""
+    forward + backward.  Each correlation's pairs are visited twice per step
+    (the lens-side pass of the recorded forward and the source-side pass of
+    the backward), so the roofline counts 2 x (17 x kappa-kappa pairs +
+    33 x shear pairs) FLOPs over the step's wall time."""
     import torch
     tp = importlib.import_module("2pcf_amd")
     ag = importlib.import_module("2pcf_amd.autograd")
+    synth = importlib.import_module("2pcf_amd.synth")
     dev = torch.device("cuda:0")
     n = int(5_000_000 * args.scale)
     side = 1200.0 * np.sqrt(args.scale)
@@ -250,21 +334,101 @@ def bench_adjoint(args) -> int:
                                      w=kw.data_ptr()),
                       dict(x=gx.data_ptr(), y=gy.data_ptr(), z=gz.data_ptr(), e1=e1.data_ptr(), e2=e2.data_ptr(),
                            w=gw.data_ptr(), m=gm.data_ptr()), bins, device_inputs=True, ctx=ctx, n_A=n, n_B=n)
-    fwd_pairs = int(r.n_pairs.sum()) + int(r2.n_pairs.sum())
+    kk, sk = int(r.n_pairs.sum()), int(r2.n_pairs.sum())
+    fwd_pairs = kk + sk
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
         log(f"step {i} done")
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    step_s = el / args.steps
+    flops = 2 * (kk * FLOPS_PER_PAIR[1] + sk * FLOPS_PER_PAIR[0])
+    achieved = flops / step_s / 1e12
     res = {"metric": "pairs/sec of forward + adjoint on 1 MI355X", "value": fwd_pairs * args.steps / el,
            "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": "adjoint_5Mx5M", "n": n, "forward_pairs_per_step": fwd_pairs,
+                      "pairs_kappa_kappa": kk, "pairs_shear_scalar": sk,
                       "passes_per_step": "per correlation: the autograd forward records the lens-side marks "
                                          "pass (its marks give the forward sums), the backward replays it with "
-                                         "the source-side marks pass"}}
+                                         "the source-side marks pass"},
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_pairs (4 marks passes)",
+                        "time_base": "step wall time (forward, chi^2 and backward)",
+                        "flops_per_step": flops,
+                        "flops_per_pair": {"kappa-kappa": FLOPS_PER_PAIR[1], "shear-scalar": FLOPS_PER_PAIR[0],
+                                           "passes_per_pair": 2}}}
+    if not args.no_cpu:
+        # the reference has no derivative: its forward correlations of the same
+        # catalogues, blended like the step's pairs
+        kx_h, ky_h = kx.detach().cpu().numpy(), ky.detach().cpu().numpy()
+        lens = dict(x=kx_h, y=ky_h, val=kap.detach().cpu().numpy(), w=kw.detach().cpu().numpy(),
+                    z=np.zeros(n, np.float32))
+        ksrc = dict(x=kx_h, y=ky_h, kappa_E=lens["val"], kappa_B=kb.cpu().numpy(), w=lens["w"])
+        gsrc = dict(x=gx.cpu().numpy(), y=gy.cpu().numpy(), z=np.ones(n, np.float32), e1=e1.detach().cpu().numpy(),
+                    e2=e2.detach().cpu().numpy(), w=gw.detach().cpu().numpy(), m=np.zeros(n, np.float32))
+        parts = [(host_part("adjoint_5Mx5M kappa-kappa", 1, lens, ksrc, 1.0, 200.0, 20), kk, kk / n),
+                 (host_part("adjoint_5Mx5M shear-scalar", 0, lens, gsrc, 1.0, 200.0, 20), sk, sk / n)]
+        res["cpu_baseline"] = blended_cpu_baseline(parts, args.cpu_seconds, cpu_threads())
+        res["cpu_baseline"]["note"] = "forward correlations only: the reference has no gradient"
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+    print(json.dumps(res), flush=True)
+    ctx.close()
+    return 0
+
+
+def bench_kk100k(args) -> int:
+    """The survey's first configuration: the kappa-kappa auto-correlation of
+    100k points uniform on a 600'x600' patch (FP64, 20 log bins 1'-100', 16
+    k-means patches) and 200 bootstrap resamples of the 16 patches with both
+    covariances.  A step is the correlation call (host arrays in, as the
+    reference's driver holds them) plus block sums and the bootstrap; value =
+    in-bin ordered pairs per second of the step.  The roofline is the pair
+    kernel's (17 FLOPs per pair, FP64 vector peak); the CPU baseline is the
+    reference's correlation on the whole workload."""
+    import torch
+    tp = importlib.import_module("2pcf_amd")
+    synth = importlib.import_module("2pcf_amd.synth")
+    wl = synth.config("kk_100k_auto", args.scale)
+    bins = tp.create_bins(wl.x_min, wl.x_max, wl.n_bin, tp.LOG)
+    ctx = tp.Context(0)
+
+    def step():
+        g = tp.correlate(tp.KAPPA, wl.lens, wl.src, bins, precision="f64", patch_A=wl.patch_A, patch_B=wl.patch_B,
+                         n_patch_A=wl.n_patch, n_patch_B=wl.n_patch, real_bytes=8, ctx=ctx)
+        blocks = tp.block_sums(g, wl.patch_A, wl.lens["w"], wl.n_patch)
+        bt = tp.bootstrap(blocks, 200, seed=1001, ctx=ctx)
+        return g, bt
+
+    for i in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms, pairs = [], 0
+    for i in range(args.steps):
+        g, bt = step()
+        kernel_ms.append(g.stats["ms_pairs"])
+        pairs += int(g.n_pairs.sum())
+        log(f"step {i}: correlation {g.stats['ms_total']:.1f} ms (pairs kernel {g.stats['ms_pairs']:.2f} ms)")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    per_step = pairs / args.steps
+    k_ms = float(np.mean(kernel_ms))
+    achieved = per_step * FLOPS_PER_PAIR[1] / (k_ms * 1e-3) / 1e12
+    res = {"metric": METRIC, "value": pairs / el, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "kk_100k_auto", "n": len(wl.lens["x"]), "n_bin": wl.n_bin,
+                      "theta": [wl.x_min, wl.x_max], "patches": wl.n_patch, "bootstrap_resamples": 200,
+                      "pairs_per_step": per_step, "io": "host arrays in and out"},
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_pairs",
+                        "kernel_ms": k_ms, "flops_per_pair": FLOPS_PER_PAIR[1]}}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(wl, per_step / len(wl.lens["x"]), args.cpu_seconds, cpu_threads())
+        res["speedup_vs_cpu"] = res["value"] / res["cpu_baseline"]["value"]
     print(json.dumps(res), flush=True)
     ctx.close()
     return 0
@@ -340,11 +504,18 @@ def main() -> int:
                          "and D2H of every output inside each step)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (reference kernel for HBM counters)")
+    ap.add_argument("--source-hash", action="store_true",
+                    help="print the pair kernel's source hash (recorded in the profiles) and exit")
     args = ap.parse_args()
+    if args.source_hash:
+        print(kernel_source_hash())
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the pair-counting path runs on a single MI355X; "
               "sharding over several GPUs is not implemented", file=sys.stderr)
         return 2
+    if args.config == "kk_100k_auto":
+        return bench_kk100k(args)
     if args.config == "mocks_256":
         return bench_mocks(args)
     if args.config == "adjoint_5Mx5M":
@@ -426,16 +597,20 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": fpp},
+                     "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": fpp,
+                     "kernel_source_hash": kernel_source_hash()},
     }
-    if args.scale == 1.0 and args.cell == 0.0:
+    if args.scale == 1.0 and args.cell == 0.0 and not args.marks:
         traffic, src = profiled_traffic(wl.name)
         if traffic is not None:
             out["roofline"]["traffic"] = traffic
-            out["roofline"]["traffic_source"] = f"{src} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
+            out["roofline"]["traffic_source"] = (f"{src} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated; "
+                                                 f"profiled on the kernel of this source hash)")
+        else:
+            out["roofline"]["traffic_source"] = "no profile of this workload on this pair kernel's sources"
     # the C port has no spherical mode: the spherical CPU baseline needs the reference build
     if not args.no_cpu and (not wl.spherical or _have_ref()):
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         ppl = per_step_pairs / max(stats["n_lens_used"], 1)
         out["cpu_baseline"] = cpu_baseline(wl, ppl, args.cpu_seconds, threads)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]

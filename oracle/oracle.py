@@ -320,6 +320,7 @@ def ref_lib():
         L.ref_covariance_c.argtypes = [ci, ci, _fp, _fp]
         L.ref_kmeans_c.argtypes = [ci, _fp, _fp, _fp, ci, ci, cf, ci, cf, cf, _i32p, _fp]
         L.ref_bins_c.argtypes = [cf, cf, ci, ci, cf, ci, _fp, _fp, _fp, _i32p]
+        L.ref_flush_c.argtypes = []
         _ref_lib = L
     return _ref_lib
 
@@ -347,7 +348,7 @@ def ref_sph_pair(lon1, lat1, lon2, lat2):
 
 
 def reference(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ihs_theta=1.0,
-              n_threads: int = 1, use_tree: bool = True, spherical: bool = False) -> dict:
+              n_threads: int = 1, use_tree: bool = True, spherical: bool = False, quiet: bool = False) -> dict:
     """Runs the reference's tangential_shear_tpcf (kind 0) or scalar_kappa_tpcf
     (kind 1), src/tpcf.f90:190-545, on FP32 copies of the inputs.  Returns its
     outputs: mean_theta, g_t, g_x, weights, n_pairs and the per-lens marks
@@ -372,16 +373,33 @@ def reference(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ih
         [_ptr(res[k], _fp) for k in ("xi_e", "xi_b", "k_marks", "w_marks", "marks_weight")]
     common = [float(x_min), float(x_max), int(n_bin), int(spacing), float(ihs_theta), int(n_threads),
               1 if use_tree else 0]
-    t0 = time.perf_counter()
-    if kind == SHEAR:
-        L.ref_shear_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "z", "val", "w")), n_b,
-                      *(f(src, k, n_b) for k in ("x", "y", "z", "e1", "e2", "w", "m")), *common, *outs,
-                      1 if spherical else 0)
-    else:
-        L.ref_kappa_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "val", "w")), n_b,
-                      *(f(src, k, n_b) for k in ("x", "y", "kappa_E", "kappa_B", "w")), *common, *outs,
-                      1 if spherical else 0)
-    res["seconds"] = time.perf_counter() - t0
+    # quiet: the reference's messages (find_bin_index prints for every pair
+    # FP32 rounding puts past x_max) go to /dev/null, not into the caller's
+    # standard output
+    saved = None
+    if quiet:
+        import sys
+        sys.stdout.flush()
+        saved = os.dup(1)
+        null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(null, 1)
+        os.close(null)
+    try:
+        t0 = time.perf_counter()
+        if kind == SHEAR:
+            L.ref_shear_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "z", "val", "w")), n_b,
+                          *(f(src, k, n_b) for k in ("x", "y", "z", "e1", "e2", "w", "m")), *common, *outs,
+                          1 if spherical else 0)
+        else:
+            L.ref_kappa_c(n_a, *(f(lens, k, n_a) for k in ("x", "y", "val", "w")), n_b,
+                          *(f(src, k, n_b) for k in ("x", "y", "kappa_E", "kappa_B", "w")), *common, *outs,
+                          1 if spherical else 0)
+        res["seconds"] = time.perf_counter() - t0
+    finally:
+        if saved is not None:
+            L.ref_flush_c()
+            os.dup2(saved, 1)
+            os.close(saved)
     return res
 
 

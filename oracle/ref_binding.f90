@@ -122,4 +122,12 @@ contains
         end do
     end subroutine ref_sph_pair_c
 
+    ! Writes out what the Fortran runtime holds for standard output (the
+    ! reference's find_bin_index prints when FP32 rounding puts a distance
+    ! past the last bin, src/utils.f90:132-136), so a caller that points
+    ! standard output elsewhere around a call can restore it afterwards.
+    subroutine ref_flush_c() bind(C, name="ref_flush_c")
+        flush(6)
+    end subroutine ref_flush_c
+
 end module ref_binding
