@@ -12,7 +12,8 @@ import os
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "lib2pcf_amd.so")
 
-MAX_BIN = 63
+MAX_BIN = 1024
+GROUP_BINS = 63                              # bins per pair-kernel pass (tpcf_adjoint_*: the limit)
 LIN, LOG, IHS, SQRT = 0, 1, 2, 3            # src/utils.f90:7
 SHEAR, KAPPA = 0, 1
 

@@ -41,7 +41,7 @@ void fill_f64(hipStream_t s, double *p, int64_t n);
 namespace {
 
 struct Coef {
-    double at[TPCF_MAX_BIN], ax[TPCF_MAX_BIN], be[TPCF_MAX_BIN];
+    double at[kGroupBins], ax[kGroupBins], be[kGroupBins];
 };
 
 template <typename T>
@@ -298,6 +298,8 @@ int adj_check(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
               const T *wb, const T *mb, const tpcf_bins *bins, uint32_t flags)
 {
     TPCF_TRY(check_bins(ctx, bins));
+    if (bins->n_bin > kGroupBins)
+        return fail(ctx, TPCF_ERR_UNSUPPORTED, "the adjoint takes at most " + std::to_string(kGroupBins) + " bins");
     if (flags & TPCF_SPHERICAL)
         return fail(ctx, TPCF_ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build");
     if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE))

@@ -139,39 +139,42 @@ struct MarksOut {
 };
 
 // The marks planes [n_bin][4][n_lens] (sorted lens order) to the caller's
-// rows [n_A][n_bin] per component, through an LDS tile of 64 lenses x n_bin
-// bins: coalesced reads along the lenses, each lens' row written by
-// consecutive threads (the rows themselves land in the caller's order, i.e.
-// scattered).  marks_weight is the row sum of the W plane.
+// rows [n_A][n_bin] per component, through an LDS tile of 64 lenses x up to
+// kMarksBins bins at a time: coalesced reads along the lenses, each lens' row
+// written by consecutive threads (the rows themselves land in the caller's
+// order, i.e. scattered).  marks_weight is the row sum of the W plane.
 constexpr int kMarksTile = 64;
+constexpr int kMarksBins = 32;
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_marks_out(int64_t n_lens, int32_t n_bin, const uint32_t *__restrict__ lidx,
                                                    const double *__restrict__ mk, MarksOut<R> out)
 {
-    __shared__ double tile[TPCF_MAX_BIN][kMarksTile + 1];
+    __shared__ double tile[kMarksBins][kMarksTile + 1];
     const int64_t i0 = (int64_t)blockIdx.x * kMarksTile;
     const int nl = (int)min<int64_t>(kMarksTile, n_lens - i0);
+    double wsum = 0.0;   // marks_weight of lens i0 + threadIdx.x
     for (int q = 0; q < 4; ++q) {
         const bool want_w = (q == 3 && out.q[4] != nullptr);
         if (!out.q[q] && !want_w) continue;   // the same for the whole block
-        for (int e = threadIdx.x; e < n_bin * kMarksTile; e += blockDim.x) {
-            const int b = e / kMarksTile, i = e % kMarksTile;
-            tile[b][i] = (i < nl) ? mk[mark_at(n_lens, b, q, i0 + i)] : 0.0;
-        }
-        __syncthreads();
-        if (out.q[q])
-            for (int e = threadIdx.x; e < nl * n_bin; e += blockDim.x) {
-                const int i = e / n_bin, b = e - i * n_bin;
-                out.q[q][(size_t)lidx[i0 + i] * n_bin + b] = (R)tile[b][i];
+        for (int b0 = 0; b0 < n_bin; b0 += kMarksBins) {
+            const int nb = min(kMarksBins, n_bin - b0);
+            for (int e = threadIdx.x; e < nb * kMarksTile; e += blockDim.x) {
+                const int b = e / kMarksTile, i = e % kMarksTile;
+                tile[b][i] = (i < nl) ? mk[mark_at(n_lens, b0 + b, q, i0 + i)] : 0.0;
             }
-        if (want_w && (int)threadIdx.x < nl) {
-            double s = 0.0;
-            for (int b = 0; b < n_bin; ++b) s += tile[b][threadIdx.x];
-            out.q[4][lidx[i0 + threadIdx.x]] = (R)s;
+            __syncthreads();
+            if (out.q[q])
+                for (int e = threadIdx.x; e < nl * nb; e += blockDim.x) {
+                    const int i = e / nb, b = e - i * nb;
+                    out.q[q][(size_t)lidx[i0 + i] * n_bin + b0 + b] = (R)tile[b][i];
+                }
+            if (want_w && (int)threadIdx.x < nl)
+                for (int b = 0; b < nb; ++b) wsum += tile[b][threadIdx.x];
+            __syncthreads();
         }
-        __syncthreads();
     }
+    if (out.q[4] && (int)threadIdx.x < nl) out.q[4][lidx[i0 + threadIdx.x]] = (R)wsum;
 }
 
 // The K plane of every bin (blockIdx.y) = its W plane.

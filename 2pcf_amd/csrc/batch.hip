@@ -67,6 +67,17 @@ struct BatchCat {
     int mrb = 4;
 };
 
+// A bin group's pair arguments (bins, edges, tests) with one catalogue's work
+// items: plan_cat fills only the item count and layout of the catalogue's A.
+template <typename U>
+PairArgs<U> with_items(const PairArgs<U> &group, const PairArgs<U> &planned)
+{
+    PairArgs<U> A = group;
+    A.n_items = planned.n_items;
+    A.per_xcd = planned.per_xcd;
+    return A;
+}
+
 int sub_ctx(tpcf_ctx *parent, tpcf_ctx **out)
 {
     tpcf_ctx *c = new (std::nothrow) tpcf_ctx();
@@ -108,14 +119,34 @@ struct Batch final : tpcf_batch {
     tpcf_bins kb{};            // the bins the pair kernel compares with (chord edges on the sphere)
     bool sph = false;
     std::vector<std::unique_ptr<BatchCat<U>>> cats;
-    const U *e2lane = nullptr;
-    PairArgs<U> A0{};
+    // per bin group (kGroupBins bins per pair-kernel pass): the pair arguments
+    // and the squared edges one per lane (the batch's own copies)
+    std::vector<PairArgs<U>> A0;
+    std::vector<const U *> e2lane;
     hipStream_t br[kBranches] = {};
     hipEvent_t fork = nullptr, join[kBranches] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool use_graph = true;
-    void *e2own = nullptr;   // the batch's own copy of the squared edges
+    void *e2own = nullptr;   // the batch's own copy of the squared edges of every group
+
+    // The pair arguments and edge tables of every bin group, in the batch's own memory.
+    int make_groups()
+    {
+        const int ng = bin_groups(kb.n_bin);
+        A0.resize(ng);
+        e2lane.resize(ng);
+        TPCF_HIP(ctx, hipMalloc(&e2own, sizeof(U) * kWave * ng));
+        for (int g = 0; g < ng; ++g) {
+            const U *tmp;
+            TPCF_TRY(make_pair_args<U>(ctx, &kb, g, A0[g], &tmp));
+            U *dst = static_cast<U *>(e2own) + (size_t)g * kWave;
+            TPCF_HIP(ctx, hipMemcpyAsync(dst, tmp, sizeof(U) * kWave, hipMemcpyDeviceToDevice, ctx->stream));
+            e2lane[g] = dst;
+        }
+        TPCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return TPCF_OK;
+    }
 
     ~Batch() override
     {
@@ -160,8 +191,13 @@ struct Batch final : tpcf_batch {
         fill_f64(s, c.pp, kNQ * kSlots * c.pp_total);
         fill_u64(s, c.ppn, kSlots * c.pp_total);
         if (c.marks) fill_f64(s, c.mk, 4 * c.p.n_lens * n_bin);
-        const CatArgs<U> C = cat_args<U>(c.p, c.args.n_pa, c.args.n_pb, c.pp, c.ppn, c.marks ? c.mk : nullptr);
-        TPCF_TRY(launch_pairs<U>(sc, s, c.args.kind, c.p.m_zero, c.marks, c.A, C, e2lane, c.p.items, nullptr));
+        for (size_t g = 0; g < A0.size(); ++g) {
+            const size_t g0 = g * kGroupBins;
+            const CatArgs<U> C = cat_args<U>(c.p, c.args.n_pa, c.args.n_pb, c.pp + g0 * kNQ, c.ppn + g0,
+                                             c.marks ? c.mk + g0 * 4 * (size_t)c.p.n_lens : nullptr);
+            TPCF_TRY(launch_pairs<U>(sc, s, c.args.kind, c.p.m_zero, c.marks, with_items(A0[g], c.A), C, e2lane[g],
+                                     c.p.items, nullptr));
+        }
         if (c.marks)
             TPCF_TRY(launch_marks_out(sc, s, c.p.n_lens, n_bin, c.args.n_a, c.p.lidx, c.mk, c.mout, c.mrb,
                                       c.args.kind == KAPPA || c.p.m_zero));
@@ -393,17 +429,6 @@ int plan_cat(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_bins *kb, bool sph
     return TPCF_OK;
 }
 
-// The batch's pair arguments (bins, edges, tests) with one catalogue's work
-// items: plan_cat fills only the item count and layout.
-template <typename U>
-PairArgs<U> with_items(const PairArgs<U> &A0, const PairArgs<U> &planned)
-{
-    PairArgs<U> A = A0;
-    A.n_items = planned.n_items;
-    A.per_xcd = planned.per_xcd;
-    return A;
-}
-
 // Times one eager run of the pair work of the given catalogues (ms), without
 // their marks.
 template <typename T, typename U>
@@ -417,13 +442,12 @@ int time_plan(tpcf_ctx *ctx, const tpcf_bins *bins, const tpcf_bins *kb, bool sp
     b.kb = *kb;
     b.sph = sph;
     b.use_graph = false;
-    TPCF_TRY(make_pair_args<U>(ctx, &b.kb, b.A0, &b.e2lane));
+    TPCF_TRY(b.make_groups());
     for (const tpcf_catalog *d : sample) {
         auto c = std::make_unique<BatchCat<U>>();
         tpcf_catalog dd = *d;
         dd.out = nullptr;
         TPCF_TRY((plan_cat<T, U>(ctx, &b.bins, &b.kb, sph, dd, *c, true)));
-        c->A = with_items(b.A0, c->A);
         b.cats.push_back(std::move(c));
     }
     TPCF_HIP(ctx, hipEventCreate(&b.fork));
@@ -507,23 +531,18 @@ int batch_build(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const tp
     }
     b->stats.cell_target = ctx->opt_cell_target;
     b->stats.split = ctx->opt_split;
-    int rc = make_pair_args<U>(ctx, &b->kb, b->A0, &b->e2lane);
+    // pair arguments and a private copy of the squared edges per bin group
+    // (later calls on ctx reuse its own edge table)
+    int rc = b->make_groups();
     for (int i = 0; i < n_cat && !rc; ++i) {
         auto c = std::make_unique<BatchCat<U>>();
         rc = plan_cat<T, U>(ctx, &b->bins, &b->kb, sph, cats[i], *c, true);
-        if (!rc) {
-            c->A = with_items(b->A0, c->A);
-            b->stats.n_items += c->p.n_items;
-        }
+        if (!rc) b->stats.n_items += c->p.n_items;
         b->cats.push_back(std::move(c));
     }
     ctx->opt_cell_target = saved_target;
     ctx->opt_split = saved_split;
     if (rc) return rc;
-    // a private copy of the squared edges: later calls on ctx reuse its table
-    TPCF_HIP(ctx, hipMalloc(&b->e2own, sizeof(U) * kWave));
-    TPCF_HIP(ctx, hipMemcpyAsync(b->e2own, b->e2lane, sizeof(U) * kWave, hipMemcpyDeviceToDevice, ctx->stream));
-    b->e2lane = static_cast<const U *>(b->e2own);
     TPCF_HIP(ctx, hipEventCreate(&b->fork));
     for (int i = 0; i < kBranches; ++i) {
         TPCF_HIP(ctx, hipStreamCreate(&b->br[i]));

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_boot_draw(int32_t n_block, int64_t n_dr
     }
 }
 
-// one wave per sample, lane b = bin b
+// one wave per sample, lane b = bin b (bins b, b + 64, ... when there are more)
 __global__ __launch_bounds__(64) void k_boot_sum(int32_t n_block, int32_t n_bin, const int32_t *__restrict__ m,
                                                  const double *__restrict__ gt, const double *__restrict__ gx,
                                                  const double *__restrict__ k, const double *__restrict__ w,
@@ -62,26 +62,28 @@ __global__ __launch_bounds__(64) void k_boot_sum(int32_t n_block, int32_t n_bin,
                                                  double *__restrict__ ow)
 {
     const int64_t i = blockIdx.x;
-    const int b = threadIdx.x;
     const int32_t *row = m + i * (int64_t)n_block;
-    const bool on = b < n_bin;
-    double st = 0.0, sx = 0.0, sk = 0.0, sw = 0.0;
-    for (int32_t p = 0; p < n_block; ++p) {
-        const double mp = (double)row[p];
-        if (mp == 0.0) continue;
-        if (on) {
-            const size_t o = (size_t)p * n_bin + b;
-            st = fma(mp, gt[o], st);
-            sx = fma(mp, gx[o], sx);
-            sk = fma(mp, k[o], sk);
+    for (int b0 = 0; b0 < n_bin; b0 += 64) {
+        const int b = b0 + (int)threadIdx.x;
+        const bool on = b < n_bin;
+        double st = 0.0, sx = 0.0, sk = 0.0, sw = 0.0;
+        for (int32_t p = 0; p < n_block; ++p) {
+            const double mp = (double)row[p];
+            if (mp == 0.0) continue;
+            if (on) {
+                const size_t o = (size_t)p * n_bin + b;
+                st = fma(mp, gt[o], st);
+                sx = fma(mp, gx[o], sx);
+                sk = fma(mp, k[o], sk);
+            }
+            if (w) sw = fma(mp, w[p], sw);
         }
-        if (w) sw = fma(mp, w[p], sw);
+        if (on) {
+            ot[i * n_bin + b] = sk > 0.0 ? st / sk : 0.0;   // src/tpcf.f90:1183-1189
+            ox[i * n_bin + b] = sk > 0.0 ? sx / sk : 0.0;
+        }
+        if (b0 == 0 && threadIdx.x == 0 && ow) ow[i] = sw;
     }
-    if (on) {
-        ot[i * n_bin + b] = sk > 0.0 ? st / sk : 0.0;   // src/tpcf.f90:1183-1189
-        ox[i * n_bin + b] = sk > 0.0 ? sx / sk : 0.0;
-    }
-    if (b == 0 && ow) ow[i] = sw;
 }
 
 // column means of x [n][d]
@@ -120,7 +122,8 @@ extern "C" int tpcf_bootstrap(tpcf_ctx *ctx, const tpcf_blocks *blocks, int32_t 
     ctx->err.clear();
     if (!blocks || !blocks->g_t || !blocks->g_x || !blocks->k) return fail(ctx, TPCF_ERR_ARG, "blocks is incomplete");
     const int32_t P = blocks->n_block, nb = blocks->n_bin;
-    if (P < 1 || nb < 1 || nb > TPCF_MAX_BIN) return fail(ctx, TPCF_ERR_ARG, "need n_block >= 1 and 1 <= n_bin <= 64");
+    if (P < 1 || nb < 1 || nb > TPCF_MAX_BIN)
+        return fail(ctx, TPCF_ERR_ARG, "need n_block >= 1 and 1 <= n_bin <= " + std::to_string(TPCF_MAX_BIN));
     if (n_boot < 1) return fail(ctx, TPCF_ERR_ARG, "n_boot must be >= 1");
     if (!g_t || !g_x) return fail(ctx, TPCF_ERR_ARG, "g_t and g_x outputs are required");
     if (flags & ~(uint32_t)TPCF_DEVICE_INPUTS) return fail(ctx, TPCF_ERR_ARG, "unknown flag");
@@ -159,11 +162,11 @@ extern "C" int tpcf_bootstrap(tpcf_ctx *ctx, const tpcf_blocks *blocks, int32_t 
     hipLaunchKernelGGL(k_boot_sum, dim3((unsigned)n_boot), dim3(64), 0, s, P, nb, dm, dgt, dgx, dk, dw, ot, ox, ow);
     const unsigned cb = (unsigned)((nb * nb + 255) / 256);
     if (cov_t) {
-        hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(64), 0, s, n_boot, nb, ot, mean);
+        hipLaunchKernelGGL(k_col_mean, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, n_boot, nb, ot, mean);
         hipLaunchKernelGGL(k_cov, dim3(cb), dim3(256), 0, s, n_boot, nb, ot, mean, ct);
     }
     if (cov_x) {
-        hipLaunchKernelGGL(k_col_mean, dim3(1), dim3(64), 0, s, n_boot, nb, ox, mean + 0);
+        hipLaunchKernelGGL(k_col_mean, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, n_boot, nb, ox, mean + 0);
         hipLaunchKernelGGL(k_cov, dim3(cb), dim3(256), 0, s, n_boot, nb, ox, mean + 0, cx);
     }
     TPCF_HIP(ctx, hipGetLastError());

@@ -20,6 +20,12 @@ constexpr int kSlots = 8;          // per-XCD copies of the patch-pair accumulat
 constexpr int kNQ = 5;             // accumulated FP64 quantities: Theta, G1, G2, W, K
 constexpr uint32_t kSegMax = 256;  // longest source segment (bounds per-segment FP32 sums)
 constexpr int kRecPad = 4;         // readable records past the last source (prefetch slack)
+// Bins one pair-kernel pass handles: one squared edge per wave lane (edges
+// 0..63) and one bin per lane in the per-bin registers.  A call with more
+// bins runs the pass once per group of up to kGroupBins consecutive bins.
+constexpr int kGroupBins = 63;
+
+inline int bin_groups(int n_bin) { return (n_bin + kGroupBins - 1) / kGroupBins; }
 
 enum Kind { SHEAR = 0, KAPPA = 1 };
 
@@ -307,13 +313,14 @@ template <typename T>
 struct PairArgs {
     int64_t n_items;
     int64_t per_xcd;       // blocks per XCD of the remapped grid
-    int32_t n_bin, counters;
-    double x_max;          // outer edge (FP64)
+    int32_t n_bin, counters;   // n_bin: the bins of this pass (a group of the call's bins)
+    int32_t pp_stride;     // bins per row of the accumulators (all bins of the call)
+    double x_max;          // outer edge of this pass (FP64)
     int32_t xmin_zero;     // x_min == 0: pairs at zero separation fall into bin 0
     T cls_lo, cls_hi;      // classification margins on squared box distances
     float band;            // relative ambiguity band on d^2 (FP32 path)
-    float e2f[TPCF_MAX_BIN + 1];   // squared edges rounded to FP32 (+inf past n_bin)
-    double e2d[TPCF_MAX_BIN + 1];  // squared edges in FP64; e2d[n_bin] = next double above x_max^2
+    float e2f[kGroupBins + 1];   // squared edges rounded to FP32 (+inf past n_bin)
+    double e2d[kGroupBins + 1];  // squared edges in FP64; for the last group e2d[n_bin] = next double above x_max^2
 };
 
 // One catalogue's prepared state, as the pair kernels read it.
@@ -348,7 +355,7 @@ struct CatArgs {
 
 
 template <typename T>
-int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, PairArgs<T> &A, const T **e2lane_out);
+int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, int group, PairArgs<T> &A, const T **e2lane_out);
 template <typename T>
 CatArgs<T> cat_args(const Prepared<T> &p, int32_t n_pa, int32_t n_pb, double *pp, unsigned long long *ppn,
                     double *marks);

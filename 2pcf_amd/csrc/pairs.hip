@@ -20,6 +20,7 @@
 // (DPP) into FP64 per-bin registers (lane b holds bin b), which go to the
 // per-patch-pair FP64 accumulators with global atomics when the source patch
 // changes.
+#include <algorithm>
 #include <cmath>
 
 #include "internal.hpp"
@@ -832,9 +833,10 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
             }
         }
         win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
-        // hand the patch's per-bin sums to the patch-pair accumulators
+        // hand the patch's per-bin sums to the patch-pair accumulators (rows
+        // of all the call's bins; pp and ppn point at this pass's first bin)
         if (lane < n_bin && w.tn > 0) {
-            const size_t o = (((size_t)slot * C.n_pa + ch.pa) * C.n_pb + pb) * n_bin + lane;
+            const size_t o = (((size_t)slot * C.n_pa + ch.pa) * C.n_pb + pb) * A.pp_stride + lane;
             unsafeAtomicAdd(&C.pp[o * kNQ + 0], w.t[QT]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 1], w.t[QG1]);
             unsafeAtomicAdd(&C.pp[o * kNQ + 2], w.t[QG2]);
@@ -917,15 +919,27 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     pair_item<T, KIND, MARKS, MZ, SPH>(A, C, items[it], slot, lane, s_acc[wv], e2lane[lane], probe);
 }
 
+// The pair kernel's arguments for bin group `group` of bins: bins
+// [g0, g1) with g0 = group kGroupBins, at most kGroupBins of them.  The
+// group's edges are the call's edges g0..g1; d == x_max belongs to the last
+// bin of the last group, and a pair at an inner group's top edge to the next
+// group (both groups test that edge alike).  The squared edges one lane each
+// go to slot `group` of the context's edge table (sized for every group of
+// the call, so no later group reallocates it while a pass may run).
 template <typename T>
-int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, PairArgs<T> &A, const T **e2lane_out)
+int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, int group, PairArgs<T> &A, const T **e2lane_out)
 {
     std::memset(&A, 0, sizeof A);
-    A.n_bin = bins->n_bin;
+    const int n_all = bins->n_bin, n_groups = bin_groups(n_all);
+    const int g0 = group * kGroupBins, g1 = std::min(n_all, g0 + kGroupBins);
+    if (group < 0 || g0 >= n_all) return fail(ctx, TPCF_ERR_ARG, "bin group out of range");
+    A.n_bin = g1 - g0;
+    A.pp_stride = n_all;
     A.counters = ctx->opt_counters ? 1 : 0;
     const int n = A.n_bin;
-    A.x_max = bins->edges[n];
-    A.xmin_zero = (bins->edges[0] == 0.0) ? 1 : 0;
+    const double *edges = bins->edges + g0;
+    A.x_max = edges[n];
+    A.xmin_zero = (g0 == 0 && edges[0] == 0.0) ? 1 : 0;
     if (sizeof(T) == 4) {
         A.cls_lo = (T)(1.0 - 2e-6);
         A.cls_hi = (T)(1.0 + 2e-6);
@@ -934,9 +948,9 @@ int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, PairArgs<T> &A, const T
         A.cls_hi = (T)(1.0 + 1e-12);
     }
     A.band = 1e-5f;
-    for (int i = 0; i <= TPCF_MAX_BIN; ++i) {
+    for (int i = 0; i <= kGroupBins; ++i) {
         if (i <= n) {
-            const double e = bins->edges[i];
+            const double e = edges[i];
             A.e2d[i] = e * e;
             A.e2f[i] = (float)(e * e);
         } else {
@@ -944,9 +958,10 @@ int make_pair_args(tpcf_ctx *ctx, const tpcf_bins *bins, PairArgs<T> &A, const T
             A.e2f[i] = INFINITY;
         }
     }
-    A.e2d[n] = std::nextafter(A.e2d[n], INFINITY);   // d == x_max belongs to the last bin
+    if (g1 == n_all) A.e2d[n] = std::nextafter(A.e2d[n], INFINITY);   // d == x_max belongs to the last bin
     T *e2lane;
-    TPCF_TRY(ensure(ctx, tpcf_ctx::B_E2, sizeof(T) * kWave, (void **)&e2lane));
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_E2, sizeof(T) * kWave * n_groups, (void **)&e2lane));
+    e2lane += (size_t)group * kWave;
     T h[kWave];
     for (int i = 0; i < kWave; ++i) h[i] = (i <= n) ? (sizeof(T) == 4 ? (T)A.e2f[i] : (T)A.e2d[i]) : (T)INFINITY;
     TPCF_HIP(ctx, hipMemcpyAsync(e2lane, h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
@@ -987,21 +1002,28 @@ template <typename T>
 int run_pairs(tpcf_ctx *ctx, const CallArgs<T> &a, const Prepared<T> &p, double *pp, unsigned long long *ppn,
               double *marks)
 {
-    PairArgs<T> A;
-    const T *e2lane;
-    TPCF_TRY(make_pair_args<T>(ctx, a.kbins ? a.kbins : a.bins, A, &e2lane));
-    A.n_items = p.n_items;
-    const int64_t nblk = (p.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
-    A.per_xcd = (nblk + kSlots - 1) / kSlots;
-    const CatArgs<T> C = cat_args<T>(p, a.n_pa, a.n_pb, pp, ppn, marks);
+    const tpcf_bins *kb = a.kbins ? a.kbins : a.bins;
     unsigned long long *probe = nullptr;
-    if (A.counters) {
+    if (ctx->opt_counters) {
         TPCF_TRY(ensure(ctx, tpcf_ctx::B_PROBE, sizeof(unsigned long long) * 8, (void **)&probe));
         fill_u64(ctx->stream, probe, 8);
     }
     if (p.n_items == 0) return TPCF_OK;
-    TPCF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    TPCF_TRY(launch_pairs<T>(ctx, ctx->stream, a.kind, p.m_zero, a.marks, A, C, e2lane, p.items, probe));
+    const int64_t nblk = (p.n_items + kWavesPerBlock - 1) / kWavesPerBlock;
+    // one pass per group of up to kGroupBins bins, each into its bins' columns
+    // of the accumulators and marks planes
+    for (int g = 0; g < bin_groups(kb->n_bin); ++g) {
+        PairArgs<T> A;
+        const T *e2lane;
+        TPCF_TRY(make_pair_args<T>(ctx, kb, g, A, &e2lane));
+        A.n_items = p.n_items;
+        A.per_xcd = (nblk + kSlots - 1) / kSlots;
+        const size_t g0 = (size_t)g * kGroupBins;
+        const CatArgs<T> C = cat_args<T>(p, a.n_pa, a.n_pb, pp + g0 * kNQ, ppn + g0,
+                                         marks ? marks + g0 * 4 * (size_t)p.n_lens : nullptr);
+        if (g == 0) TPCF_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        TPCF_TRY(launch_pairs<T>(ctx, ctx->stream, a.kind, p.m_zero, a.marks, A, C, e2lane, p.items, probe));
+    }
     TPCF_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     return TPCF_OK;
 }
@@ -1047,8 +1069,8 @@ int launch_pairs(tpcf_ctx *ctx, hipStream_t s, int kind, bool m_zero, bool marks
     return TPCF_OK;
 }
 
-template int make_pair_args<float>(tpcf_ctx *, const tpcf_bins *, PairArgs<float> &, const float **);
-template int make_pair_args<double>(tpcf_ctx *, const tpcf_bins *, PairArgs<double> &, const double **);
+template int make_pair_args<float>(tpcf_ctx *, const tpcf_bins *, int, PairArgs<float> &, const float **);
+template int make_pair_args<double>(tpcf_ctx *, const tpcf_bins *, int, PairArgs<double> &, const double **);
 template CatArgs<float> cat_args<float>(const Prepared<float> &, int32_t, int32_t, double *, unsigned long long *,
                                         double *);
 template CatArgs<double> cat_args<double>(const Prepared<double> &, int32_t, int32_t, double *, unsigned long long *,

@@ -22,7 +22,7 @@ module tpcf_gpu_c
     use, intrinsic :: iso_c_binding
     implicit none
 
-    integer(c_int32_t), parameter :: TPCF_MAX_BIN = 63
+    integer(c_int32_t), parameter :: TPCF_MAX_BIN = 1024
     integer(c_int32_t), parameter :: TPCF_LIN = 0, TPCF_LOG = 1, TPCF_IHS = 2, TPCF_SQRT = 3
     integer(c_int32_t), parameter :: TPCF_OK = 0, TPCF_ERR_ARG = 1, TPCF_ERR_BINS = 2, TPCF_ERR_ALLOC = 3, &
                                      TPCF_ERR_HIP = 4, TPCF_ERR_UNSUPPORTED = 5, TPCF_ERR_NONFINITE = 6, &

@@ -31,8 +31,11 @@
 extern "C" {
 #endif
 
-/* Largest n_bin the kernels support (one bin edge per wavefront lane). */
-#define TPCF_MAX_BIN 63
+/* Largest n_bin (the reference takes any --n-bin, src/main.f90:354-357).  One
+ * pass of the pair kernel handles up to 63 consecutive bins (one squared edge
+ * per wavefront lane); a call with more bins runs one pass per group of 63.
+ * tpcf_adjoint_* take at most 63 bins. */
+#define TPCF_MAX_BIN 1024
 
 /* Bin spacing enum of src/utils.f90:7. */
 enum tpcf_spacing { TPCF_LIN = 0, TPCF_LOG = 1, TPCF_IHS = 2, TPCF_SQRT = 3 };

@@ -41,11 +41,28 @@ def test_library_exports_every_declared_symbol(tp):
 def test_struct_layout_matches_header(tp):
     from importlib import import_module
     N = import_module("2pcf_amd._native")
-    # tpcf_bins: 6 scalars (24 B) + 64 doubles + 63 floats, 8-byte aligned
-    assert C.sizeof(N.Bins) == 24 + 64 * 8 + 63 * 4 + 4
+    # tpcf_bins: 6 scalars (24 B) + (TPCF_MAX_BIN + 1) doubles + TPCF_MAX_BIN floats, 8-byte aligned
+    assert N.MAX_BIN == 1024
+    assert C.sizeof(N.Bins) == 24 + 1025 * 8 + 1024 * 4
+    m = re.search(r"#define TPCF_MAX_BIN (\d+)", open(os.path.join(ROOT, "include", "tpcf_gpu.h")).read())
+    assert int(m.group(1)) == N.MAX_BIN
     assert N.Bins.edges.offset == 24
     assert C.sizeof(N.Out) == 8 + 22 * 8
     assert C.sizeof(N.Stats) == 5 * 8 + 5 * 8 + 8 + 3 * 8 + 3 * 8 + 8
+
+
+@pytest.mark.parametrize("spacing, n_bin", [(1, 1000), (0, 200), (2, 300), (3, 1024)])
+def test_many_bins_edges_equal_oracle_edges(tp, orc, spacing, n_bin):
+    """More bins than one pair-kernel pass takes (the reference takes any
+    n_bin): the edges and centres are the oracle's closed form."""
+    b = tp.create_bins(0.5, 250.0, n_bin, spacing, 2.0)
+    assert b.n_bin == n_bin and len(b.bin_edges) == n_bin + 1
+    np.testing.assert_array_equal(b.bin_edges, orc.edges(spacing, 0.5, 250.0, n_bin, 2.0))
+    np.testing.assert_allclose(b.bin_centers, (b.bin_edges[:-1] + b.bin_edges[1:]) / 2, rtol=1e-7)
+    x = np.sort(np.random.default_rng(n_bin).uniform(0.3, 260.0, 400))
+    want = orc.bin_of_d2(x * x, b.bin_edges)
+    got = np.array([tp.find_bin_index(v, b) for v in x])
+    np.testing.assert_array_equal(np.where(want >= 0, want + 1, -1), got)
 
 
 @pytest.mark.parametrize("case", range(9))
@@ -74,7 +91,7 @@ def test_bin_size_follows_reference(tp):
 
 @pytest.mark.parametrize("args", [
     (1.0, 100.0, 0, 1, 1.0),      # no bins
-    (1.0, 100.0, 64, 1, 1.0),     # more than TPCF_MAX_BIN
+    (1.0, 100.0, 1025, 1, 1.0),   # more than TPCF_MAX_BIN
     (0.0, 100.0, 10, 1, 1.0),     # log spacing needs x_min > 0
     (5.0, 1.0, 10, 0, 1.0),       # x_max <= x_min
     (1.0, 10.0, 10, 7, 1.0),      # unknown spacing

@@ -185,11 +185,12 @@ def test_shear_marks_m_zero_cancelling_weights(tp, orc, ctx):
 
 
 @pytest.mark.parametrize("kind", [KAPPA, SHEAR])
-@pytest.mark.parametrize("n_bin,spacing,x_min", [(63, 1, 0.3), (1, 1, 2.0), (63, 0, 0.0)])
+@pytest.mark.parametrize("n_bin,spacing,x_min", [(63, 1, 0.3), (1, 1, 2.0), (63, 0, 0.0), (64, 1, 0.3)])
 def test_bin_count_extremes(tp, orc, ctx, kind, n_bin, spacing, x_min):
-    """The largest bin count (TPCF_MAX_BIN = 63: every lane of the per-bin
-    table but one, the two-bin window at the last bin) and a single bin, with
-    marks and patches."""
+    """The largest bin count of one pair-kernel pass (63: every lane of the
+    per-bin table but one, the two-bin window at the last bin), one more (two
+    passes, the second of a single bin), and a single bin, with marks and
+    patches."""
     lens, src = uniform_case(91 + n_bin + spacing + kind, 2500, 3000, 60.0, kind, z=kind == SHEAR, zero_w=True)
     r = _rng(92)
     pa = r.integers(0, 3, len(lens["x"])).astype(np.int32)
