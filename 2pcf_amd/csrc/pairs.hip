@@ -13,11 +13,13 @@
 //   edge  -- pairs straddle one bin edge: per-pair FP32 test against the
 //            squared edge, FP64 re-test for pairs within the band; each
 //            side's sums are taken directly, in exec-masked branches;
-//   multi -- several edges (or a z test): windows of two bins, bounded tests.
+//   multi -- several edges (or a z test): sweeps of up to 4 bins, each pair
+//            added to its bin's lane-private row in LDS.
 // Each segment's pairs are summed per lane in the input precision (at most
 // kSegMax terms), then added to per-lane FP64 sums for a window of two
-// adjacent bins.  When the window moves these are reduced across the wave
-// (DPP) into FP64 per-bin registers (lane b holds bin b), which go to the
+// adjacent bins (fast, edge) or reduced straight from the sweep's rows
+// (multi).  When the window moves these are reduced across the wave (DPP)
+// into FP64 per-bin registers (lane b holds bin b), which go to the
 // per-patch-pair FP64 accumulators with global atomics when the source patch
 // changes.
 #include <algorithm>
@@ -127,30 +129,59 @@ __device__ __forceinline__ void arrived(const SrcRec<T> &q)
 // waited for (`arrived`) before the loads of the other set are issued.  The
 // record array is padded with kRecPad records, so reading up to three records
 // past a segment's end stays inside the allocation.
+// The source records are read through a pointer into the constant address
+// space: loads from it are scalar whenever the address is wave-uniform (a
+// global pointer that moves through the loop loses the provenance that lets
+// the compiler prove the records unmodified, and is then read with vector
+// loads).  The records are written only by the preparation kernels.
+template <typename T>
+using ConstRec = const __attribute__((address_space(4))) SrcRec<T>;
+
+template <typename T>
+__device__ __forceinline__ ConstRec<T> *const_rec(const SrcRec<T> *r)
+{
+    return (ConstRec<T> *)r;
+}
+
+template <typename T>
+__device__ __forceinline__ SrcRec<T> rec_at(ConstRec<T> *r)
+{
+    SrcRec<T> q;
+    q.x = r->x; q.y = r->y; q.w = r->w; q.wk = r->wk;
+    q.s1w = r->s1w; q.s2w = r->s2w; q.s1w2 = r->s1w2; q.s2w2 = r->s2w2;
+    return q;
+}
+
 template <typename T, typename F>
-__device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_t count, F &&f)
+__device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ rg, uint32_t count, F &&f)
 {
     if (count == 0) return;
-    SrcRec<T> a0 = r[0], a1 = r[1];
-    uint32_t j = 0;
+    // a moving record pointer and a down-counter: both change every half
+    // iteration, so they stay in registers (a fixed base pointer live across
+    // the loop was displaced and reloaded for every load)
+    ConstRec<T> *r = const_rec(rg);
+    SrcRec<T> a0 = rec_at(r), a1 = rec_at(r + 1);
+    uint32_t left = count;
     for (;;) {
         arrived(a0);
         arrived(a1);
-        const SrcRec<T> b0 = r[j + 2], b1 = r[j + 3];
+        const SrcRec<T> b0 = rec_at(r + 2), b1 = rec_at(r + 3);
         asm volatile("" ::: "memory");   // the loads stay ahead of this half's work
         f(a0);
-        if (j + 1 < count) f(a1);
-        j += 2;
-        if (j >= count) break;
+        if (left > 1) f(a1);
+        if (left <= 2) break;
+        left -= 2;
+        r += 2;
         arrived(b0);
         arrived(b1);
-        a0 = r[j + 2];
-        a1 = r[j + 3];
+        a0 = rec_at(r + 2);
+        a1 = rec_at(r + 3);
         asm volatile("" ::: "memory");
         f(b0);
-        if (j + 1 < count) f(b1);
-        j += 2;
-        if (j >= count) break;
+        if (left > 1) f(b1);
+        if (left <= 2) break;
+        left -= 2;
+        r += 2;
     }
 }
 
@@ -162,8 +193,9 @@ __device__ __forceinline__ void stream2(const SrcRec<T> *__restrict__ r, uint32_
 // q = Theta, G1, G2, W, K kept in FP64 registers and the pair count N in a
 // u64 register, by the lane of the bin (K only when some m != 0: otherwise it
 // equals W and is not kept).  Only
-// the window sums use LDS (5.6 KB per wave), which leaves room for 7
-// workgroups of 4 waves per CU.
+// the window sums (or, for a multi segment, the sweep's rows in their place)
+// and the grid use LDS: 4.6 KB per wave when all m == 0, 5.7 KB otherwise,
+// which leaves room for 8 and 7 workgroups of 4 waves per CU.
 // window quantities per bin: Theta, G1, G2, W (+ K unless all m == 0)
 template <bool MZ>
 constexpr int win_q() { return MZ ? 4 : 5; }
@@ -552,82 +584,267 @@ __device__ __forceinline__ void edge_side_pass(Win &w, const LaneData<T> &L, con
     w.edge = true;
 }
 
-// General pass over a segment for the window bins wk (mask m) and wk+1 (m2),
-// bounded on both sides of the window, with the z test when needed.
-template <typename T, int KIND, bool MZ, bool ZCHECK, bool SPH>
-__device__ __forceinline__ void win_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
-                                         const SrcRec<T> *__restrict__ r, const T *__restrict__ zr, uint32_t count,
-                                         int n_bin)
+// Multi pass: a segment whose pairs span more bins than an edge pass takes
+// is swept once per S consecutive bins k0 .. k0 + S - 1.  Each pair's bin
+// follows from the S + 1 edge tests of the sweep (FP32 d2 < lo; the lanes
+// within the band of some edge re-decide every test in FP64, in a branch the
+// wave takes only when such a lane exists), and the pair's terms are added to
+// lane-private rows in LDS at that bin (the address carries the bin, so no
+// VALU work per bin).  FP32: one float4 (Theta, G1, G2, W) per bin and lane,
+// read, added to and written back (the LDS float atomics run at a small
+// fraction of the rate of a b128 read and write), and the pair count with an
+// integer LDS add; with K kept, a float2 (K, pair count) per bin and lane
+// instead of the integer count.  FP64: rows[q][bin][lane] per quantity with
+// FP64 LDS adds, and the u32 counts.  The rows overlay the wave's window sums
+// (a window is flushed before a multi segment), and a sweep's rows hold at
+// most kSegMax terms per lane and bin, like the other passes' FP32 sums.
+// Bins per sweep: FP32 4 (3 with K kept), FP64 2, which fit the LDS the
+// kernel's occupancy allows.
+template <typename T, bool MZ>
+constexpr int multi_bins() { return sizeof(T) == 4 ? (MZ ? 4 : 3) : 2; }
+
+template <typename T, bool MZ>
+constexpr int multi_row_bytes()
 {
-    constexpr int WQ = win_q<MZ>();
-    const int lo_e = w.wk, hi_e = w.wk + 2;
-    const bool has_lo = lo_e >= 0, has_hi = hi_e <= n_bin;
-    const EdgeTest<T> emid(A, w.wk + 1), elo(A, has_lo ? lo_e : 0), ehi(A, has_hi ? hi_e : 0);
-    const bool track0 = (KIND == SHEAR) && A.xmin_zero && (w.wk == 0);
-    T sT0 = 0, sT1 = 0, sG10 = 0, sG11 = 0, sG20 = 0, sG21 = 0;
-    T sW0 = 0, sW1 = 0, sK0 = 0, sK1 = 0, sZ = 0;
-    uint32_t n0 = 0, n1 = 0;
-    for (uint32_t j = 0; j < count; ++j) {
-        const SrcRec<T> q = r[j];
+    constexpr int S = multi_bins<T, MZ>();
+    if constexpr (sizeof(T) == 4) return S * kWave * (MZ ? 16 : 24) + (MZ ? S / 2 * kWave * 4 : 0);
+    else return S * kWave * (win_q<MZ>() * 8 + 4);
+}
+
+// 8-byte units of a wave's LDS block: the FP64 window sums or the multi-pass rows
+template <typename T, bool MZ>
+constexpr int wave_lds_doubles()
+{
+    constexpr int win = (2 * win_q<MZ>() + 1) * kWave;
+    constexpr int rows = (multi_row_bytes<T, MZ>() + 7) / 8;
+    return win > rows ? win : rows;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+    int x = (int)v;
+    x += dpp_i<0x111>(x);
+    x += dpp_i<0x112>(x);
+    x += dpp_i<0x114>(x);
+    x += dpp_i<0x118>(x);
+    x += dpp_i<0x142, 0xa, false>(x);
+    x += dpp_i<0x143, 0xc, false>(x);
+    return (uint32_t)__builtin_amdgcn_readlane(x, 63);
+}
+
+// One sweep over a segment for bins k0 .. k0 + S - 1 (edges clamped at
+// n_bin: a bin past the last one is empty).  ZCHECK: the z test,
+// src/tpcf.f90:322-324.  ZSEP: shear with x_min == 0 and overlapping boxes,
+// where a zero separation is a pair of bin 0: its position angle is 0/0, so
+// its G terms are NaN (the reference's G_t, G_x of bin 0 and the lens' xi
+// marks become NaN, src/tpcf.f90:343-355) and its separation is 0.
+template <typename T, int KIND, bool MZ, bool ZCHECK, bool ZSEP, bool SPH>
+__device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
+                                            const SrcRec<T> *__restrict__ r, const T *__restrict__ zr,
+                                            uint32_t count, int n_bin, int k0)
+{
+    constexpr int S = multi_bins<T, MZ>(), QR = win_q<MZ>();
+    const int lane = L.lane;
+    // FP32 rows: float4 r4[S][kWave], then u32 cnt[S / 2][kWave] holding the
+    // counts of bins 2 i and 2 i + 1 in its low and high 16 bits (MZ) or
+    // float2 r2[S][kWave] = (K, count); FP64 rows: T rows[QR][S][kWave], then
+    // u32 cnt[S][kWave]
+    float4 *const r4 = reinterpret_cast<float4 *>(w.acc);
+    float2 *const r2 = reinterpret_cast<float2 *>(r4 + S * kWave);
+    T *const rows = reinterpret_cast<T *>(w.acc);
+    uint32_t *const cnt = sizeof(T) == 4 ? reinterpret_cast<uint32_t *>(r4 + S * kWave)
+                                         : reinterpret_cast<uint32_t *>(rows + QR * S * kWave);
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            r4[i * kWave + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!MZ) r2[i * kWave + lane] = make_float2(0.f, 0.f);
+            else if (i % 2 == 0) cnt[i / 2 * kWave + lane] = 0u;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < QR * S; ++i) rows[i * kWave + lane] = (T)0;
+#pragma unroll
+        for (int i = 0; i < S; ++i) cnt[i * kWave + lane] = 0u;
+    }
+    // the sweep's edges (FP32: the band [lo, hi] around each; FP64: the edge)
+    T lo[S + 1], hi[S + 1];
+#pragma unroll
+    for (int j = 0; j <= S; ++j) {
+        const int e = min(k0 + j, n_bin);
+        if constexpr (sizeof(T) == 4) {
+            lo[j] = A.e2f[e] - A.band * A.e2f[e];
+            hi[j] = A.e2f[e] + A.band * A.e2f[e];
+        } else {
+            lo[j] = (T)A.e2d[e];
+            hi[j] = lo[j];
+        }
+        // held in vector registers: the scalar registers are the kernel's
+        // scarcer resource (spilled values would be reloaded in every pair loop)
+        if constexpr (sizeof(T) == 4) asm volatile("" : "+v"(lo[j]), "+v"(hi[j]));
+        else asm volatile("" : "+v"(lo[j]));
+    }
+    uint32_t j = 0;
+    stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
         const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
         const T d2 = g.d2;
-        const bool plo = emid.below(d2, L.xa, L.ya, q.x, q.y);
-        const bool ge_lo = has_lo ? !elo.below(d2, L.xa, L.ya, q.x, q.y) : true;
-        const bool lt_hi = has_hi ? ehi.below(d2, L.xa, L.ya, q.x, q.y) : true;
-        bool in0 = plo && ge_lo, in1 = !plo && lt_hi;
-        if (KIND == SHEAR && ZCHECK) {
-            if (L.za > zr[j]) { in0 = false; in1 = false; }   // source in front of the lens, src/tpcf.f90:322-324
-        }
-        // a segment spanning many bins is walked once per window: a source
-        // none of whose pairs fall in this window adds nothing (whole wave)
-        if (!__builtin_amdgcn_ballot_w64(in0 || in1)) continue;
-        // 0/1 masks rather than branches: the window's two bins are usually
-        // both present in a wave, and masked FMAs measured faster here
-        const T m = in0 ? (T)1 : (T)0, m2 = in1 ? (T)1 : (T)0;
-        n0 += __popcll(__builtin_amdgcn_ballot_w64(m != (T)0) & w.valid);
-        n1 += __popcll(__builtin_amdgcn_ballot_w64(m2 != (T)0) & w.valid);
-        // keep every per-pair value finite so that 0/1 masks cannot leak a NaN
-        // from an excluded pair; in-bin zero separations are tracked apart
-        T d, inv, u, v;
-        pair_proj<T, SPH, true>(L, g, d, inv, u, v);
-        if (track0) sZ += (d2 == (T)0) ? m : (T)0;
-        const T tv = d * q.w;
-        sT0 = fma(tv, m, sT0);
-        sT1 = fma(tv, m2, sT1);
-        if (KIND == SHEAR) {
-            const T g1 = fma(q.s1w, u, q.s2w2 * v) * inv;
-            const T g2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)) * inv;
-            sG10 = fma(g1, m, sG10);
-            sG11 = fma(g1, m2, sG11);
-            sG20 = fma(g2, m, sG20);
-            sG21 = fma(g2, m2, sG21);
+        // lane masks (wave ballots of the compares, kept in scalar registers):
+        // ge[i] = the lanes whose d2 is at or above edge k0 + i
+        uint64_t ge[S + 1];
+        if constexpr (sizeof(T) == 4) {
+            uint64_t amb = 0;
+#pragma unroll
+            for (int i = 0; i <= S; ++i) {
+                ge[i] = __builtin_amdgcn_ballot_w64(!(d2 < lo[i]));
+                amb |= ge[i] & __builtin_amdgcn_ballot_w64(d2 <= hi[i]);
+            }
+            if (amb) {
+                // rare (a wave-uniform branch): the band's lanes take every
+                // test in FP64
+                float xa = L.xa, ya = L.ya;
+                asm volatile("" : "+v"(xa), "+v"(ya));
+                const double dx = (double)xa - (double)q.x, dy = (double)ya - (double)q.y;
+                const double dd2 = fma(dx, dx, dy * dy);
+#pragma unroll
+                for (int i = 0; i <= S; ++i)
+                    ge[i] = (ge[i] & ~amb) | (__builtin_amdgcn_ballot_w64(!(dd2 < A.e2d[min(k0 + i, n_bin)])) & amb);
+            }
         } else {
-            sG10 = fma(q.s1w, m, sG10);
-            sG11 = fma(q.s1w, m2, sG11);
-            sG20 = fma(q.s2w, m, sG20);
-            sG21 = fma(q.s2w, m2, sG21);
+#pragma unroll
+            for (int i = 0; i <= S; ++i) ge[i] = __builtin_amdgcn_ballot_w64(!(d2 < lo[i]));
         }
-        sW0 = fma(q.w, m, sW0);
-        sW1 = fma(q.w, m2, sW1);
-        if (!MZ) {
-            sK0 = fma(q.wk, m, sK0);
-            sK1 = fma(q.wk, m2, sK1);
+        uint64_t in = ge[0] & ~ge[S] & w.valid;
+        if constexpr (KIND == SHEAR && ZCHECK) in &= __builtin_amdgcn_ballot_w64(!(L.za > zr[j]));   // source in front of the lens
+        ++j;
+        if (__builtin_amdgcn_inverse_ballot_w64(in)) {
+            int k = 0;
+#pragma unroll
+            for (int i = 1; i < S; ++i) k += __builtin_amdgcn_inverse_ballot_w64(ge[i]) ? 1 : 0;
+            T d, t1, t2;
+            if constexpr (KIND == SHEAR) {
+                T inv, u, v;
+                pair_proj<T, SPH, false>(L, g, d, inv, u, v);
+                t1 = fma(q.s1w, u, q.s2w2 * v) * inv;
+                t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)) * inv;
+                if constexpr (ZSEP) {
+                    const bool z0 = d2 == (T)0;
+                    d = z0 ? (T)0 : d;
+                    t1 = z0 ? (T)__builtin_nan("") : t1;
+                    t2 = z0 ? (T)__builtin_nan("") : t2;
+                }
+            } else {
+                if constexpr (SPH) d = sph_angle(d2);
+                else d = Geo<T>::dist(d2);
+                t1 = q.s1w;
+                t2 = q.s2w;
+            }
+            const int at = k * kWave + lane;
+            if constexpr (sizeof(T) == 4) {
+                float4 a = r4[at];
+                a.x += d * q.w;
+                a.y += t1;
+                a.z += t2;
+                a.w += q.w;
+                r4[at] = a;
+                if constexpr (MZ) {
+                    // bin k counts in half k % 2 of word k / 2; the edge masks
+                    // are nested (ge[i + 1] within ge[i]), so k >= 2 is ge[2]
+                    // and k is odd where ge[1] ^ ge[2] ^ ge[3] holds (at most
+                    // kSegMax < 2^16 pairs per lane and bin in a sweep)
+                    static_assert(S == 4, "two count words of two bins");
+                    const bool up = __builtin_amdgcn_inverse_ballot_w64(ge[2]);
+                    const bool odd = __builtin_amdgcn_inverse_ballot_w64(ge[1] ^ ge[2] ^ ge[3]);
+                    atomicAdd(&cnt[(up ? kWave : 0) + lane], odd ? 0x10000u : 1u);
+                } else {
+                    float2 b = r2[at];
+                    b.x += q.wk;
+                    b.y += 1.0f;   // exact: at most kSegMax per sweep
+                    r2[at] = b;
+                }
+            } else {
+                T *const p = rows + at;
+                atomicAdd(p + QT * S * kWave, d * q.w);
+                atomicAdd(p + QG1 * S * kWave, t1);
+                atomicAdd(p + QG2 * S * kWave, t2);
+                atomicAdd(p + QW * S * kWave, q.w);
+                if constexpr (!MZ) atomicAdd(p + QK * S * kWave, q.wk);
+                atomicAdd(&cnt[at], 1u);
+            }
         }
-    }
+    });
+}
+
+// The sweep's rows into the per-bin registers (lane b holds bin b) and, with
+// marks, into the marks of the chunk's lenses, as win_flush does for a window.
+template <typename T, int KIND, bool MARKS, bool MZ>
+__device__ __forceinline__ void multi_flush(Win &w, const LaneData<T> &L, int n_bin, double *__restrict__ marks,
+                                            int k0)
+{
+    constexpr int S = multi_bins<T, MZ>(), QR = win_q<MZ>();
+    const float4 *r4 = reinterpret_cast<const float4 *>(w.acc);
+    const float2 *r2 = reinterpret_cast<const float2 *>(r4 + S * kWave);
+    const T *rows = reinterpret_cast<const T *>(w.acc);
+    const uint32_t *cnt = sizeof(T) == 4 ? reinterpret_cast<const uint32_t *>(r4 + S * kWave)
+                                         : reinterpret_cast<const uint32_t *>(rows + QR * S * kWave);
     const int lane = L.lane;
-    wacc(w, QT, lane) += (double)sT0; wacc(w, WQ + QT, lane) += (double)sT1;
-    wacc(w, QG1, lane) += (double)sG10; wacc(w, WQ + QG1, lane) += (double)sG11;
-    wacc(w, QG2, lane) += (double)sG20; wacc(w, WQ + QG2, lane) += (double)sG21;
-    wacc(w, QW, lane) += (double)sW0; wacc(w, WQ + QW, lane) += (double)sW1;
-    if (!MZ) {
-        wacc(w, QK, lane) += (double)sK0;
-        wacc(w, WQ + QK, lane) += (double)sK1;
+    const double dwa = (double)L.wa, daa = (double)L.aa;
+    const double gs = (KIND == SHEAR) ? -1.0 : 1.0;   // G_t -= ... for shear, src/tpcf.f90:348-349
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int bin = k0 + s;
+        if (bin >= n_bin) break;
+        const int at = s * kWave + lane;
+        double aT, aG1, aG2, aW, aK;
+        uint32_t nl;
+        if constexpr (sizeof(T) == 4) {
+            const float4 a = r4[at];
+            aT = a.x;
+            aG1 = a.y;
+            aG2 = a.z;
+            aW = a.w;
+            if constexpr (MZ) {
+                nl = (cnt[s / 2 * kWave + lane] >> (16 * (s % 2))) & 0xffffu;
+                aK = aW;
+            } else {
+                const float2 b = r2[at];
+                aK = b.x;
+                nl = (uint32_t)b.y;
+            }
+        } else {
+            aT = rows[QT * S * kWave + at];
+            aG1 = rows[QG1 * S * kWave + at];
+            aG2 = rows[QG2 * S * kWave + at];
+            aW = rows[QW * S * kWave + at];
+            aK = MZ ? aW : (double)rows[QK * S * kWave + at];
+            nl = cnt[at];
+        }
+        if (!__ballot(nl != 0u)) continue;   // no pair of the chunk in this bin
+        const uint32_t n = wave_sum_u32(nl);
+        const double sT = wave_sum(dwa * aT);
+        const double sG1 = gs * wave_sum(daa * aG1);
+        const double sG2 = gs * wave_sum(daa * aG2);
+        const double sW = wave_sum(dwa * aW);
+        const double sK = MZ ? sW : wave_sum(dwa * aK);
+        if (lane == bin) {
+            w.t[QT] += sT;
+            w.t[QG1] += sG1;
+            w.t[QG2] += sG2;
+            w.t[QW] += sW;
+            if (!MZ) w.t[QK] += sK;
+            w.tn += n;
+        }
+        if (MARKS && L.valid) {
+            const double m0 = gs * daa * aG1, m1 = gs * daa * aG2, m2 = dwa * aK, m3 = dwa * aW;
+            if (m0 != 0.0 || m1 != 0.0 || m2 != 0.0 || m3 != 0.0) {   // NaN != 0
+                const int64_t nm = w.mstride;
+                double *mk = marks + mark_at(nm, bin, 0, L.li);
+                unsafeAtomicAdd(&mk[0], m0);
+                unsafeAtomicAdd(&mk[nm], m1);
+                if (!MZ) unsafeAtomicAdd(&mk[2 * nm], m2);
+                unsafeAtomicAdd(&mk[3 * nm], m3);
+            }
+        }
     }
-    if (KIND == SHEAR) wacc(w, 2 * WQ, lane) += (double)sZ;
-    tab_count(w, lane, w.wk, n_bin, n0);
-    tab_count(w, lane, w.wk + 1, n_bin, n1);
-    w.used0 = w.used1 = true;
-    w.edge = true;
 }
 
 
@@ -673,11 +890,22 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
     for (int q = 0; q < kNQ; ++q) w.t[q] = 0.0;
     w.tn = 0;
 
-    uint64_t cseg0 = 0, cseg1 = 0, cseg2 = 0, cpair0 = 0, cpair1 = 0, cpair2 = 0;   // class counters
     const double xm = A.x_max, mg = C.margin;
-    const Grid g = C.g;
+    // The grid is kept in this wave's LDS block (after its sums) and read
+    // again for each row: held in scalar registers across the item it
+    // displaced values that the pair loops then reloaded on every iteration.
+    typedef __attribute__((address_space(3))) double LdsD;
+    LdsD *const gsh = (LdsD *)(acc + wave_lds_doubles<T, MZ>());
+    if (lane == 0) {
+        gsh[0] = C.g.x0;
+        gsh[1] = C.g.y0;
+        gsh[2] = C.g.cell;
+        gsh[3] = C.g.inv;
+        gsh[4] = __longlong_as_double(((long long)C.g.ny << 32) | (unsigned)C.g.nx);
+        gsh[5] = __longlong_as_double((long long)C.g.ncells);
+    }
     int ry0, ry1;
-    row_span(g, (double)ch.by0, (double)ch.by1, xm, mg, ry0, ry1);
+    row_span(C.g, (double)ch.by0, (double)ch.by1, xm, mg, ry0, ry1);
     for (int pb = 0; pb < C.n_pb; ++pb) {
         const int4 pbx = C.pbox[pb];
         if (pbx.x > pbx.y) continue;   // no sources in this patch
@@ -688,6 +916,17 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
         const int first = cy0 + ((split - (cy0 - ry0)) % n_split + n_split) % n_split;
         for (int cy = first; cy <= cy1; cy += n_split) {
             int cx0, cx1;
+            LdsD *gq = gsh;
+            asm volatile("" : "+s"(gq));   // read at each row, not hoisted
+            Grid g;
+            g.x0 = gq[0];
+            g.y0 = gq[1];
+            g.cell = gq[2];
+            g.inv = gq[3];
+            const long long nxy = __double_as_longlong(gq[4]);
+            g.nx = (int32_t)(nxy & 0xffffffffll);
+            g.ny = (int32_t)(nxy >> 32);
+            g.ncells = (int64_t)__double_as_longlong(gq[5]);
             // opaque copies: the FP64 box is formed per row instead of being
             // hoisted into eight registers live across the whole loop nest
             T bx0 = ch.bx0, bx1 = ch.bx1, by0 = ch.by0, by1 = ch.by1;
@@ -813,22 +1052,34 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                     }
                 } else {
                     cls = 2;
-                    // ---- multi: two-bin windows over [kmin, kmax], bounded tests ----
+                    // ---- multi: sweeps of multi_bins() bins over [kmin, kmax] ----
                     const T *zr = (KIND == SHEAR) ? C.sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
-                    // a window starting at n_bin would hold no in-range bin
+                    // the sweeps' rows overlay the window sums
+                    win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
                     const int wlast = min(kmax, n_bin - 1);
-                    for (int w0 = kmin; w0 <= wlast; w0 += 2) {
-                        win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, w0);
-                        if (zc) win_pass<T, KIND, MZ, true, SPH>(w, L, A, r, zr, cnt, n_bin);
-                        else win_pass<T, KIND, MZ, false, SPH>(w, L, A, r, zr, cnt, n_bin);
-                        win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
+                    for (int k0 = max(kmin, 0); k0 <= wlast; k0 += multi_bins<T, MZ>()) {
+                        // a zero separation is in range only at x_min == 0, in
+                        // bin 0, with the boxes overlapping
+                        const bool zsep = KIND == SHEAR && A.xmin_zero && k0 == 0 && lo2 == (T)0;
+                        if (zc) {
+                            if (zsep) multi_sweep<T, KIND, MZ, true, true, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                            else multi_sweep<T, KIND, MZ, true, false, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                        } else {
+                            if (zsep) multi_sweep<T, KIND, MZ, false, true, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                            else multi_sweep<T, KIND, MZ, false, false, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                        }
+                        multi_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, k0);
                     }
+                    win_clear<MZ>(w, lane);   // the window sums start from zero again
                 }
                 asm volatile("" ::"v"(pfv), "v"(pfv2));
                 if (A.counters) {
-                    if (cls == 0) { cseg0 += 1; cpair0 += (uint64_t)cnt * ch.count; }
-                    else if (cls == 1) { cseg1 += 1; cpair1 += (uint64_t)cnt * ch.count; }
-                    else { cseg2 += 1; cpair2 += (uint64_t)cnt * ch.count; }
+                    // class counters (diagnostic runs only): added per segment,
+                    // so that no register is held for them across the item
+                    if (lane == 0) {
+                        atomicAdd(&probe[cls], 1ull);
+                        atomicAdd(&probe[3 + cls], (unsigned long long)cnt * ch.count);
+                    }
                 }
             }
         }
@@ -847,21 +1098,13 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
         for (int q = 0; q < kNQ; ++q) w.t[q] = 0.0;
         w.tn = 0;
     }
-    if (A.counters && lane == 0) {
-        atomicAdd(&probe[0], (unsigned long long)cseg0);
-        atomicAdd(&probe[1], (unsigned long long)cseg1);
-        atomicAdd(&probe[2], (unsigned long long)cseg2);
-        atomicAdd(&probe[3], (unsigned long long)cpair0);
-        atomicAdd(&probe[4], (unsigned long long)cpair1);
-        atomicAdd(&probe[5], (unsigned long long)cpair2);
-    }
 }
 
 // Waves per SIMD the register budget is set for.  The pair loops are
 // latency-bound: on the 20M x 20M shear workload 5 waves take 6.73 s, 6 take
 // 5.92 s, 7 take 5.60 s and 8 take 5.35 s.  FP32 with all m == 0 (MZ) keeps
-// 18 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
-// few cold-path spills); otherwise 22.5 KB allows 7 (72 VGPRs).  FP64 with
+// 18.2 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
+// few cold-path spills); otherwise 22.7 KB allows 7 (72 VGPRs).  FP64 with
 // MZ runs 6 waves (80 VGPRs, some spills): on the FP64 1M x 2M shear workload
 // 4 waves take 208 ms, 5 take 187 ms and 6 take 182 ms (7: the same as 6,
 // with twice the spills).  FP64 with some m != 0 runs 5 waves (96 VGPRs): on
@@ -894,8 +1137,8 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     const int slot = (int)(b % kSlots);
     const int64_t it = ((b % kSlots) * A.per_xcd + b / kSlots) * kWavesPerBlock + wv;
     if (it >= A.n_items) return;
-    constexpr int WQ = win_q<MZ>();
-    __shared__ double s_acc[kWavesPerBlock][(2 * WQ + 1) * kWave];
+    // per wave: the window sums or multi-pass rows, then the grid (5 x 8 bytes)
+    __shared__ __align__(16) double s_acc[kWavesPerBlock][wave_lds_doubles<T, MZ>() + 6];
     CatArgs<T> C;
     C.g = g;
     C.n_pa = n_pa;
