@@ -119,11 +119,14 @@ __host__ __device__ inline uint32_t hilbert16(uint32_t x, uint32_t y)
     return d;
 }
 
-// One unit of pair-kernel work: rows (cy - row0) % n_split == split of a chunk.
+// One unit of pair-kernel work: the source cell rows row0..row1 of a lens
+// chunk (one band of rows, clipped to the rows the chunk can reach).  Items
+// with chunk == kNoChunk pad the per-XCD item ranges to equal length.
 struct Item {
     uint32_t chunk;
-    uint16_t split, n_split;
+    uint16_t row0, row1;
 };
+constexpr uint32_t kNoChunk = 0xffffffffu;
 
 // ---- wavefront reductions (DPP row shifts + row broadcasts, gfx9 family) ----
 template <int CTRL, int ROWMASK = 0xf, bool BC = true>
@@ -187,9 +190,11 @@ struct tpcf_ctx {
     hipEvent_t ev[6] = {};
     double opt_cell = 0.0;
     // defaults measured on the 20M x 20M clustered shear workload: cell targets
-    // 48-96 and splits 0.15-0.5 are within 1 % of the best (64, 0.35)
+    // 48-96 are within 1 % of the best (64); work-item bands of 0.1-0.35
+    // x_max are within 0.5 % of each other there, and 0.2-0.25 are the best
+    // on the FP64 1M x 2M workload
     double opt_cell_target = 64.0;
-    double opt_split = 0.35;      // work-item size in units of the mean chunk work
+    double opt_split = 0.25;      // work-item band height in units of x_max
     bool opt_counters = false;
     tpcf_stats stats{};
     std::string err;

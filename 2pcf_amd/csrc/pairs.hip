@@ -714,7 +714,10 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
             for (int i = 0; i <= S; ++i) ge[i] = __builtin_amdgcn_ballot_w64(!(d2 < lo[i]));
         }
         uint64_t in = ge[0] & ~ge[S] & w.valid;
-        if constexpr (KIND == SHEAR && ZCHECK) in &= __builtin_amdgcn_ballot_w64(!(L.za > zr[j]));   // source in front of the lens
+        if constexpr (KIND == SHEAR && ZCHECK) {
+            typedef const __attribute__((address_space(4))) T CT;
+            in &= __builtin_amdgcn_ballot_w64(!(L.za > ((CT *)zr)[j]));   // source in front of the lens
+        }
         ++j;
         if (__builtin_amdgcn_inverse_ballot_w64(in)) {
             int k = 0;
@@ -850,7 +853,7 @@ __device__ __forceinline__ void multi_flush(Win &w, const LaneData<T> &L, int n_
 
 
 // The work of one item: lens chunk item.chunk against the source segments in
-// its share of cell rows, into the per-XCD accumulator copy `slot`.
+// its band of cell rows, into the per-XCD accumulator copy `slot`.
 template <typename T, int KIND, bool MARKS, bool MZ, bool SPH>
 __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T> &C, const Item item, const int slot,
                                           const int lane, double *acc, const T e2_lane,
@@ -858,7 +861,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
 {
     constexpr int WQ = win_q<MZ>();
     const Chunk<T> ch = C.chunks[item.chunk];
-    const int split = item.split, n_split = item.n_split;
+    const int row0 = item.row0, row1 = item.row1;   // this item's band of cell rows
     LaneData<T> L;
     L.lane = lane;
     L.valid = lane < (int)ch.count;
@@ -891,9 +894,13 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
     w.tn = 0;
 
     const double xm = A.x_max, mg = C.margin;
-    // The grid is kept in this wave's LDS block (after its sums) and read
-    // again for each row: held in scalar registers across the item it
-    // displaced values that the pair loops then reloaded on every iteration.
+    // The grid and the catalogue's segment pointers are kept in this wave's
+    // LDS block (after its sums) and read again where the row and segment
+    // loops use them: held in scalar registers across the item they displaced
+    // values that the pair loops then reloaded on every iteration.  The
+    // pointers are read back as constant-address-space pointers (the records
+    // they address are not written while the kernel runs), so their loads
+    // stay scalar.
     typedef __attribute__((address_space(3))) double LdsD;
     LdsD *const gsh = (LdsD *)(acc + wave_lds_doubles<T, MZ>());
     if (lane == 0) {
@@ -903,18 +910,28 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
         gsh[3] = C.g.inv;
         gsh[4] = __longlong_as_double(((long long)C.g.ny << 32) | (unsigned)C.g.nx);
         gsh[5] = __longlong_as_double((long long)C.g.ncells);
+        gsh[6] = __longlong_as_double((long long)(uintptr_t)C.segs);
+        gsh[7] = __longlong_as_double((long long)(uintptr_t)C.seg_first);
+        gsh[8] = __longlong_as_double((long long)(uintptr_t)C.srec);
+        gsh[9] = __longlong_as_double((long long)(uintptr_t)C.sz);
+        gsh[10] = __longlong_as_double((long long)(uintptr_t)C.pbox);
     }
-    int ry0, ry1;
-    row_span(C.g, (double)ch.by0, (double)ch.by1, xm, mg, ry0, ry1);
+    typedef const __attribute__((address_space(4))) Seg<T> CSeg;
+    typedef const __attribute__((address_space(4))) uint32_t CU32;
+    typedef const __attribute__((address_space(4))) int4 CI4;
+    auto lds_ptr = [&](int k) __attribute__((always_inline)) {
+        LdsD *q = gsh;
+        asm volatile("" : "+s"(q));   // read where used, not hoisted
+        return (uintptr_t)__double_as_longlong(q[k]);
+    };
     for (int pb = 0; pb < C.n_pb; ++pb) {
-        const int4 pbx = C.pbox[pb];
+        CI4 *const pbq = (CI4 *)lds_ptr(10) + pb;
+        const int4 pbx = make_int4(pbq->x, pbq->y, pbq->z, pbq->w);
         if (pbx.x > pbx.y) continue;   // no sources in this patch
-        const int cy0 = max(ry0, pbx.z);
-        const int cy1 = min(ry1, pbx.w);
+        const int cy0 = max(row0, pbx.z);
+        const int cy1 = min(row1, pbx.w);
         if (cy0 > cy1) continue;
-        // this item's rows: (cy - ry0) % n_split == split
-        const int first = cy0 + ((split - (cy0 - ry0)) % n_split + n_split) % n_split;
-        for (int cy = first; cy <= cy1; cy += n_split) {
+        for (int cy = cy0; cy <= cy1; ++cy) {
             int cx0, cx1;
             LdsD *gq = gsh;
             asm volatile("" : "+s"(gq));   // read at each row, not hoisted
@@ -936,12 +953,18 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
             cx1 = min(cx1, pbx.y);
             if (cx0 > cx1) continue;
             const int64_t key0 = (int64_t)pb * g.ncells + (int64_t)cy * g.nx + cx0;
-            const uint32_t s_begin = __builtin_amdgcn_readfirstlane(C.seg_first[key0]);
-            const uint32_t s_end = __builtin_amdgcn_readfirstlane(C.seg_first[key0 + (cx1 - cx0) + 1]);
+            CU32 *const seg_first = (CU32 *)lds_ptr(7);
+            const uint32_t s_begin = __builtin_amdgcn_readfirstlane(seg_first[key0]);
+            const uint32_t s_end = __builtin_amdgcn_readfirstlane(seg_first[key0 + (cx1 - cx0) + 1]);
             for (uint32_t si = s_begin; si < s_end; ++si) {
-                const Seg<T> sg = C.segs[si];
+                CSeg *const sq = (CSeg *)lds_ptr(6) + si;
+                Seg<T> sg;
+                sg.start = sq->start; sg.count = sq->count;
+                sg.x0 = sq->x0; sg.x1 = sq->x1; sg.y0 = sq->y0; sg.y1 = sq->y1; sg.zmin = sq->zmin;
+                sg.W = sq->W; sg.K = sq->K; sg.S1 = sq->S1; sg.S2 = sq->S2;
                 const uint32_t cnt = __builtin_amdgcn_readfirstlane(sg.count);
-                const SrcRec<T> *__restrict__ r = C.srec + __builtin_amdgcn_readfirstlane(sg.start);
+                const SrcRec<T> *__restrict__ r =
+                    (const SrcRec<T> *)lds_ptr(8) + __builtin_amdgcn_readfirstlane(sg.start);
                 // Warm L2 with the segment's records: lane l reads one value of
                 // each 128-B line it covers (FP32, 32-B records: record 4 l, so
                 // 64 lanes cover 256 records, a whole segment; FP64, 64-B
@@ -951,13 +974,16 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                 // with the lines on their way to L2 it is the L2 latency
                 // rather than HBM's (2 % on the 20M x 20M workload).  The
                 // values are only consumed after the pass.
+                // (global-address-space loads: a flat load would also hold up
+                // the waits of the scalar loads)
+                typedef const __attribute__((address_space(1))) float GF;
                 constexpr uint32_t kRecLine = 128 / sizeof(SrcRec<T>);
                 const uint32_t pfi = min(kRecLine * (uint32_t)lane, cnt - 1);
-                const float pfv = reinterpret_cast<const float *>(r + pfi)[0];
+                const float pfv = ((GF *)(r + pfi))[0];
                 float pfv2 = 0.0f;
                 if constexpr (kRecLine * kWave < kSegMax) {
                     const uint32_t pfj = min(kRecLine * (uint32_t)lane + kRecLine * kWave, cnt - 1);
-                    pfv2 = reinterpret_cast<const float *>(r + pfj)[0];
+                    pfv2 = ((GF *)(r + pfj))[0];
                 }
                 // squared distance range between the chunk box and the segment box
                 const T dxl = fmax((T)0, fmax(sg.x0 - ch.bx1, ch.bx0 - sg.x1));
@@ -1053,7 +1079,7 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                 } else {
                     cls = 2;
                     // ---- multi: sweeps of multi_bins() bins over [kmin, kmax] ----
-                    const T *zr = (KIND == SHEAR) ? C.sz + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
+                    const T *zr = (KIND == SHEAR) ? (const T *)lds_ptr(9) + __builtin_amdgcn_readfirstlane(sg.start) : nullptr;
                     // the sweeps' rows overlay the window sums
                     win_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks);
                     const int wlast = min(kmax, n_bin - 1);
@@ -1116,8 +1142,10 @@ constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : (MZ
 template <typename T, bool MZ, bool SPH>
 constexpr int pair_waves() { return SPH ? 4 : pair_waves_per_simd<T, MZ>(); }
 
-// One catalogue: one item per wave, items dealt so that consecutive items
-// (neighbouring chunks) run on one XCD and share its L2.  The catalogue's
+// One catalogue: one item per wave; workgroup b runs on XCD b % kSlots (the
+// hardware deals workgroups to the XCDs round-robin) and takes the next items
+// of that XCD's range, which the preparation ordered band by band, so the
+// waves running together on an XCD read the same source rows.  The catalogue's
 // pointers arrive as separate arguments (scalar loads on demand).
 template <typename T, int KIND, bool MARKS, bool MZ, bool SPH>
 __global__ __launch_bounds__(kWave *kWavesPerBlock)
@@ -1137,8 +1165,9 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     const int slot = (int)(b % kSlots);
     const int64_t it = ((b % kSlots) * A.per_xcd + b / kSlots) * kWavesPerBlock + wv;
     if (it >= A.n_items) return;
-    // per wave: the window sums or multi-pass rows, then the grid (5 x 8 bytes)
-    __shared__ __align__(16) double s_acc[kWavesPerBlock][wave_lds_doubles<T, MZ>() + 6];
+    // per wave: the window sums or multi-pass rows, then the grid and the
+    // segment pointers (11 x 8 bytes)
+    __shared__ __align__(16) double s_acc[kWavesPerBlock][wave_lds_doubles<T, MZ>() + 12];
     CatArgs<T> C;
     C.g = g;
     C.n_pa = n_pa;
@@ -1159,7 +1188,9 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     C.sph_lf = sph_lf;
     C.sph_segp = sph_segp;
     C.sph_chp = sph_chp;
-    pair_item<T, KIND, MARKS, MZ, SPH>(A, C, items[it], slot, lane, s_acc[wv], e2lane[lane], probe);
+    const Item item = items[it];
+    if (item.chunk == kNoChunk) return;   // padding of this XCD's item range
+    pair_item<T, KIND, MARKS, MZ, SPH>(A, C, item, slot, lane, s_acc[wv], e2lane[lane], probe);
 }
 
 // The pair kernel's arguments for bin group `group` of bins: bins

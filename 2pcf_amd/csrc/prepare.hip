@@ -6,9 +6,11 @@
 //    (patch, cell) run becomes a "segment" with a tight bounding box and the
 //    sums needed when all its pairs fall into one bin;
 //  * lenses by (patch_A, Hilbert index) and cut into compact chunks of up to
-//    64 lenses (one lens per lane), each with its bounding box; chunks that
-//    meet many sources are split into several work items (cell rows dealt
-//    round-robin) so that clustered fields keep the GPU evenly loaded.
+//    64 lenses (one lens per lane), each with its bounding box; a chunk's
+//    work is cut into items of one band of source cell rows each, ordered
+//    band by band on each XCD, so that the waves running together read the
+//    same rows of sources (from L2), and each XCD gets an equal estimated
+//    share of the work.
 // Sources masked by the reference (z_B < 0, src/tpcf.f90:306-309) and lenses
 // it skips (w_A == 0, src/tpcf.f90:285 / :474; z_A < 0 for shear) are dropped
 // here, which removes the same pairs.
@@ -383,18 +385,20 @@ __global__ __launch_bounds__(kBlk) void k_chunk_sums(int64_t n_chunks, const T *
     chunks[c].asum = as;
 }
 
-// Sources a chunk can meet: the sum over its candidate row ranges.
+// Sources a chunk can meet (the sum over its candidate row ranges) and the
+// cell rows it can reach.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_chunk_work(int64_t n_chunks, Grid g, int32_t n_pb, double xm, double mg,
                                                      const Chunk<T> *__restrict__ chunks,
                                                      const int4 *__restrict__ pbox, const uint32_t *__restrict__ cstart,
-                                                     float *__restrict__ work)
+                                                     float *__restrict__ work, int2 *__restrict__ rows)
 {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= n_chunks) return;
     const Chunk<T> ch = chunks[c];
     int32_t ry0, ry1;
     row_span(g, (double)ch.by0, (double)ch.by1, xm, mg, ry0, ry1);
+    rows[c] = make_int2(ry0, ry1);
     double total = 0.0;
     for (int32_t pb = 0; pb < n_pb; ++pb) {
         const int4 b = pbox[pb];
@@ -415,6 +419,65 @@ __global__ __launch_bounds__(kBlk) void k_chunk_work(int64_t n_chunks, Grid g, i
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlk - 1) / kBlk); }
+
+// The pair kernel's work items.  A chunk's reachable rows [ry0, ry1] are cut
+// at multiples of a band height of split * x_max (whole cell rows, at least
+// one), one item per (band, chunk) with work estimated as the chunk's sources
+// times the item's share of its rows.  The items are ordered band by band
+// (chunks in their sorted order within a band) and the sequence is cut into
+// kSlots consecutive ranges of equal estimated work, one per XCD: the pair
+// kernel deals range s to XCD s in order, so the waves running on an XCD
+// together cover one band of source rows for neighbouring chunks, and the
+// rows they read stay in that XCD's L2.  Each range is padded with kNoChunk
+// items to a common length (a multiple of the waves per workgroup).  Chunks
+// that meet no source get no item.
+std::vector<Item> band_items(const std::vector<float> &work, const std::vector<int2> &rows, const Grid &g,
+                             double x_max, double split)
+{
+    const int64_t nch = (int64_t)work.size();
+    const int band = (int)std::max(1.0, std::min(65535.0, std::floor(split * x_max / g.cell + 0.5)));
+    const int n_bands = (int)((g.ny + band - 1) / band);
+    // counting sort of the items by band
+    std::vector<int64_t> first(n_bands + 1, 0);
+    for (int64_t c = 0; c < nch; ++c) {
+        if (!(work[c] > 0.0f)) continue;
+        for (int b = rows[c].x / band; b <= rows[c].y / band; ++b) ++first[b + 1];
+    }
+    for (int b = 0; b < n_bands; ++b) first[b + 1] += first[b];
+    const int64_t n = first[n_bands];
+    std::vector<Item> seq(n);
+    std::vector<float> wseq(n);
+    std::vector<int64_t> at(first.begin(), first.end() - 1);
+    for (int64_t c = 0; c < nch; ++c) {
+        if (!(work[c] > 0.0f)) continue;
+        const int r0 = rows[c].x, r1 = rows[c].y;
+        for (int b = r0 / band; b <= r1 / band; ++b) {
+            const int lo = std::max(r0, b * band), hi = std::min(r1, b * band + band - 1);
+            const int64_t k = at[b]++;
+            seq[k] = Item{(uint32_t)c, (uint16_t)lo, (uint16_t)hi};
+            wseq[k] = work[c] * (float)(hi - lo + 1) / (float)(r1 - r0 + 1);
+        }
+    }
+    if (n == 0) return {};
+    // kSlots consecutive ranges of equal estimated work
+    double tot = 0.0;
+    for (float v : wseq) tot += v;
+    std::vector<int64_t> cut(kSlots + 1, n);
+    cut[0] = 0;
+    double acc = 0.0;
+    int s = 1;
+    for (int64_t k = 0; k < n && s < kSlots; ++k) {
+        acc += wseq[k];
+        while (s < kSlots && acc >= tot * s / kSlots) cut[s++] = k + 1;
+    }
+    int64_t len = 0;
+    for (int q = 0; q < kSlots; ++q) len = std::max(len, cut[q + 1] - cut[q]);
+    len = (len + kWavesPerBlock - 1) / kWavesPerBlock * kWavesPerBlock;
+    std::vector<Item> out((size_t)(len * kSlots), Item{kNoChunk, 0, 0});
+    for (int q = 0; q < kSlots; ++q)
+        std::copy(seq.begin() + cut[q], seq.begin() + cut[q + 1], out.begin() + q * len);
+    return out;
+}
 
 inline int bitlen(uint64_t v)
 {
@@ -628,26 +691,22 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
         hipLaunchKernelGGL((k_chunk_blocks<T, true>), dim3(blocks_for(nblk * kWave)), dim3(kBlk), 0, s, nblk, dblk,
                            dmax, lx, ly, nullptr, dnch + nblk, chunks);
         hipLaunchKernelGGL(k_chunk_sums<T>, dim3(blocks_for(nch)), dim3(kBlk), 0, s, nch, lz, lw, la, chunks);
-        // split chunks that meet many sources into several work items
+        // each chunk's reachable source rows and the sources in them
         float *dwork;
-        TPCF_TRY(ensure(ctx, tpcf_ctx::B_WORK, sizeof(float) * nch, (void **)&dwork));
+        TPCF_TRY(ensure(ctx, tpcf_ctx::B_WORK, (sizeof(float) + sizeof(int2)) * nch + 16, (void **)&dwork));
+        int2 *drows = reinterpret_cast<int2 *>(reinterpret_cast<char *>(dwork) + ((sizeof(float) * nch + 15) & ~(size_t)15));
         hipLaunchKernelGGL(k_chunk_work<T>, dim3(blocks_for(nch)), dim3(kBlk), 0, s, nch, g, a.n_pb, x_max, p.margin,
-                           chunks, pbox, cstart, dwork);
+                           chunks, pbox, cstart, dwork, drows);
         std::vector<float> hw(nch);
+        std::vector<int2> hrows(nch);
         TPCF_HIP(ctx, hipMemcpyAsync(hw.data(), dwork, sizeof(float) * nch, hipMemcpyDeviceToHost, s));
+        TPCF_HIP(ctx, hipMemcpyAsync(hrows.data(), drows, sizeof(int2) * nch, hipMemcpyDeviceToHost, s));
         TPCF_HIP(ctx, hipStreamSynchronize(s));
-        double tot = 0;
-        for (float v : hw) tot += v;
-        const double target = std::max(1.0, ctx->opt_split * tot / (double)nch);
-        std::vector<Item> hit;
-        hit.reserve(nch * 2);
-        for (int64_t c = 0; c < nch; ++c) {
-            const int ns = (int)std::min(1024.0, std::max(1.0, std::ceil(hw[c] / target)));
-            for (int k = 0; k < ns; ++k) hit.push_back(Item{(uint32_t)c, (uint16_t)k, (uint16_t)ns});
-        }
+        std::vector<Item> hit = band_items(hw, hrows, g, x_max, ctx->opt_split);
         p.n_items = (int64_t)hit.size();
-        TPCF_TRY(ensure(ctx, tpcf_ctx::B_ITEMS, sizeof(Item) * hit.size(), (void **)&items));
-        TPCF_HIP(ctx, hipMemcpyAsync(items, hit.data(), sizeof(Item) * hit.size(), hipMemcpyHostToDevice, s));
+        TPCF_TRY(ensure(ctx, tpcf_ctx::B_ITEMS, sizeof(Item) * std::max<size_t>(hit.size(), 1), (void **)&items));
+        if (!hit.empty())
+            TPCF_HIP(ctx, hipMemcpyAsync(items, hit.data(), sizeof(Item) * hit.size(), hipMemcpyHostToDevice, s));
         TPCF_HIP(ctx, hipStreamSynchronize(s));
         p.items = items;
     } else {
