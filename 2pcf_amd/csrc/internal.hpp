@@ -170,6 +170,19 @@ __device__ inline double wave_sum(double v)
     return *reinterpret_cast<double *>(&p);
 }
 
+// Sum over all 64 lanes of an integer; every lane must be active.
+__device__ inline uint32_t wave_sum_u32(uint32_t v)
+{
+    int x = (int)v;
+    x += dpp_i<0x111>(x);
+    x += dpp_i<0x112>(x);
+    x += dpp_i<0x114>(x);
+    x += dpp_i<0x118>(x);
+    x += dpp_i<0x142, 0xa, false>(x);
+    x += dpp_i<0x143, 0xc, false>(x);
+    return (uint32_t)__builtin_amdgcn_readlane(x, 63);
+}
+
 // ---- host plumbing ----
 struct DevBuf {
     void *p = nullptr;

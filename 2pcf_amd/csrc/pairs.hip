@@ -426,30 +426,28 @@ __device__ __forceinline__ void win_open(Win &w, const LaneData<T> &L, int n_bin
     w.open = true;
 }
 
-// The edge decision of the edge passes: FP32 d2 < lo, re-decided in FP64 for
-// the lanes inside the band; the branch is taken by the wave only when some
-// lane needs it.
+// The edge decision of the edge passes as a lane mask (the lanes whose pair
+// lies below the edge): FP32 d2 < lo, and for the lanes inside the band
+// [lo, hi] the FP64 decision, in a branch the wave takes only when such a
+// lane exists.  The masks stay in scalar registers: a lane's side is read back
+// with inverse_ballot where the sums are taken.
 template <typename T>
-__device__ __forceinline__ bool edge_below(const EdgeTest<T> &et, T d2, const LaneData<T> &L, const SrcRec<T> &q)
+__device__ __forceinline__ uint64_t edge_below(const EdgeTest<T> &et, T d2, const LaneData<T> &L, const SrcRec<T> &q)
 {
     if constexpr (sizeof(T) == 4) {
-        bool below = d2 < et.lo;
-        const bool amb = !below && d2 <= et.hi;
-        if (__ballot(amb)) {
-            // keeps this rare path a branch (not evaluated for every pair)
-            asm volatile("" ::: "memory");
-            if (amb) {
-                // opaque copies: the FP64 lens coordinates are formed here,
-                // not hoisted out of the loop into four long-lived registers
-                float xa = L.xa, ya = L.ya;
-                asm volatile("" : "+v"(xa), "+v"(ya));
-                const double ddx = (double)xa - (double)q.x, ddy = (double)ya - (double)q.y;
-                below = fma(ddx, ddx, ddy * ddy) < et.ed;
-            }
+        uint64_t below = __builtin_amdgcn_ballot_w64(d2 < et.lo);
+        const uint64_t amb = __builtin_amdgcn_ballot_w64(d2 <= et.hi) & ~below;
+        if (amb) {
+            // opaque copies: the FP64 lens coordinates are formed here,
+            // not hoisted out of the loop into four long-lived registers
+            float xa = L.xa, ya = L.ya;
+            asm volatile("" : "+v"(xa), "+v"(ya));
+            const double ddx = (double)xa - (double)q.x, ddy = (double)ya - (double)q.y;
+            below |= __builtin_amdgcn_ballot_w64(fma(ddx, ddx, ddy * ddy) < et.ed) & amb;
         }
         return below;
     } else {
-        return d2 < et.ed;
+        return __builtin_amdgcn_ballot_w64(d2 < et.ed);
     }
 }
 
@@ -459,7 +457,8 @@ __device__ __forceinline__ bool edge_below(const EdgeTest<T> &et, T d2, const La
 // large next to a side that holds few of the segment's pairs.  The two sides
 // are two exec-masked branches (empty asm statements keep them from being
 // turned into selects), so each pair issues one side's instructions.  Pair
-// counts come from wave ballots.  GUARD: the chunk and segment
+// counts are per lane, added in the upper side's branch, and summed over the
+// valid lanes after the segment.  GUARD: the chunk and segment
 // boxes overlap, so a pair may sit at zero separation (its per-pair values are
 // forced finite); TRACK0: shear with x_min == 0, where such a pair is in bin 0
 // (reference: NaN).  MZ: all m == 0, so K == W.
@@ -470,13 +469,11 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
     constexpr int WQ = win_q<MZ>();
     const EdgeTest<T> et(A, w.wk + 1);
     T lT = 0, hT = 0, lW = 0, hW = 0, lK = 0, hK = 0, lG1 = 0, lG2 = 0, hG1 = 0, hG2 = 0, sZ = 0;
-    uint32_t n_hi = 0;   // pairs at or above the edge (wave total, scalar)
+    uint32_t hN = 0;   // this lane's pairs at or above the edge
     stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
         const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
         const T d2 = g.d2;
-        const bool below = edge_below<T>(et, d2, L, q);
-        // the count of the upper bin is a wave total from the lane mask
-        n_hi += __popcll(__builtin_amdgcn_ballot_w64(!below) & w.valid);
+        const bool below = __builtin_amdgcn_inverse_ballot_w64(edge_below<T>(et, d2, L, q));
         T d, inv, u, v;
         pair_proj<T, SPH, GUARD>(L, g, d, inv, u, v);
         // the shear terms before the branches (both sides use them)
@@ -499,6 +496,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
             }
             asm volatile("");
         } else {
+            ++hN;
             hT = fma(d, q.w, hT);
             hW += q.w;
             if (!MZ) hK += q.wk;
@@ -526,6 +524,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         wacc(w, WQ + QK, lane) += (double)hK;
     }
     if (KIND == SHEAR && TRACK0) wacc(w, 2 * WQ, lane) += (double)sZ;
+    const uint32_t n_hi = wave_sum_u32(L.valid ? hN : 0u);   // the upper bin's pairs (a lens' lane must be valid)
     tab_count(w, lane, w.wk, n_bin, (uint64_t)count * w.n_valid - n_hi);
     tab_count(w, lane, w.wk + 1, n_bin, n_hi);
     w.used0 = w.used1 = true;
@@ -547,17 +546,17 @@ __device__ __forceinline__ void edge_side_pass(Win &w, const LaneData<T> &L, con
     constexpr int WQ = win_q<MZ>();
     const EdgeTest<T> et(A, w.wk + 1);
     T sT = 0, sG1 = 0, sG2 = 0, sW = 0, sK = 0;
-    uint32_t n_in = 0;   // in-range pairs (wave total, scalar)
+    uint32_t nIn = 0;   // this lane's in-range pairs
     stream2<T>(r, count, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
         const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
-        const bool below = edge_below<T>(et, g.d2, L, q);
-        const bool in = UPPER ? !below : below;
-        n_in += __popcll(__builtin_amdgcn_ballot_w64(in) & w.valid);
+        const uint64_t below = edge_below<T>(et, g.d2, L, q);
+        const bool in = __builtin_amdgcn_inverse_ballot_w64(UPPER ? ~below : below);
         T d, inv, u, v;
         pair_proj<T, SPH, false>(L, g, d, inv, u, v);
         // an exec-masked branch (kept one by the empty asm): out-of-range
         // lanes, with their non-finite values at zero separation, add nothing
         if (in) {
+            ++nIn;
             sT = fma(d, q.w, sT);
             sW += q.w;
             if (!MZ) sK += q.wk;
@@ -578,7 +577,7 @@ __device__ __forceinline__ void edge_side_pass(Win &w, const LaneData<T> &L, con
     wacc(w, o + QG2, lane) += (double)sG2;
     wacc(w, o + QW, lane) += (double)sW;
     if (!MZ) wacc(w, o + QK, lane) += (double)sK;
-    tab_count(w, lane, UPPER ? w.wk + 1 : w.wk, n_bin, n_in);
+    tab_count(w, lane, UPPER ? w.wk + 1 : w.wk, n_bin, wave_sum_u32(L.valid ? nIn : 0u));
     if (UPPER) w.used1 = true;
     else w.used0 = true;
     w.edge = true;
@@ -618,18 +617,6 @@ constexpr int wave_lds_doubles()
     constexpr int win = (2 * win_q<MZ>() + 1) * kWave;
     constexpr int rows = (multi_row_bytes<T, MZ>() + 7) / 8;
     return win > rows ? win : rows;
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
-{
-    int x = (int)v;
-    x += dpp_i<0x111>(x);
-    x += dpp_i<0x112>(x);
-    x += dpp_i<0x114>(x);
-    x += dpp_i<0x118>(x);
-    x += dpp_i<0x142, 0xa, false>(x);
-    x += dpp_i<0x143, 0xc, false>(x);
-    return (uint32_t)__builtin_amdgcn_readlane(x, 63);
 }
 
 // One sweep over a segment for bins k0 .. k0 + S - 1 (edges clamped at
