@@ -207,7 +207,7 @@ struct tpcf_ctx {
     // x_max are within 0.5 % of each other there, and 0.2-0.25 are the best
     // on the FP64 1M x 2M workload
     double opt_cell_target = 64.0;
-    double opt_split = 0.25;      // work-item band height in units of x_max
+    double opt_split = 0.2;       // work-item band height in units of x_max
     bool opt_counters = false;
     tpcf_stats stats{};
     std::string err;

@@ -594,20 +594,26 @@ __device__ __forceinline__ void edge_side_pass(Win &w, const LaneData<T> &L, con
 // fraction of the rate of a b128 read and write), and the pair count with an
 // integer LDS add; with K kept, a float2 (K, pair count) per bin and lane
 // instead of the integer count.  FP64: rows[q][bin][lane] per quantity with
-// FP64 LDS adds, and the u32 counts.  The rows overlay the wave's window sums
-// (a window is flushed before a multi segment), and a sweep's rows hold at
-// most kSegMax terms per lane and bin, like the other passes' FP32 sums.
-// Bins per sweep: FP32 4 (3 with K kept), FP64 2, which fit the LDS the
+// FP64 LDS adds, and the integer counts.  The integer counts of bins 2 i and
+// 2 i + 1 share a word (16 bits each: at most kSegMax < 2^16 pairs per lane
+// and bin in a sweep).  The rows overlay the wave's window sums (a window is
+// flushed before a multi segment), and a sweep's rows hold at most kSegMax
+// terms per lane and bin, like the other passes' FP32 sums.  Bins per sweep:
+// FP32 4 (3 with K kept), FP64 3 (2 with K kept), which fit the LDS the
 // kernel's occupancy allows.
 template <typename T, bool MZ>
-constexpr int multi_bins() { return sizeof(T) == 4 ? (MZ ? 4 : 3) : 2; }
+constexpr int multi_bins() { return sizeof(T) == 4 ? (MZ ? 4 : 3) : (MZ ? 3 : 2); }
+
+// the integer counts are packed two bins to a word (all but FP32 with K kept)
+template <typename T, bool MZ>
+constexpr bool multi_packed() { return !(sizeof(T) == 4 && !MZ); }
 
 template <typename T, bool MZ>
 constexpr int multi_row_bytes()
 {
-    constexpr int S = multi_bins<T, MZ>();
-    if constexpr (sizeof(T) == 4) return S * kWave * (MZ ? 16 : 24) + (MZ ? S / 2 * kWave * 4 : 0);
-    else return S * kWave * (win_q<MZ>() * 8 + 4);
+    constexpr int S = multi_bins<T, MZ>(), words = (S + 1) / 2;
+    if constexpr (sizeof(T) == 4) return S * kWave * (MZ ? 16 : 24) + (MZ ? words * kWave * 4 : 0);
+    else return S * kWave * win_q<MZ>() * 8 + words * kWave * 4;
 }
 
 // 8-byte units of a wave's LDS block: the FP64 window sums or the multi-pass rows
@@ -632,10 +638,10 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
 {
     constexpr int S = multi_bins<T, MZ>(), QR = win_q<MZ>();
     const int lane = L.lane;
-    // FP32 rows: float4 r4[S][kWave], then u32 cnt[S / 2][kWave] holding the
-    // counts of bins 2 i and 2 i + 1 in its low and high 16 bits (MZ) or
+    // FP32 rows: float4 r4[S][kWave], then u32 cnt[(S + 1) / 2][kWave] holding
+    // the counts of bins 2 i and 2 i + 1 in its low and high 16 bits (MZ) or
     // float2 r2[S][kWave] = (K, count); FP64 rows: T rows[QR][S][kWave], then
-    // u32 cnt[S][kWave]
+    // the packed counts
     float4 *const r4 = reinterpret_cast<float4 *>(w.acc);
     float2 *const r2 = reinterpret_cast<float2 *>(r4 + S * kWave);
     T *const rows = reinterpret_cast<T *>(w.acc);
@@ -652,8 +658,20 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
 #pragma unroll
         for (int i = 0; i < QR * S; ++i) rows[i * kWave + lane] = (T)0;
 #pragma unroll
-        for (int i = 0; i < S; ++i) cnt[i * kWave + lane] = 0u;
+        for (int i = 0; i < S; i += 2) cnt[i / 2 * kWave + lane] = 0u;
     }
+    // bin k counts in half k % 2 of word k / 2; the edge masks are nested
+    // (ge[i + 1] within ge[i]), so k >= 2 is ge[2] and k is odd where the
+    // exclusive or of ge[1] .. ge[S - 1] holds
+    auto count_pair = [&](const uint64_t *ge) __attribute__((always_inline)) {
+        static_assert(S >= 2 && S <= 4, "two count words");
+        const bool up = S > 2 && __builtin_amdgcn_inverse_ballot_w64(ge[2 < S ? 2 : 1]);
+        uint64_t par = 0;
+#pragma unroll
+        for (int i = 1; i < S; ++i) par ^= ge[i];
+        const bool odd = __builtin_amdgcn_inverse_ballot_w64(par);
+        atomicAdd(&cnt[(up ? kWave : 0) + lane], odd ? 0x10000u : 1u);
+    };
     // the sweep's edges (FP32: the band [lo, hi] around each; FP64: the edge)
     T lo[S + 1], hi[S + 1];
 #pragma unroll
@@ -737,14 +755,7 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
                 a.w += q.w;
                 r4[at] = a;
                 if constexpr (MZ) {
-                    // bin k counts in half k % 2 of word k / 2; the edge masks
-                    // are nested (ge[i + 1] within ge[i]), so k >= 2 is ge[2]
-                    // and k is odd where ge[1] ^ ge[2] ^ ge[3] holds (at most
-                    // kSegMax < 2^16 pairs per lane and bin in a sweep)
-                    static_assert(S == 4, "two count words of two bins");
-                    const bool up = __builtin_amdgcn_inverse_ballot_w64(ge[2]);
-                    const bool odd = __builtin_amdgcn_inverse_ballot_w64(ge[1] ^ ge[2] ^ ge[3]);
-                    atomicAdd(&cnt[(up ? kWave : 0) + lane], odd ? 0x10000u : 1u);
+                    count_pair(ge);
                 } else {
                     float2 b = r2[at];
                     b.x += q.wk;
@@ -758,7 +769,7 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
                 atomicAdd(p + QG2 * S * kWave, t2);
                 atomicAdd(p + QW * S * kWave, q.w);
                 if constexpr (!MZ) atomicAdd(p + QK * S * kWave, q.wk);
-                atomicAdd(&cnt[at], 1u);
+                count_pair(ge);
             }
         }
     });
@@ -806,7 +817,7 @@ __device__ __forceinline__ void multi_flush(Win &w, const LaneData<T> &L, int n_
             aG2 = rows[QG2 * S * kWave + at];
             aW = rows[QW * S * kWave + at];
             aK = MZ ? aW : (double)rows[QK * S * kWave + at];
-            nl = cnt[at];
+            nl = (cnt[s / 2 * kWave + lane] >> (16 * (s % 2))) & 0xffffu;
         }
         if (!__ballot(nl != 0u)) continue;   // no pair of the chunk in this bin
         const uint32_t n = wave_sum_u32(nl);
