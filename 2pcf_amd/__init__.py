@@ -10,10 +10,10 @@ from ._native import TpcfError  # noqa: F401
 from .tpcf import (IHS, KAPPA, LIN, LOG, SHEAR, SQRT, Batch, Binning, Context, KappaMap, LensCatalog,  # noqa: F401
                    LensingSurveyField, Shear2pcfResults, ShapeCatalog, Tape, adjoint, adjoint_record, adjoint_replay,
                    block_sums, bootstrap, correlate, create_bins, default_context, find_bin_index, kappa_2pcf, kmeans,
-                   shear_2pcf, stack_fields)
+                   rect_blocks, shear_2pcf, stack_fields)
 
 __all__ = [
     "TpcfError", "Batch", "Binning", "Context", "KappaMap", "LensCatalog", "LensingSurveyField", "Shear2pcfResults",
     "ShapeCatalog", "Tape", "adjoint", "adjoint_record", "adjoint_replay", "block_sums", "bootstrap", "correlate", "create_bins", "default_context",
-    "find_bin_index", "kappa_2pcf", "kmeans", "shear_2pcf", "stack_fields", "LIN", "LOG", "IHS", "SQRT", "SHEAR", "KAPPA",
+    "find_bin_index", "kappa_2pcf", "kmeans", "rect_blocks", "shear_2pcf", "stack_fields", "LIN", "LOG", "IHS", "SQRT", "SHEAR", "KAPPA",
 ]

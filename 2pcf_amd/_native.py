@@ -46,6 +46,7 @@ EXPORTS = (
     "tpcf_bootstrap", "tpcf_kmeans_f32", "tpcf_kmeans_f64",
     "tpcf_adjoint_record_shear_f32", "tpcf_adjoint_record_shear_f64", "tpcf_adjoint_record_kappa_f32",
     "tpcf_adjoint_record_kappa_f64", "tpcf_adjoint_replay", "tpcf_tape_destroy", "tpcf_shear_f32_noalloc",
+    "tpcf_rect_blocks_f32", "tpcf_rect_blocks_f64",
 )
 
 
@@ -213,6 +214,10 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [_vp, _i64, _vp, _vp, _vp, C.POINTER(KmeansOpts), C.c_uint32, _vp, _vp, _vp,
                       C.POINTER(_i32), C.POINTER(_i64)]
+        f.restype = C.c_int
+    for name in ("tpcf_rect_blocks_f32", "tpcf_rect_blocks_f64"):
+        f = getattr(L, name)
+        f.argtypes = [_vp, _i64, _vp, _vp, _i32, _i32, _i32, C.c_uint32, _vp]
         f.restype = C.c_int
     _lib = L
     return L

@@ -13,7 +13,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "lib2pcf_amd.so")
 SOURCES = ["api.hip", "prepare.hip", "pairs.hip", "adjoint.hip", "batch.hip", "sphere.hip", "bootstrap.hip", "kmeans.hip",
-           "noalloc.hip"]
+           "noalloc.hip", "blocks.hip"]
 ARCH = os.environ.get("TPCF_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

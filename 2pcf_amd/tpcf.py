@@ -496,13 +496,47 @@ def block_sums(res: "Shear2pcfResults", patch_A, w_A, n_patch_A: int) -> dict:
     outputs: G_t, G_x, K [n_block][n_bin] (the patch-pair sums over source
     patches, which equal the block sums of the marks, src/tpcf.f90:1130-1137)
     and the block weights (sum of lens w).  Empty blocks (weight 0) are
-    dropped, as src/tpcf.f90:1054-1062 does."""
+    dropped, and a block's G_t, G_x are set to 0 in the bins where its K is 0,
+    as src/tpcf.f90:1054-1069 does."""
     gt = np.asarray(res.pp_gt).sum(axis=0)
     gx = np.asarray(res.pp_gx).sum(axis=0)
     k = np.asarray(res.pp_k).sum(axis=0)
     w = np.bincount(np.asarray(patch_A), weights=np.asarray(w_A, dtype=np.float64), minlength=n_patch_A)
     keep = w > 0
-    return dict(g_t=gt[keep], g_x=gx[keep], k=k[keep], weight=w[keep])
+    gt, gx, k = gt[keep], gx[keep], k[keep]
+    gt = np.where(k == 0, 0.0, gt)
+    gx = np.where(k == 0, 0.0, gx)
+    return dict(g_t=gt, g_x=gx, k=k, weight=w[keep])
+
+
+def rect_blocks(x, y, n_x_blocks: int, n_y_blocks: int, *, offset: int = 0, device_inputs: bool = False,
+                labels=None, n: Optional[int] = None, precision: Optional[str] = None,
+                ctx: Optional[Context] = None):
+    """Rectangular bootstrap block labels of one field on the GPU
+    (tpcf_rect_blocks_*): the reference's default block scheme,
+    src/tpcf.f90:1017-1049.  0-based labels in [offset, offset + n_x_blocks
+    n_y_blocks).  Host arrays in, an int32 array out; with ``device_inputs``
+    x, y and ``labels`` are device addresses (``n`` objects, ``precision``
+    "f32" or "f64")."""
+    ctx = ctx or default_context()
+    if device_inputs:
+        if labels is None or n is None or precision is None:
+            raise ValueError("device_inputs needs labels, n and precision")
+        fn = N.lib().tpcf_rect_blocks_f32 if precision == "f32" else N.lib().tpcf_rect_blocks_f64
+        rc = fn(ctx.handle, int(n), x, y, int(n_x_blocks), int(n_y_blocks), int(offset), N.FLAG_DEVICE_INPUTS, labels)
+        if rc:
+            raise TpcfError(rc, ctx.last_error())
+        return labels
+    prec = precision or _precision_of(x)
+    dt = np.float32 if prec == "f32" else np.float64
+    xs, ys = _host(x, dt), _host(y, dt)
+    out = np.zeros(len(xs), dtype=np.int32)
+    fn = N.lib().tpcf_rect_blocks_f32 if prec == "f32" else N.lib().tpcf_rect_blocks_f64
+    rc = fn(ctx.handle, len(xs), xs.ctypes.data, ys.ctypes.data, int(n_x_blocks), int(n_y_blocks), int(offset), 0,
+            out.ctypes.data)
+    if rc:
+        raise TpcfError(rc, ctx.last_error())
+    return out
 
 
 def bootstrap(blocks: dict, n_boot: int, *, multiplicity=None, seed: int = 0, supersampling: float = 1.0,

@@ -421,6 +421,25 @@ int tpcf_kmeans_f64(tpcf_ctx *ctx, int64_t n, const double *x, const double *y, 
                     const tpcf_kmeans_opts *opts, uint32_t flags, int32_t *labels, double *centroids,
                     double *cluster_weights, int32_t *rounds, int64_t *last_moves);
 
+/* ---- rectangular bootstrap blocks ------------------------------------------
+ * Replaces the block assignment of the rectangular_blocks scheme of
+ * survey_marked_point_block_bootstrap (src/tpcf.f90:1017-1049), the
+ * reference's default bootstrap scheme (src/main.f90:100), for one field: the
+ * bounding box of the field's n objects is cut into n_x_blocks x n_y_blocks
+ * linear bins (create_bins / find_bin_index in single precision,
+ * src/utils.f90:101-230), and object j gets the 0-based label
+ * (iy - 1) * n_y_blocks + ix - 1 + offset, ix and iy its 1-based bins (the
+ * reference's row stride is n_y_blocks).  offset: the labels of the fields
+ * before this one (the reference adds n_x_blocks n_y_blocks per field).
+ * Double coordinates are rounded to single precision first.  A label outside
+ * [offset, offset + n_x_blocks n_y_blocks) (n_x_blocks < n_y_blocks, where the
+ * reference writes past the field's blocks) returns TPCF_ERR_ARG.  Flags:
+ * TPCF_DEVICE_INPUTS (x, y and labels are device arrays). */
+int tpcf_rect_blocks_f32(tpcf_ctx *ctx, int64_t n, const float *x, const float *y, int32_t n_x_blocks,
+                         int32_t n_y_blocks, int32_t offset, uint32_t flags, int32_t *labels);
+int tpcf_rect_blocks_f64(tpcf_ctx *ctx, int64_t n, const double *x, const double *y, int32_t n_x_blocks,
+                         int32_t n_y_blocks, int32_t offset, uint32_t flags, int32_t *labels);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,9 @@ Reference behaviour restated (tilmantroester/2pcf):
                       visited, FP64 geometry, long-double sums
   * ``finalize``   -- src/tpcf.f90:369-376 / :536-543
   * ``ref_port``   -- the reference algorithm (kd-tree, FP32, OpenMP) for timing
+  * ``rect_block_labels`` / ``survey_block_sums`` -- the rectangular block
+                      scheme of survey_marked_point_block_bootstrap,
+                      src/tpcf.f90:1017-1069 (labels through rp_find_bin)
   * ``reference``  -- the reference's own tangential_shear_tpcf /
                       scalar_kappa_tpcf, compiled by ``build_ref.sh`` into
                       ``_ref/libref2pcf.so`` (present only where the reference
@@ -401,6 +404,43 @@ def reference(kind: int, lens: dict, src: dict, x_min, x_max, n_bin, spacing, ih
             os.dup2(saved, 1)
             os.close(saved)
     return res
+
+
+def rect_block_labels(x, y, n_x_blocks: int, n_y_blocks: int, offset: int = 0) -> np.ndarray:
+    """Block labels of one field in the rectangular_blocks scheme
+    (src/tpcf.f90:1029-1049), 0-based: lin bins over the field's single
+    precision bounding box (create_bins, src/utils.f90:139-230), find_bin_index
+    through rp_find_bin (src/utils.f90:101-137), and the reference's index
+    (iy - 1) * n_y_blocks + ix + offset minus one."""
+    xs = np.asarray(x, dtype=np.float32)
+    ys = np.asarray(y, dtype=np.float32)
+    ix = rp_find_bin(xs, float(xs.min()), float(xs.max()), int(n_x_blocks), 0).astype(np.int64)
+    iy = rp_find_bin(ys, float(ys.min()), float(ys.max()), int(n_y_blocks), 0).astype(np.int64)
+    return (iy - 1) * int(n_y_blocks) + ix - 1 + int(offset)
+
+
+def survey_block_sums(labels, w, xi_e, xi_b, k_marks, n_blocks: int) -> dict:
+    """Block sums of survey_marked_point_block_bootstrap
+    (src/tpcf.f90:1043-1069) from per-object marks [n][n_bin] and 0-based
+    labels: sums in object order, then the empty blocks (weight 0) dropped
+    and G_t, G_x set to 0 where a block's K is 0."""
+    lab = np.asarray(labels, dtype=np.int64)
+    nb = np.asarray(xi_e).shape[1]
+    bw = np.zeros(n_blocks)
+    bt = np.zeros((n_blocks, nb))
+    bx = np.zeros((n_blocks, nb))
+    bk = np.zeros((n_blocks, nb))
+    for j in range(len(lab)):
+        b = lab[j]
+        bw[b] += w[j]
+        bt[b] += xi_e[j]
+        bx[b] += xi_b[j]
+        bk[b] += k_marks[j]
+    keep = bw > 0
+    bt, bx, bk = bt[keep], bx[keep], bk[keep]
+    bt[bk == 0] = 0.0
+    bx[bk == 0] = 0.0
+    return dict(g_t=bt, g_x=bx, k=bk, weight=bw[keep])
 
 
 def bootstrap(blocks: dict, multiplicity: np.ndarray) -> dict:
