@@ -11,7 +11,9 @@ replays it (tpcf_adjoint_replay: the source-side pass only), so a forward and
 a backward cost two pair passes.  Backward returns gradients for val and w of
 the lenses and for the value components, w (and m) of the sources; positions
 and z get none.  The same tensor may be passed on both sides
-(auto-correlation): autograd adds the two contributions.
+(auto-correlation): autograd adds the two contributions.  With
+``spherical=True`` x and y are longitude and latitude and the bin edges angles,
+all in radians (the shear angle is the one at the lens, as in the forward).
 """
 from __future__ import annotations
 
@@ -35,7 +37,7 @@ def _prec(t: torch.Tensor) -> str:
 
 class _Correlation(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, kind, bins, tctx, *tensors):
+    def forward(ctx, kind, bins, tctx, spherical, *tensors):
         lk, sk = _LENS[kind], _SRC[kind]
         lens = dict(zip(lk, tensors[:len(lk)]))
         src = dict(zip(sk, tensors[len(lk):]))
@@ -46,7 +48,7 @@ class _Correlation(torch.autograd.Function):
         torch.cuda.current_stream().synchronize()
         g1, g2, tape = T.adjoint_record(kind, {k: v.data_ptr() for k, v in lens.items()},
                                         {k: v.data_ptr() for k, v in src.items()}, bins, precision=prec, ctx=tctx,
-                                        n_A=lens["x"].numel(), n_B=src["x"].numel())
+                                        n_A=lens["x"].numel(), n_B=src["x"].numel(), spherical=spherical)
         ctx.kind, ctx.bins, ctx.tape = kind, bins, tape
         # keeps the inputs alive (the tape holds their addresses) and detects
         # their modification in place before backward
@@ -61,7 +63,7 @@ class _Correlation(torch.autograd.Function):
         lk, sk = _LENS[kind], _SRC[kind]
         lens = dict(zip(lk, tensors[:len(lk)]))
         src = dict(zip(sk, tensors[len(lk):]))
-        needs = ctx.needs_input_grad[3:]
+        needs = ctx.needs_input_grad[4:]
         need = {("lens", k): needs[i] for i, k in enumerate(lk)}
         need.update({("src", k): needs[len(lk) + i] for i, k in enumerate(sk)})
         dev = lens["x"].device
@@ -82,21 +84,21 @@ class _Correlation(torch.autograd.Function):
             for k in keys:
                 b = bufs.get((side, k))
                 out.append(b.to(tensors_[k].dtype) if b is not None else None)
-        return (None, None, None, *out)
+        return (None, None, None, None, *out)
 
 
-def shear_scalar(lens: dict, src: dict, bins, ctx=None):
+def shear_scalar(lens: dict, src: dict, bins, ctx=None, spherical: bool = False):
     """(g_t, g_x) per bin of tangential_shear_tpcf (src/tpcf.f90:190-378), differentiable."""
     ctx = ctx or T.default_context()
     args = [lens[k] for k in _LENS[T.SHEAR]] + [src[k] for k in _SRC[T.SHEAR]]
-    return _Correlation.apply(T.SHEAR, bins, ctx, *args)
+    return _Correlation.apply(T.SHEAR, bins, ctx, bool(spherical), *args)
 
 
-def kappa_kappa(lens: dict, src: dict, bins, ctx=None):
+def kappa_kappa(lens: dict, src: dict, bins, ctx=None, spherical: bool = False):
     """(xi_E, xi_B) per bin of scalar_kappa_tpcf (src/tpcf.f90:380-545), differentiable."""
     ctx = ctx or T.default_context()
     args = [lens[k] for k in _LENS[T.KAPPA]] + [src[k] for k in _SRC[T.KAPPA]]
-    return _Correlation.apply(T.KAPPA, bins, ctx, *args)
+    return _Correlation.apply(T.KAPPA, bins, ctx, bool(spherical), *args)
 
 
 def chi2(d: torch.Tensor, model: torch.Tensor, sigma: torch.Tensor) -> torch.Tensor:

@@ -177,21 +177,42 @@ int unit_arrays(tpcf_ctx *ctx, int64_t n, T **ones, T **zeros)
     return TPCF_OK;
 }
 
+// The geometry the passes run on.  Flat sky: the inputs themselves.  Sphere:
+// the FP64 projected positions and values of sph_setup (one basis for both
+// passes), the chord bins, and the sources' frames for the swapped pass.  The
+// per-object contractions read the caller's values in both cases.
+struct SphGeo {
+    tpcf_bins kb{};
+    CallArgs<double> d;
+    const double *rev_frames = nullptr;
+};
+
+template <typename T>
+int sph_geo(tpcf_ctx *ctx, const CallArgs<T> &base, SphGeo &g)
+{
+    TPCF_TRY(sph_chord_bins(ctx, base.bins, g.kb));
+    TPCF_TRY(sph_setup<T>(ctx, base, &g.kb, g.d, nullptr));
+    TPCF_TRY(sph_rev_frames(ctx, g.d, &g.rev_frames));
+    return TPCF_OK;
+}
+
 // Lens side: unit lens values and weights, real sources.  The marks stay in
 // the context's marks buffer and the lens order in its preparation buffers
 // (both valid until the next pass on this context); the forward sums are
-// contracted from the marks with a_i and w_i.
-template <typename T>
-int lens_pass(tpcf_ctx *ctx, const CallArgs<T> &base, uint32_t run_flags, double t0, LensSide &ls)
+// contracted from the marks with a_i and w_i.  geo: the pass's geometry (U = T
+// on the flat sky, FP64 on the sphere).
+template <typename T, typename U>
+int lens_pass(tpcf_ctx *ctx, const CallArgs<T> &base, const CallArgs<U> &geo, uint32_t run_flags, double t0,
+              LensSide &ls)
 {
     hipStream_t s = ctx->stream;
     const int n_bin = base.bins->n_bin;
     ls.fwd.assign(3 * (size_t)n_bin, 0.0);
     ls.n_lens = 0;
     if (base.n_a == 0 || base.n_b == 0) return TPCF_OK;
-    T *ones, *zeros;
-    TPCF_TRY(unit_arrays<T>(ctx, std::max(base.n_a, base.n_b), &ones, &zeros));
-    CallArgs<T> l = base;
+    U *ones, *zeros;
+    TPCF_TRY(unit_arrays<U>(ctx, std::max(base.n_a, base.n_b), &ones, &zeros));
+    CallArgs<U> l = geo;
     l.vala = ones;
     l.wa = ones;
     l.marks = true;
@@ -199,8 +220,8 @@ int lens_pass(tpcf_ctx *ctx, const CallArgs<T> &base, uint32_t run_flags, double
     tpcf_out none;
     std::memset(&none, 0, sizeof none);
     none.real_bytes = 8;
-    Prepared<T> p;
-    TPCF_TRY(run_call<T>(ctx, l, run_flags, &none, t0, &p));
+    Prepared<U> p;
+    TPCF_TRY(run_call<U>(ctx, l, run_flags, &none, t0, &p));
     ls.mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
     ls.lidx = p.lidx;
     ls.n_lens = p.n_lens;
@@ -246,35 +267,43 @@ int lens_grad(tpcf_ctx *ctx, const LensSide &ls, const CallArgs<T> &base, const 
     return TPCF_OK;
 }
 
-// Source side: the sources as lanes, the lenses streamed with (a_i, 0).
-template <typename T>
-int src_pass(tpcf_ctx *ctx, const CallArgs<T> &base, uint32_t run_flags, double t0, const Coef &c, double *d1B,
-             double *d2B, double *dwB, double *dmB)
+// Source side: the sources as lanes, the lenses streamed with (a_i, 0).  On
+// the sphere the shear angle stays the one at the lens, now the streamed
+// object (CallArgs::sph_rev, with the sources' frames rev_frames).
+template <typename T, typename U>
+int src_pass(tpcf_ctx *ctx, const CallArgs<T> &base, const CallArgs<U> &geo, const double *rev_frames,
+             uint32_t run_flags, double t0, const Coef &c, double *d1B, double *d2B, double *dwB, double *dmB)
 {
     if (base.n_a == 0 || base.n_b == 0 || !(d1B || d2B || dwB || dmB)) return TPCF_OK;
     hipStream_t s = ctx->stream;
     const bool shear = base.kind == SHEAR;
-    T *ones, *zeros;
-    TPCF_TRY(unit_arrays<T>(ctx, std::max(base.n_a, base.n_b), &ones, &zeros));
-    CallArgs<T> r = base;
-    r.n_a = base.n_b;
-    r.xa = base.xb; r.ya = base.yb; r.za = base.zb;
+    U *ones, *zeros;
+    TPCF_TRY(unit_arrays<U>(ctx, std::max(base.n_a, base.n_b), &ones, &zeros));
+    CallArgs<U> r = geo;
+    r.n_a = geo.n_b;
+    r.xa = geo.xb; r.ya = geo.yb; r.za = geo.zb;
     r.vala = ones;
     r.wa = ones;
-    r.n_b = base.n_a;
-    r.xb = base.xa; r.yb = base.ya; r.zb = base.za;
-    r.s1b = base.vala;
+    r.n_b = geo.n_a;
+    r.xb = geo.xa; r.yb = geo.ya; r.zb = geo.za;
+    r.s1b = geo.vala;
     r.s2b = zeros;
-    r.wb = base.wa;
+    r.wb = geo.wa;
     r.mb = shear ? zeros : nullptr;
     r.zsign = -1;   // the z test z_A > z_B of the original roles
     r.marks = true;
     r.marks_device = true;
+    if (geo.sph_pa) {
+        r.sph_pa = geo.sph_pb;
+        r.sph_pb = geo.sph_pa;
+        r.sph_frame = rev_frames;
+        r.sph_rev = shear;
+    }
     tpcf_out none;
     std::memset(&none, 0, sizeof none);
     none.real_bytes = 8;
-    Prepared<T> p;
-    TPCF_TRY(run_call<T>(ctx, r, run_flags, &none, t0, &p));
+    Prepared<U> p;
+    TPCF_TRY(run_call<U>(ctx, r, run_flags, &none, t0, &p));
     const double *mk = static_cast<const double *>(ctx->buf[tpcf_ctx::B_MARKS].p);
     if (p.n_lens > 0) {
         const int n_bin = base.bins->n_bin;
@@ -300,9 +329,7 @@ int adj_check(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, co
     TPCF_TRY(check_bins(ctx, bins));
     if (bins->n_bin > kGroupBins)
         return fail(ctx, TPCF_ERR_UNSUPPORTED, "the adjoint takes at most " + std::to_string(kGroupBins) + " bins");
-    if (flags & TPCF_SPHERICAL)
-        return fail(ctx, TPCF_ERR_UNSUPPORTED, "spherical coordinates are not implemented by this build");
-    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE))
+    if (flags & ~(uint32_t)(TPCF_DEVICE_INPUTS | TPCF_VERBOSE | TPCF_NO_KD_TREE | TPCF_SPHERICAL))
         return fail(ctx, TPCF_ERR_ARG, "unknown flag");
     if (n_a < 0 || n_b < 0 || n_a >= ((int64_t)1 << 31) || n_b >= ((int64_t)1 << 31))
         return fail(ctx, TPCF_ERR_ARG, "catalogue sizes must be in [0, 2^31)");
@@ -383,10 +410,20 @@ int adjoint(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, cons
         if (zero_list[q] && zero_n[q] > 0) fill_f64(s, zero_list[q], zero_n[q]);
 
     LensSide ls;
-    TPCF_TRY(lens_pass<T>(ctx, base, run_flags, t0, ls));
-    const Coef c = coef_of(ls.fwd, bins->n_bin, gbar1, gbar2);
-    TPCF_TRY(lens_grad<T>(ctx, ls, base, c, dvA, dwA));
-    TPCF_TRY(src_pass<T>(ctx, base, run_flags, t0, c, d1B, d2B, dwB, dmB));
+    Coef c;
+    if (flags & TPCF_SPHERICAL) {
+        SphGeo g;
+        TPCF_TRY(sph_geo<T>(ctx, base, g));
+        TPCF_TRY((lens_pass<T, double>(ctx, base, g.d, run_flags, t0, ls)));
+        c = coef_of(ls.fwd, bins->n_bin, gbar1, gbar2);
+        TPCF_TRY(lens_grad<T>(ctx, ls, base, c, dvA, dwA));
+        TPCF_TRY((src_pass<T, double>(ctx, base, g.d, g.rev_frames, run_flags, t0, c, d1B, d2B, dwB, dmB)));
+    } else {
+        TPCF_TRY((lens_pass<T, T>(ctx, base, base, run_flags, t0, ls)));
+        c = coef_of(ls.fwd, bins->n_bin, gbar1, gbar2);
+        TPCF_TRY(lens_grad<T>(ctx, ls, base, c, dvA, dwA));
+        TPCF_TRY((src_pass<T, T>(ctx, base, base, nullptr, run_flags, t0, c, d1B, d2B, dwB, dmB)));
+    }
     if (!dev) {
         double *dst[6] = {dval_a, dw_a, d1_b, d2_b, dw_b, shear ? dm_b : nullptr};
         for (int q = 0; q < 6; ++q)
@@ -408,6 +445,7 @@ struct tpcf_tape {
     int kind = 0;
     int real_bytes = 4;
     int device = 0;
+    bool spherical = false;
     tpcf_bins bins{};
     int64_t n_a = 0, n_b = 0;
     const void *in[12] = {};   // x_A, y_A, z_A, val_A, w_A, x_B, y_B, z_B, s1_B, s2_B, w_B, m_B
@@ -459,11 +497,19 @@ int record(tpcf_ctx *ctx, int kind, int64_t n_a, const T *xa, const T *ya, const
     t->bins = *bins;
     t->n_a = n_a;
     t->n_b = n_b;
+    t->spherical = (flags & TPCF_SPHERICAL) != 0;
     const T *in[12] = {xa, ya, za, vala, wa, xb, yb, zb, s1b, s2b, wb, mb};
     for (int q = 0; q < 12; ++q) t->in[q] = in[q];
     const CallArgs<T> base = tape_base<T>(*t);
     LensSide ls;
-    int rc = lens_pass<T>(ctx, base, flags, t0, ls);
+    int rc;
+    if (t->spherical) {
+        SphGeo g;
+        rc = sph_geo<T>(ctx, base, g);
+        if (rc == TPCF_OK) rc = lens_pass<T, double>(ctx, base, g.d, flags, t0, ls);
+    } else {
+        rc = lens_pass<T, T>(ctx, base, base, flags, t0, ls);
+    }
     if (rc == TPCF_OK && ls.n_lens > 0) {
         const size_t nm = sizeof(double) * 4 * (size_t)ls.n_lens * bins->n_bin;
         if (hipMalloc((void **)&t->mk, nm) != hipSuccess || hipMalloc((void **)&t->lidx, 4 * ls.n_lens) != hipSuccess) {
@@ -513,7 +559,14 @@ int replay(tpcf_ctx *ctx, const tpcf_tape *t, const double *gbar1, const double 
     ls.fwd = t->fwd;
     const Coef c = coef_of(t->fwd, t->bins.n_bin, gbar1, gbar2);
     TPCF_TRY(lens_grad<T>(ctx, ls, base, c, dval_a, dw_a));
-    TPCF_TRY(src_pass<T>(ctx, base, TPCF_DEVICE_INPUTS, t0, c, d1_b, d2_b, dw_b, dm_b));
+    if (t->spherical) {
+        SphGeo g;
+        TPCF_TRY(sph_geo<T>(ctx, base, g));
+        TPCF_TRY((src_pass<T, double>(ctx, base, g.d, g.rev_frames, TPCF_DEVICE_INPUTS | TPCF_SPHERICAL, t0, c, d1_b,
+                                      d2_b, dw_b, dm_b)));
+    } else {
+        TPCF_TRY((src_pass<T, T>(ctx, base, base, nullptr, TPCF_DEVICE_INPUTS, t0, c, d1_b, d2_b, dw_b, dm_b)));
+    }
     TPCF_HIP(ctx, hipStreamSynchronize(s));
     ctx->stats.ms_total = now_ms() - t0;
     return TPCF_OK;

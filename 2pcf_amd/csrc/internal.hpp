@@ -228,7 +228,7 @@ struct tpcf_ctx {
         B_BLK, B_BLKCH, B_WORK, B_ITEMS,
         B_GRAD_A, B_GRAD_B, B_ONES, B_ZEROS,
         // spherical geometry: FP64 inputs, projected coordinates, lens frames, sorted extras
-        B_SPH_IN, B_SPH_LP, B_SPH_LF, B_SPH_SEGP, B_SPH_CHP, B_SPH_PART,
+        B_SPH_IN, B_SPH_LP, B_SPH_LF, B_SPH_SEGP, B_SPH_CHP, B_SPH_PART, B_SPH_RF,
         B_BOOT, B_KMEANS,
         B_COUNT
     };
@@ -280,6 +280,12 @@ struct CallArgs {
     // (the angles) give the outputs.  kbins == bins on the flat sky.
     const tpcf_bins *kbins = nullptr;
     const double *sph_pa = nullptr, *sph_pb = nullptr, *sph_frame = nullptr;
+    // Shear on the sphere with the catalogues' roles swapped (the adjoint's
+    // source-side pass): sph_frame then holds (u x P, P) of the lane objects,
+    // P the pole in the rotated basis, and the angle is taken at the streamed
+    // object.
+    bool sph_rev = false;
+    double sph_pole[3] = {0.0, 0.0, 1.0};   // the pole (latitude pi/2) in the rotated basis
 };
 
 // Device-side result of the preparation stage.
@@ -312,6 +318,7 @@ struct Prepared {
     // (east, north), per segment and per chunk (min, max) of p
     const double *sph_lp, *sph_lf;
     const double2 *sph_segp, *sph_chp;
+    bool sph_rev;                // angles at the streamed object (CallArgs::sph_rev)
 };
 
 template <typename T>
@@ -368,6 +375,7 @@ struct CatArgs {
     int64_t marks_n;                // n_lens, the stride of the marks planes
     const double *sph_lp, *sph_lf;  // spherical geometry (else NULL)
     const double2 *sph_segp, *sph_chp;
+    bool sph_rev;                   // angles at the streamed object
 };
 
 
@@ -423,6 +431,10 @@ template <typename T>
 int sph_setup(tpcf_ctx *ctx, const CallArgs<T> &a, const tpcf_bins *kb, CallArgs<double> &d,
               std::vector<F64Copy> *copies);
 void launch_to_f64(hipStream_t s, const float *in, double *out, int64_t n);
+// Frames of the sources of d for a pass with the roles swapped (CallArgs::
+// sph_rev): (u_B x P, P) per source [n_b][6] in the rotated basis, into the
+// context's buffer B_SPH_RF.
+int sph_rev_frames(tpcf_ctx *ctx, const CallArgs<double> &d, const double **frames);
 // The sources' p into record slot s1w2 (after a refresh_values re-gather).
 void sph_records_p(hipStream_t s, const Prepared<double> &p, const double *pb);
 

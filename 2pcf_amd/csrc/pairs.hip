@@ -240,12 +240,21 @@ struct LaneData {
 // lens, u = E^2 - N^2, v = E N, inv = 1/(E^2 + N^2) = 1/sin^2 d, which are the
 // reference's cos 2phi and sin 2phi from angles_sphere (src/utils.f90:618-639)
 // without trigonometry; the separation angle is 2 asin(chord / 2).
-template <typename T, bool SPH>
+// SPH == 2 is the sphere with the roles of the catalogues swapped (the
+// adjoint's source-side pass: the lanes hold sources, the stream holds
+// lenses) while the angle stays the one at the lens, i.e. at the streamed
+// object A.  With P the pole and D = u_A - u_B, the direction at A toward B
+// scaled by cos(lat_A) > 0 is E' = D . (u_B x P), N' = -P . D + (P . u_A) |D|^2 / 2
+// (from e_A = P x u_A / cos lat_A, n_A = u_A x e_A); the lane's frame slots
+// hold u_B x P and P, and pu = P . u_A.  The common factor drops out of u, v
+// and inv.
+template <typename T, int SPH>
 struct PairGeo {
     T a, b, c, d2;
+    T pu;   // SPH == 2: P . u_A of the streamed object
 };
 
-template <typename T, bool SPH>
+template <typename T, int SPH>
 __device__ __forceinline__ PairGeo<T, SPH> pair_geo(const LaneData<T> &L, const SrcRec<T> &q)
 {
     PairGeo<T, SPH> g;
@@ -259,6 +268,7 @@ __device__ __forceinline__ PairGeo<T, SPH> pair_geo(const LaneData<T> &L, const 
         g.b = q.x - L.xa;
         g.c = q.y - L.ya;
         g.d2 = fma(g.a, g.a, fma(g.b, g.b, g.c * g.c));
+        if constexpr (SPH == 2) g.pu = fma(L.fn0, q.s1w2, fma(L.fn1, q.x, L.fn2 * q.y));
     }
     return g;
 }
@@ -304,7 +314,7 @@ __device__ __forceinline__ double sph_angle(double d2)
 
 // Separation, 1/d^2 (or 1/sin^2 d) and the projection terms of one pair; d
 // and inv are 0 at zero separation (and at an antipodal pair on the sphere).
-template <typename T, bool SPH, bool GUARD>
+template <typename T, int SPH, bool GUARD>
 __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T, SPH> &g, T &d, T &inv, T &u, T &v)
 {
     if constexpr (!SPH) {
@@ -317,7 +327,8 @@ __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T,
         v = g.a * g.b;
     } else {
         const T E = fma(L.fe0, g.a, fma(L.fe1, g.b, L.fe2 * g.c));
-        const T N = fma(L.fn0, g.a, fma(L.fn1, g.b, L.fn2 * g.c));
+        T N = fma(L.fn0, g.a, fma(L.fn1, g.b, L.fn2 * g.c));
+        if constexpr (SPH == 2) N = fma((T)0.5 * g.d2, g.pu, -N);
         const T s2 = fma(E, E, N * N);
         inv = (s2 > (T)0) ? rcp_pos(s2) : (T)0;
         u = fma(E, E, -(N * N));
@@ -327,7 +338,7 @@ __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T,
 }
 
 // 2 e1 w of a record (slot s1w2 holds p on the sphere)
-template <typename T, bool SPH>
+template <typename T, int SPH>
 __device__ __forceinline__ T s1w2_of(const SrcRec<T> &q)
 {
     if constexpr (SPH) return q.s1w + q.s1w;
@@ -462,7 +473,7 @@ __device__ __forceinline__ uint64_t edge_below(const EdgeTest<T> &et, T d2, cons
 // boxes overlap, so a pair may sit at zero separation (its per-pair values are
 // forced finite); TRACK0: shear with x_min == 0, where such a pair is in bin 0
 // (reference: NaN).  MZ: all m == 0, so K == W.
-template <typename T, int KIND, bool MZ, bool GUARD, bool TRACK0, bool SPH>
+template <typename T, int KIND, bool MZ, bool GUARD, bool TRACK0, int SPH>
 __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                           const SrcRec<T> *__restrict__ r, uint32_t count, int n_bin)
 {
@@ -539,7 +550,7 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
 // kernel uses it at x_max, where the edge pass would give bin n_bin - 1 as
 // the segment's G total minus the pairs beyond x_max, whose rounding can be
 // large next to a bin that holds few of the segment's pairs.
-template <typename T, int KIND, bool MZ, bool UPPER, bool SPH>
+template <typename T, int KIND, bool MZ, bool UPPER, int SPH>
 __device__ __forceinline__ void edge_side_pass(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                                const SrcRec<T> *__restrict__ r, uint32_t count, int n_bin)
 {
@@ -631,7 +642,7 @@ constexpr int wave_lds_doubles()
 // where a zero separation is a pair of bin 0: its position angle is 0/0, so
 // its G terms are NaN (the reference's G_t, G_x of bin 0 and the lens' xi
 // marks become NaN, src/tpcf.f90:343-355) and its separation is 0.
-template <typename T, int KIND, bool MZ, bool ZCHECK, bool ZSEP, bool SPH>
+template <typename T, int KIND, bool MZ, bool ZCHECK, bool ZSEP, int SPH>
 __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const PairArgs<T> &A,
                                             const SrcRec<T> *__restrict__ r, const T *__restrict__ zr,
                                             uint32_t count, int n_bin, int k0)
@@ -852,7 +863,7 @@ __device__ __forceinline__ void multi_flush(Win &w, const LaneData<T> &L, int n_
 
 // The work of one item: lens chunk item.chunk against the source segments in
 // its band of cell rows, into the per-XCD accumulator copy `slot`.
-template <typename T, int KIND, bool MARKS, bool MZ, bool SPH>
+template <typename T, int KIND, bool MARKS, bool MZ, int SPH>
 __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T> &C, const Item item, const int slot,
                                           const int lane, double *acc, const T e2_lane,
                                           unsigned long long *__restrict__ probe)
@@ -1137,7 +1148,7 @@ template <typename T, bool MZ>
 constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : (MZ ? 6 : 5); }
 // the spherical geometry (FP64) holds seven more lane values: 4 waves, up to
 // 128 VGPRs (3 waves were slower on the 20M x 20M sphere workload)
-template <typename T, bool MZ, bool SPH>
+template <typename T, bool MZ, int SPH>
 constexpr int pair_waves() { return SPH ? 4 : pair_waves_per_simd<T, MZ>(); }
 
 // One catalogue: one item per wave; workgroup b runs on XCD b % kSlots (the
@@ -1145,7 +1156,7 @@ constexpr int pair_waves() { return SPH ? 4 : pair_waves_per_simd<T, MZ>(); }
 // of that XCD's range, which the preparation ordered band by band, so the
 // waves running together on an XCD read the same source rows.  The catalogue's
 // pointers arrive as separate arguments (scalar loads on demand).
-template <typename T, int KIND, bool MARKS, bool MZ, bool SPH>
+template <typename T, int KIND, bool MARKS, bool MZ, int SPH>
 __global__ __launch_bounds__(kWave *kWavesPerBlock)
 __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     const PairArgs<T> A, const Grid g, const int32_t n_pa, const int32_t n_pb, const double margin,
@@ -1267,6 +1278,7 @@ CatArgs<T> cat_args(const Prepared<T> &p, int32_t n_pa, int32_t n_pb, double *pp
     C.sph_lf = p.sph_lf;
     C.sph_segp = p.sph_segp;
     C.sph_chp = p.sph_chp;
+    C.sph_rev = p.sph_rev;
     return C;
 }
 
@@ -1315,7 +1327,12 @@ int launch_pairs(tpcf_ctx *ctx, hipStream_t s, int kind, bool m_zero, bool marks
     if (C.sph_lp) {
         // spherical geometry: FP64 only
         if constexpr (sizeof(T) == 8) {
-            if (kind == SHEAR) {
+            if (C.sph_rev && kind == SHEAR) {
+                // the adjoint's source-side pass: marks, unit m == 0 stream
+                if (!(marks && m_zero))
+                    return fail(ctx, TPCF_ERR_UNSUPPORTED, "a swapped spherical pass computes marks with m == 0");
+                TPCF_LAUNCH(SHEAR, true, true, 2);
+            } else if (kind == SHEAR) {
                 if (m_zero) {
                     if (marks) TPCF_LAUNCH(SHEAR, true, true, true); else TPCF_LAUNCH(SHEAR, false, true, true);
                 } else {

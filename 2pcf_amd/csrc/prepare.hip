@@ -496,6 +496,7 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     const double x_max = (double)(a.kbins ? a.kbins : a.bins)->x_max;
     p.sph_lp = p.sph_lf = nullptr;
     p.sph_segp = p.sph_chp = nullptr;
+    p.sph_rev = false;
 
     // ---- bounding boxes and finiteness of both catalogues ----
     const unsigned nbs = (unsigned)std::min<int64_t>(1024, (a.n_b + kBlk - 1) / kBlk);

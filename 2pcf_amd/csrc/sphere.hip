@@ -158,6 +158,27 @@ __global__ __launch_bounds__(kSphBlk) void k_sph_chp(int64_t n, const Chunk<doub
     chp[i] = make_double2(lo, hi);
 }
 
+// Frames for a pass with the catalogues' roles swapped: (u x P, P) of each
+// object from its rotated unit vector (p, q, s); the basis is right-handed
+// (c x e1 = e2), so the cross product is taken in rotated coordinates.
+__global__ __launch_bounds__(kSphBlk) void k_sph_rev_frame(int64_t n, const double *__restrict__ p,
+                                                           const double *__restrict__ q,
+                                                           const double *__restrict__ s, const double P0,
+                                                           const double P1, const double P2,
+                                                           double *__restrict__ frame)
+{
+    const int64_t i = blockIdx.x * (int64_t)kSphBlk + threadIdx.x;
+    if (i >= n) return;
+    const double u0 = p[i], u1 = q[i], u2 = s[i];
+    double *f = frame + 6 * i;
+    f[0] = fma(u1, P2, -(u2 * P1));
+    f[1] = fma(u2, P0, -(u0 * P2));
+    f[2] = fma(u0, P1, -(u1 * P0));
+    f[3] = P0;
+    f[4] = P1;
+    f[5] = P2;
+}
+
 inline unsigned sph_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kSphBlk - 1) / kSphBlk); }
 
 void cross(const double *a, const double *b, double *o)
@@ -311,7 +332,22 @@ int sph_setup(tpcf_ctx *ctx, const CallArgs<T> &a, const tpcf_bins *kb, CallArgs
     d.sph_pa = pa;
     d.sph_pb = pbv;
     d.sph_frame = fa;
+    d.sph_pole[0] = B.c[2];
+    d.sph_pole[1] = B.e1[2];
+    d.sph_pole[2] = B.e2[2];
     TPCF_HIP(ctx, hipGetLastError());
+    return TPCF_OK;
+}
+
+int sph_rev_frames(tpcf_ctx *ctx, const CallArgs<double> &d, const double **frames)
+{
+    double *f;
+    TPCF_TRY(ensure(ctx, tpcf_ctx::B_SPH_RF, sizeof(double) * 6 * std::max<int64_t>(d.n_b, 1), (void **)&f));
+    if (d.n_b > 0)
+        hipLaunchKernelGGL(k_sph_rev_frame, dim3(sph_blocks(d.n_b)), dim3(kSphBlk), 0, ctx->stream, d.n_b, d.sph_pb,
+                           d.xb, d.yb, d.sph_pole[0], d.sph_pole[1], d.sph_pole[2], f);
+    TPCF_HIP(ctx, hipGetLastError());
+    *frames = f;
     return TPCF_OK;
 }
 
@@ -351,6 +387,7 @@ int sph_finish(tpcf_ctx *ctx, const CallArgs<double> &a, Prepared<double> &p)
     p.sph_lf = lf;
     p.sph_segp = segp;
     p.sph_chp = chp;
+    p.sph_rev = a.sph_rev;
     return TPCF_OK;
 }
 

@@ -439,28 +439,30 @@ class Tape:
 
 
 def adjoint_record(kind: int, lens: dict, src: dict, bins: Binning, *, precision: str = "f32",
-                   ctx: Optional[Context] = None, n_A: int, n_B: int):
+                   ctx: Optional[Context] = None, n_A: int, n_B: int, spherical: bool = False):
     """Forward outputs (g_t, g_x or xi_E, xi_B: FP64 arrays [n_bin]) and a
     :class:`Tape` for ``adjoint_replay``.  ``lens`` / ``src`` hold device
     addresses (ints) of arrays of the given precision, which must stay valid
-    and unchanged while the tape is replayed."""
+    and unchanged while the tape is replayed.  ``spherical``: x, y are
+    longitude and latitude and the bin edges angles, all in radians."""
     ctx = ctx or default_context()
     L = N.lib()
     p = lambda v: C.c_void_p(int(v)) if v is not None else None   # noqa: E731
     g1 = np.zeros(bins.n_bin, np.float64)
     g2 = np.zeros(bins.n_bin, np.float64)
     h = C.c_void_p()
+    flags = N.FLAG_DEVICE_INPUTS | (N.FLAG_SPHERICAL if spherical else 0)
     if kind == SHEAR:
         fn = L.tpcf_adjoint_record_shear_f32 if precision == "f32" else L.tpcf_adjoint_record_shear_f64
         rc = fn(ctx.handle, n_A, p(lens["x"]), p(lens["y"]), p(lens["z"]), p(lens["val"]), p(lens["w"]),
                 n_B, p(src["x"]), p(src["y"]), p(src["z"]), p(src["e1"]), p(src["e2"]), p(src["w"]), p(src["m"]),
-                C.byref(bins.c), N.FLAG_DEVICE_INPUTS, g1.ctypes.data_as(C.c_void_p),
+                C.byref(bins.c), flags, g1.ctypes.data_as(C.c_void_p),
                 g2.ctypes.data_as(C.c_void_p), C.byref(h))
     else:
         fn = L.tpcf_adjoint_record_kappa_f32 if precision == "f32" else L.tpcf_adjoint_record_kappa_f64
         rc = fn(ctx.handle, n_A, p(lens["x"]), p(lens["y"]), p(lens["val"]), p(lens["w"]),
                 n_B, p(src["x"]), p(src["y"]), p(src["kappa_E"]), p(src["kappa_B"]), p(src["w"]),
-                C.byref(bins.c), N.FLAG_DEVICE_INPUTS, g1.ctypes.data_as(C.c_void_p),
+                C.byref(bins.c), flags, g1.ctypes.data_as(C.c_void_p),
                 g2.ctypes.data_as(C.c_void_p), C.byref(h))
     if rc:
         raise TpcfError(rc, ctx.last_error())

@@ -60,8 +60,9 @@ enum tpcf_flags {
     TPCF_DEVICE_INPUTS = 1u << 0,  /* input arrays (and patch labels) are device pointers */
     TPCF_SPHERICAL = 1u << 1,      /* spherical_coords = .true.: x, y are longitude and latitude and the bin
                                       edges angles, all in radians (src/tpcf.f90:313-320, :337-345); computed
-                                      in FP64 whatever the entry point's precision.  Not for tpcf_adjoint_*
-                                      (TPCF_ERR_UNSUPPORTED). */
+                                      in FP64 whatever the entry point's precision.  Also taken by
+                                      tpcf_adjoint_* and tpcf_adjoint_record_* (the shear angle of the
+                                      source-side pass stays the one at the lens). */
     TPCF_VERBOSE = 1u << 2,        /* verbose = .true.: print a summary to stdout */
     TPCF_NO_KD_TREE = 1u << 3,     /* use_kd_tree = .false.: accepted; the pair set is identical */
     TPCF_BATCH_AUTOTUNE = 1u << 4, /* batch: time candidate cell / work-item sizes first */
@@ -225,7 +226,12 @@ int tpcf_kappa_f64(tpcf_ctx *ctx,
  * TPCF_DEVICE_INPUTS the inputs and the gradient arrays are device pointers.
  * Any gradient pointer may be NULL.  Lenses with w == 0 (skipped by the
  * reference, src/tpcf.f90:285) receive the gradient of the smooth extension;
- * objects masked by z < 0 receive zero.  Gradients are FP64. */
+ * objects masked by z < 0 receive zero.  Gradients are FP64.  Flags:
+ * TPCF_DEVICE_INPUTS, TPCF_SPHERICAL, TPCF_VERBOSE, TPCF_NO_KD_TREE.  On the
+ * sphere the source-side pass takes the position angle at the lens as the
+ * forward does, from the lens' direction scaled by cos(lat_A), so a source's
+ * gradient terms from a lens within ~1e-8 rad of a pole lose precision (at
+ * the pole itself they are 0). */
 int tpcf_adjoint_shear_f32(tpcf_ctx *ctx,
                            int64_t n_A, const float *x_A, const float *y_A, const float *z_A,
                            const float *val_A, const float *w_A,

@@ -152,10 +152,3 @@ def test_sphere_python_wrapper(tp, orc):
     scale = o["gt_abs"] / o["k"]
     assert np.all(np.abs(res.g_t - o["g_t"]) <= 1e-6 * scale)
 
-
-def test_sphere_adjoint_refused(tp, ctx):
-    lens, src = field(35, 100, 100, SHEAR, lambda r, n: r.uniform(1.0, 1.1, n), lambda r, n: r.uniform(0, 0.1, n),
-                      masks=False, dtype=np.float32)
-    bins = tp.create_bins(0.001, 0.05, 4, tp.LOG)
-    with pytest.raises(tp.TpcfError):
-        tp.adjoint(SHEAR, lens, src, bins, np.ones(4), np.zeros(4), spherical=True, ctx=ctx)
