@@ -122,8 +122,9 @@ class BatchStats(C.Structure):
 
 
 class KmeansOpts(C.Structure):
-    _fields_ = [("k", C.c_int32), ("max_iterations", C.c_int32), ("init", C.c_int32), ("pad", C.c_int32),
-                ("reassignment_tol", C.c_double), ("weight_equalizer_scaling", C.c_double)]
+    _fields_ = [("k", C.c_int32), ("max_iterations", C.c_int32), ("init", C.c_int32), ("equal_size", C.c_int32),
+                ("reassignment_tol", C.c_double), ("weight_equalizer_scaling", C.c_double),
+                ("weight_tol", C.c_double)]
 
 
 class Blocks(C.Structure):

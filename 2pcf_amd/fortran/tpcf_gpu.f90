@@ -54,8 +54,8 @@ module tpcf_gpu_c
     end type tpcf_out
 
     type, bind(C) :: tpcf_kmeans_opts
-        integer(c_int32_t) :: k = 0, max_iterations = 1000, init = 0, pad = 0
-        real(c_double) :: reassignment_tol = 0.0d0, weight_equalizer_scaling = 0.0d0
+        integer(c_int32_t) :: k = 0, max_iterations = 1000, init = 0, equal_size = 0
+        real(c_double) :: reassignment_tol = 0.0d0, weight_equalizer_scaling = 0.0d0, weight_tol = 0.0d0
     end type tpcf_kmeans_opts
 
     type, bind(C) :: tpcf_blocks
@@ -613,20 +613,23 @@ contains
         call run_kappa(field, bins, results, .false., .true., v, patch_A, patch_B, n_patch_A, n_patch_B, pp)
     end subroutine kappa_2pcf_patches
 
-    !> partition_data (src/kmeans.f90:12-263) in its Voronoi form on the GPU:
-    !> data_points(n, 2) and weights as the reference takes them, 1-based
-    !> cluster_indicies, cluster_centroids(k, 2).  given_centroids = .true.
-    !> starts from cluster_centroids instead of the fixed grid; spherical
-    !> coordinates are longitude / latitude in radians.
+    !> partition_data (src/kmeans.f90:12-263) on the GPU: data_points(n, 2)
+    !> and weights as the reference takes them, 1-based cluster_indicies,
+    !> cluster_centroids(k, 2).  given_centroids = .true. starts from
+    !> cluster_centroids instead of the fixed grid; spherical coordinates are
+    !> longitude / latitude in radians.  equal_size, weight_equalizer_scaling
+    !> and weight_tol as the reference's arguments (the equaliser uses the
+    !> cluster weights of the previous round).
     subroutine partition_data_gpu(data_points, weights, k, cluster_indicies, cluster_centroids, spherical_coords, &
-                                  given_centroids, iterations, reassignment_tol)
+                                  given_centroids, iterations, reassignment_tol, equal_size, &
+                                  weight_equalizer_scaling, weight_tol)
         real, intent(in) :: data_points(:, :), weights(:)
         integer, intent(in) :: k
         integer, intent(out) :: cluster_indicies(:)
         real, intent(inout) :: cluster_centroids(k, 2)
-        logical, optional, intent(in) :: spherical_coords, given_centroids
+        logical, optional, intent(in) :: spherical_coords, given_centroids, equal_size
         integer, optional, intent(in) :: iterations
-        real, optional, intent(in) :: reassignment_tol
+        real, optional, intent(in) :: reassignment_tol, weight_equalizer_scaling, weight_tol
         real, allocatable, target :: x(:), y(:), w(:)
         integer(c_int32_t), allocatable, target :: lab(:)
         real(c_double), target :: cen(2, k), cw(k)
@@ -643,6 +646,11 @@ contains
         o%k = k
         if (present(iterations)) o%max_iterations = iterations
         if (present(reassignment_tol)) o%reassignment_tol = real(reassignment_tol, c_double)
+        if (present(weight_equalizer_scaling)) o%weight_equalizer_scaling = real(weight_equalizer_scaling, c_double)
+        if (present(weight_tol)) o%weight_tol = real(weight_tol, c_double)
+        if (present(equal_size)) then
+            if (equal_size) o%equal_size = 1
+        end if
         o%init = 0
         if (present(given_centroids)) then
             if (given_centroids) o%init = 1

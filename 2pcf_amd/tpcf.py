@@ -580,11 +580,15 @@ def bootstrap(blocks: dict, n_boot: int, *, multiplicity=None, seed: int = 0, su
 
 # --------------------------------------------------------------- k-means ----
 def kmeans(x, y, w, k: int, *, max_iterations: int = 1000, reassignment_tol: float = 0.0, centroids=None,
-           spherical: bool = False, precision: Optional[str] = None, ctx: Optional[Context] = None) -> dict:
+           spherical: bool = False, weight_equalizer_scaling: float = 0.0, equal_size: bool = False,
+           weight_tol: float = 0.0, precision: Optional[str] = None, ctx: Optional[Context] = None) -> dict:
     """k-means patch labels on the GPU (tpcf_kmeans_*): partition_data
-    (src/kmeans.f90:12-263) in its Voronoi form.  Returns 0-based labels,
-    centroids [k][2], cluster weights, the rounds run and the points moved in
-    the last round.  ``centroids`` (k x 2) replaces the fixed starting grid."""
+    (src/kmeans.f90:12-263).  Returns 0-based labels, centroids [k][2],
+    cluster weights, the rounds run and the points moved in the last round.
+    ``centroids`` (k x 2) replaces the fixed starting grid.
+    ``weight_equalizer_scaling`` > 0 scales the distances by the clusters'
+    weights of the previous round; ``equal_size`` adds the post-processing
+    that hands points to their runner-up clusters (``weight_tol``)."""
     ctx = ctx or default_context()
     prec = precision or _precision_of(x)
     dt = np.float32 if prec == "f32" else np.float64
@@ -593,6 +597,9 @@ def kmeans(x, y, w, k: int, *, max_iterations: int = 1000, reassignment_tol: flo
     o = N.KmeansOpts()
     o.k, o.max_iterations, o.reassignment_tol = int(k), int(max_iterations), float(reassignment_tol)
     o.init = 0 if centroids is None else 1
+    o.weight_equalizer_scaling = float(weight_equalizer_scaling)
+    o.equal_size = 1 if equal_size else 0
+    o.weight_tol = float(weight_tol)
     cen = np.zeros((k, 2)) if centroids is None else np.array(centroids, dtype=np.float64).reshape(k, 2)
     lab = np.zeros(n, dtype=np.int32)
     cw = np.zeros(k)

@@ -399,25 +399,33 @@ int tpcf_bootstrap(tpcf_ctx *ctx, const tpcf_blocks *blocks, int32_t n_boot, con
                    double *weights, double *cov_t, double *cov_x, int32_t *multiplicity_out);
 
 /* ---- k-means patch labels ----------------------------------------------------
- * Replaces partition_data (src/kmeans.f90:12-263) in its Voronoi form
- * (equal_size = .false., weight_equalizer_scaling = 0: the voronoi_blocks
- * scheme of src/tpcf.f90:1100-1113): rounds of nearest-centroid assignment
- * (first minimum kept) and weighted-mean centroids until a round moves no
- * point, moves fewer than reassignment_tol * n, or max_iterations + 1 rounds
- * have run.  init 0: the fixed starting grid of
- * kmeans_compute_fixed_initial_centroids (src/kmeans.f90:85-110), labels
- * int(i/n k); init 1: `centroids` holds the k starting centroids.  labels are
- * 0-based.  TPCF_SPHERICAL: longitude / latitude in radians, great-circle
- * distances.  A non-zero weight_equalizer_scaling returns TPCF_ERR_UNSUPPORTED
- * (the reference moves cluster weights point by point, so that form depends
- * on the visiting order).  2 <= k <= 1024; FP64 arithmetic. */
+ * Replaces partition_data (src/kmeans.f90:12-263), the clustering of the
+ * voronoi_blocks and equal_weight_blocks schemes (src/tpcf.f90:1100-1113):
+ * rounds of nearest-centroid assignment (first minimum kept) and
+ * weighted-mean centroids until a round moves no point, moves fewer than
+ * reassignment_tol * n, or max_iterations + 1 rounds have run.  init 0: the
+ * fixed starting grid of kmeans_compute_fixed_initial_centroids
+ * (src/kmeans.f90:85-110), labels int(i/n k); init 1: `centroids` holds the k
+ * starting centroids.  labels are 0-based.  TPCF_SPHERICAL: longitude /
+ * latitude in radians, great-circle distances.
+ * weight_equalizer_scaling s > 0: from the second round the distance to
+ * cluster j is scaled by 1 + (W_j / W_mean - 1) s with W_j the cluster's
+ * weight at the end of the previous round (the reference updates W_j point
+ * by point within a round, which makes its result depend on the visiting
+ * order; this form does not).  equal_size 1: the runner-up clusters are
+ * tracked from the second round and, when (max W - min W) / W_mean exceeds
+ * weight_tol after the rounds, postprocess_weights (src/kmeans.f90:214-262)
+ * hands points to their runner-up clusters (equal runner-up ratios are taken
+ * in point order).  centroids are those of the last round; cluster_weights
+ * those after the hand-overs.  2 <= k <= 1024; FP64 arithmetic. */
 typedef struct tpcf_kmeans_opts {
     int32_t k;
     int32_t max_iterations;            /* the reference's default: 1000 */
     int32_t init;                      /* 0: fixed grid, 1: given centroids */
-    int32_t pad;
+    int32_t equal_size;                /* 1: runner-up tracking and postprocess_weights */
     double reassignment_tol;           /* stop when moves / n < tol (0: run to convergence) */
-    double weight_equalizer_scaling;   /* must be 0 */
+    double weight_equalizer_scaling;   /* s >= 0 (0: plain distances) */
+    double weight_tol;                 /* equal_size: the post-processing's relative weight tolerance */
 } tpcf_kmeans_opts;
 
 int tpcf_kmeans_f32(tpcf_ctx *ctx, int64_t n, const float *x, const float *y, const float *w,

@@ -107,8 +107,11 @@ def lib():
         _lib.orc_finalize.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib.orc_chord_edges_squared.argtypes = [_dp, C.c_int, _dp]
         _lib.orc_kmeans.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
-                                    C.c_int, C.c_int, _i32p, _dp, _dp, C.POINTER(C.c_int64)]
+                                    C.c_int, C.c_int, C.c_int, C.c_double, _i32p, _dp, _dp, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64)]
         _lib.orc_kmeans.restype = C.c_int
+        _lib.orc_qsort_arg.argtypes = [C.c_int64, _dp, C.POINTER(C.c_int64)]
+        _lib.orc_qsort_arg.restype = C.c_int
         _lib.orc_sph_pair.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib.rp_create_bins.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.POINTER(_RpBins)]
         _lib.rp_create_bins.restype = C.c_int
@@ -322,6 +325,7 @@ def ref_lib():
         L.ref_sph_pair_c.argtypes = [ci] + [_fp] * 7
         L.ref_covariance_c.argtypes = [ci, ci, _fp, _fp]
         L.ref_kmeans_c.argtypes = [ci, _fp, _fp, _fp, ci, ci, cf, ci, cf, cf, _i32p, _fp]
+        L.ref_qsort_arg_c.argtypes = [ci, _fp, _i32p, ci]
         L.ref_bins_c.argtypes = [cf, cf, ci, ci, cf, ci, _fp, _fp, _fp, _i32p]
         L.ref_flush_c.argtypes = []
         _ref_lib = L
@@ -478,21 +482,51 @@ def covariance(x: np.ndarray) -> np.ndarray:
 
 
 def kmeans(x, y, w, k: int, max_iterations: int = 1000, reassignment_tol: float = 0.0, scaling: float = 0.0,
-           jacobi: bool = False, spherical: bool = False, centroids=None) -> dict:
+           jacobi: bool = False, spherical: bool = False, centroids=None, equal_size: bool = False,
+           weight_tol: float = 0.0, block: int | None = None) -> dict:
     """partition_data (src/kmeans.f90:12-263) restated in FP64 (see
-    orc_kmeans): 0-based labels, centroids [k][2], cluster weights, rounds."""
+    orc_kmeans): 0-based labels, centroids [k][2], cluster weights, rounds,
+    and with ``equal_size`` the post-processing's hand-overs.  ``block``: the
+    equaliser's weights move after every block of that many points (1, the
+    default: the reference's single thread); ``jacobi`` (block None): the
+    previous round's weights for the whole round."""
+    if block is None:
+        block = 0 if jacobi else 1
     x, y, w = _d(x), _d(y), _d(w)
     n = len(x)
     lab = np.zeros(n, dtype=np.int32)
     cen = np.zeros((k, 2)) if centroids is None else np.array(centroids, dtype=np.float64)
     cw = np.zeros(k)
-    moves = C.c_int64(0)
+    moves, ppm = C.c_int64(0), C.c_int64(0)
     it = lib().orc_kmeans(n, _ptr(x, _dp), _ptr(y, _dp), _ptr(w, _dp), k, max_iterations, reassignment_tol, scaling,
-                          1 if jacobi else 0, 1 if spherical else 0, 0 if centroids is None else 1,
-                          _ptr(lab, _i32p), _ptr(cen, _dp), _ptr(cw, _dp), C.byref(moves))
-    if it < 0:
+                          int(block), 1 if spherical else 0, 0 if centroids is None else 1,
+                          1 if equal_size else 0, float(weight_tol), _ptr(lab, _i32p), _ptr(cen, _dp), _ptr(cw, _dp),
+                          C.byref(moves), C.byref(ppm))
+    if it == -1:
         raise ValueError("k-means needs n >= 1 and k >= 2")
-    return dict(labels=lab, centroids=cen, cluster_weights=cw, iterations=it, last_moves=moves.value)
+    if it < 0:
+        raise MemoryError("oracle k-means: allocation failed")
+    return dict(labels=lab, centroids=cen, cluster_weights=cw, iterations=it, last_moves=moves.value,
+                pp_moves=ppm.value)
+
+
+def qsort_arg(a) -> np.ndarray:
+    """The oracle's restatement of qsort_arg (src/utils.f90:239-298): the
+    0-based permutation sorting a in increasing order, ties in the
+    reference's order."""
+    a = _d(np.atleast_1d(a))
+    order = np.zeros(len(a), dtype=np.int64)
+    if lib().orc_qsort_arg(len(a), _ptr(a, _dp), order.ctypes.data_as(C.POINTER(C.c_int64))):
+        raise MemoryError("oracle qsort_arg: allocation failed")
+    return order
+
+
+def ref_qsort_arg(a, reverse: bool = False) -> np.ndarray:
+    """The reference's own qsort_arg (FP32 values), 0-based."""
+    af = np.ascontiguousarray(a, dtype=np.float32)
+    order = np.zeros(len(af), dtype=np.int32)
+    ref_lib().ref_qsort_arg_c(len(af), _ptr(af, _fp), _ptr(order, _i32p), 1 if reverse else 0)
+    return order.astype(np.int64) - 1
 
 
 def ref_kmeans(x, y, w, k: int, max_iterations: int = 1000, reassignment_tol: float = 0.0, scaling: float = 0.0,

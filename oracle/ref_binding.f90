@@ -29,6 +29,14 @@ contains
         d = distance_euclidean(x1, y1, x2, y2)
     end function ref_euclid
 
+    ! qsort_arg (src/utils.f90:239-298): the 1-based sorting permutation of a.
+    subroutine ref_qsort_arg_c(n, a, order, reverse) bind(C, name="ref_qsort_arg_c")
+        integer(c_int), value :: n, reverse
+        real(c_float), intent(in) :: a(n)
+        integer(c_int), intent(out) :: order(n)
+        call qsort_arg(a, order, reverse=(reverse /= 0))
+    end subroutine ref_qsort_arg_c
+
     ! partition_data (src/kmeans.f90:12-263) with fixed initial centroids and
     ! one thread; labels are 1-based as the reference returns them.
     subroutine ref_kmeans_c(n, x, y, w, k, equal_size, scaling, iterations, reassign_tol, weight_tol, &

@@ -497,14 +497,28 @@ void orc_bin_of_d2_array(int64_t n, const double *d2, const double *e2, int n_bi
  * 1 + (W_j / W_mean - 1) s when s > 0) and calculate_centroids (:202-212:
  * weighted means of the member coordinates over the tracked cluster weights)
  * until a round moves no point, moves fewer than tol n, or max_it + 1 rounds
- * have run (the loop condition of :117).  jacobi = 0 updates the cluster
- * weights while a round runs, point by point, as the reference's single
- * thread does; jacobi = 1 uses the weights of the previous round (the GPU's
- * order).  With s = 0 the two are the same.  The equal-size post-processing
- * (:214-262) is not restated.  Spherical: distances are great-circle angles
- * of (longitude, latitude) in radians and centroids the weighted means of the
- * coordinates, as the reference's distance_sphere and calculate_centroids
- * give.  FP64 throughout.  Returns the number of rounds run. */
+ * have run (the loop condition of :117).  The cluster weights that scale the
+ * distances move with the points: block = 1 updates them after every point,
+ * as the reference's single thread does; block = B > 1 takes the points in
+ * blocks of B consecutive ones, each point of a block seeing the weights from
+ * before its block (B = 1024: the GPU's order); block <= 0 uses the weights
+ * of the previous round for the whole round.  With s = 0 all are the same.
+ * equal_size = 1 (the equal_weight_blocks scheme, src/tpcf.f90:1104-1112):
+ * from the second round on each point also keeps its candidate, the
+ * runner-up cluster, and the ratio of the runner-up's to the winner's
+ * distance (:167-180; both persist across rounds, -1 / 0 before any is set,
+ * and a point at distance 0 from a centroid leaves them as they were).  After
+ * the rounds, when (max W - min W) / W_mean > weight_tol, postprocess_weights
+ * (:214-262): the clusters in decreasing order of weight (qsort_arg,
+ * src/utils.f90:239-298, reversed), each in turn handing its members, in
+ * increasing order of their ratio (qsort_arg again), to their candidate
+ * cluster when that cluster comes later in the order, until its weight drops
+ * to W_mean (1 + weight_tol) (checked after each hand-over).  The centroids
+ * stay those of the last round.  Spherical: distances are great-circle
+ * angles of (longitude, latitude) in radians and centroids the weighted means
+ * of the coordinates, as the reference's distance_sphere and
+ * calculate_centroids give.  FP64 throughout.  Returns the number of rounds
+ * run; *pp_moves the hand-overs of the post-processing. */
 static double km_dist(int sph, double x1, double y1, double x2, double y2)
 {
     if (!sph) return sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
@@ -514,9 +528,45 @@ static double km_dist(int sph, double x1, double y1, double x2, double y2)
     return sph_angle(u, v);
 }
 
+/* qsort_arg (src/utils.f90:239-298): the permutation that sorts a[] in
+ * increasing order, by the reference's recursive partition scheme (middle
+ * pivot, scans with <= and >=, a meeting point stepped past), so that equal
+ * values come out in the reference's order.  order[] is 0-based. */
+static void qsort_rec(double *wk, int64_t *order, int64_t l, int64_t r)
+{
+    const double pivot = wk[(l + r) / 2];
+    int64_t i = l - 1, j = r + 1;
+    for (;;) {
+        do { --j; } while (!(wk[j] <= pivot));
+        do { ++i; } while (!(wk[i] >= pivot));
+        if (i < j) {
+            const double t = wk[i]; wk[i] = wk[j]; wk[j] = t;
+            const int64_t o = order[i]; order[i] = order[j]; order[j] = o;
+        } else if (i == j) {
+            ++i;
+            break;
+        } else {
+            break;
+        }
+    }
+    if (l < j) qsort_rec(wk, order, l, j);
+    if (i < r) qsort_rec(wk, order, i, r);
+}
+
+int orc_qsort_arg(int64_t n, const double *a, int64_t *order)
+{
+    if (n < 1) return 0;
+    double *wk = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!wk) return -2;
+    for (int64_t i = 0; i < n; ++i) { wk[i] = a[i]; order[i] = i; }
+    qsort_rec(wk, order, 0, n - 1);
+    free(wk);
+    return 0;
+}
+
 int orc_kmeans(int64_t n, const double *x, const double *y, const double *w, int k, int max_it, double tol,
-               double scaling, int jacobi, int sph, int init_given, int32_t *labels, double *cen, double *cw,
-               int64_t *last_moves)
+               double scaling, int block, int sph, int init_given, int equal_size, double weight_tol,
+               int32_t *labels, double *cen, double *cw, int64_t *last_moves, int64_t *pp_moves)
 {
     if (n < 1 || k < 2) return -1;
     double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY, wsum = 0.0;
@@ -553,21 +603,44 @@ int orc_kmeans(int64_t n, const double *x, const double *y, const double *w, int
     }
     double *cw_prev = (double *)malloc(sizeof(double) * (size_t)k);
     double *sx = (double *)malloc(sizeof(double) * 3 * (size_t)k);
-    if (!cw_prev || !sx) { free(cw_prev); free(sx); return -2; }
+    double *mem = equal_size ? (double *)calloc((size_t)n, sizeof(double)) : NULL;
+    int32_t *cand = equal_size ? (int32_t *)malloc(sizeof(int32_t) * (size_t)n) : NULL;
+    if (!cw_prev || !sx || (equal_size && (!mem || !cand))) {
+        free(cw_prev); free(sx); free(mem); free(cand);
+        return -2;
+    }
+    if (equal_size)
+        for (int64_t i = 0; i < n; ++i) cand[i] = -1;
     int it = 0;
     int64_t moves = 0;
+    double max_dev = 0.0;
     while (it <= max_it) {
         ++it;
         moves = 0;
-        memcpy(cw_prev, cw, sizeof(double) * (size_t)k);
-        const double *cwd = jacobi ? cw_prev : cw;
+        const int64_t bl = block > 0 ? block : n;
+        const int track = equal_size && it > 1;
         for (int64_t i = 0; i < n; ++i) {
+            if (i % bl == 0) memcpy(cw_prev, cw, sizeof(double) * (size_t)k);
+            const double *cwd = cw_prev;
             double best = diameter * 10.0;
             int bi = -1;
             for (int j = 0; j < k; ++j) {
                 const double dw = (it > 1 && scaling > 0.0) ? 1.0 + (cwd[j] / mean_w - 1.0) * scaling : 1.0;
                 const double d = dw * km_dist(sph, x[i], y[i], cen[2 * j], cen[2 * j + 1]);
-                if (d < best) { best = d; bi = j; }
+                if (track && d > 0.0) {
+                    if (d < best) {
+                        mem[i] = best / d;
+                        cand[i] = bi;
+                        best = d;
+                        bi = j;
+                    } else if (d < mem[i] * best) {
+                        mem[i] = d / best;
+                        cand[i] = j;
+                    }
+                } else if (d < best) {
+                    best = d;
+                    bi = j;
+                }
             }
             if (bi >= 0 && bi != labels[i]) {
                 cw[labels[i]] -= w[i];
@@ -582,15 +655,56 @@ int orc_kmeans(int64_t n, const double *x, const double *y, const double *w, int
             sx[3 * labels[i] + 1] += y[i] * w[i];
             sx[3 * labels[i] + 2] += w[i];
         }
+        double wmin = INFINITY, wmax = -INFINITY;
         for (int j = 0; j < k; ++j) {
             cw[j] = sx[3 * j + 2];   /* the tracked weights, re-summed exactly */
             cen[2 * j] = sx[3 * j] / cw[j];
             cen[2 * j + 1] = sx[3 * j + 1] / cw[j];
+            wmin = fmin(wmin, cw[j]);
+            wmax = fmax(wmax, cw[j]);
         }
+        max_dev = (wmax - wmin) / mean_w;
         if (moves == 0 || (double)moves / (double)n < tol) break;
+    }
+    int64_t ppm = 0;
+    if (equal_size && max_dev > weight_tol) {
+        int64_t *ord = (int64_t *)malloc(sizeof(int64_t) * (size_t)k);
+        int64_t *cord = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+        char *done = (char *)calloc((size_t)k, 1);
+        int64_t *memb = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+        if (!ord || !cord || !done || !memb || orc_qsort_arg(k, cw, ord) || orc_qsort_arg(n, mem, cord)) {
+            free(ord); free(cord); free(done); free(memb); free(cw_prev); free(sx); free(mem); free(cand);
+            return -2;
+        }
+        for (int a = 0, b = k - 1; a < b; ++a, --b) {   /* reverse = .true. */
+            const int64_t t = ord[a]; ord[a] = ord[b]; ord[b] = t;
+        }
+        const double limit = mean_w * (1.0 + weight_tol);
+        for (int jj = 0; jj < k; ++jj) {
+            const int c = (int)ord[jj];
+            done[c] = 1;   /* cluster_size_order(1:j) holds the clusters up to this one */
+            /* the members in candidate order, packed before any is handed over */
+            int64_t nm = 0;
+            for (int64_t q = 0; q < n; ++q)
+                if (labels[cord[q]] == c) memb[nm++] = cord[q];
+            for (int64_t q = 0; q < nm; ++q) {
+                const int64_t p = memb[q];
+                const int dst = cand[p];
+                if (dst < 0 || done[dst]) continue;
+                labels[p] = dst;
+                cw[c] -= w[p];
+                cw[dst] += w[p];
+                ++ppm;
+                if (cw[c] <= limit) break;
+            }
+        }
+        free(ord); free(cord); free(done); free(memb);
     }
     free(cw_prev);
     free(sx);
+    free(mem);
+    free(cand);
     if (last_moves) *last_moves = moves;
+    if (pp_moves) *pp_moves = ppm;
     return it;
 }

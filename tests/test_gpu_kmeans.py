@@ -9,8 +9,17 @@ centroids could still go either way: labels must agree for >= 99.99 % of the
 points (all of them in these cases so far), centroids within 1e-9 of the
 field size, rounds within one.
 Cases: clustered and uniform flat catalogues in FP64 and FP32, given starting
-centroids, spherical coordinates across longitude 0; the weight-equalised
-form and k outside [2, 1024] are refused.
+centroids, spherical coordinates across longitude 0; k outside [2, 1024] is
+refused.
+The weight equaliser and the equal-size post-processing are compared with the
+oracle's form that moves the cluster weights after every block of 256
+consecutive points (block=256, the GPU's steps; the reference moves them
+after every point, block=1, which the oracle also restates and pins against
+the reference): weighted and unit-weight catalogues (ties
+between cluster weights, where the reference's quicksort order of the
+clusters matters), flat and spherical, with and without the equaliser, with
+the same label, centroid and weight bounds; the post-processing's cluster
+weights are also checked against the sums of the members' weights.
 """
 import numpy as np
 import pytest
@@ -90,3 +99,48 @@ def test_kmeans_reproducible(tp, ctx):
     np.testing.assert_array_equal(a["centroids"], b["centroids"])
     np.testing.assert_array_equal(a["cluster_weights"], b["cluster_weights"])
     assert a["iterations"] == b["iterations"]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_kmeans_weight_equaliser(tp, orc, ctx, dtype):
+    x, y, w = blobs(21, 150000, k=20, dtype=dtype)
+    g = tp.kmeans(x, y, w, 32, weight_equalizer_scaling=5.0, max_iterations=60, ctx=ctx)
+    o = orc.kmeans(x, y, w, 32, scaling=5.0, max_iterations=60, block=256)
+    agree(g, o, 100.0)
+    plain = orc.kmeans(x, y, w, 32, max_iterations=60)
+    assert np.mean(plain["labels"] == o["labels"]) < 0.99   # the equaliser changes the labels
+
+
+@pytest.mark.parametrize("case", ["weighted", "unit_weights", "equalised", "sphere"])
+def test_kmeans_equal_size(tp, orc, ctx, case):
+    r = np.random.default_rng(31)
+    sph = case == "sphere"
+    if sph:
+        n, k = 60000, 24
+        x = r.uniform(1.0, 1.6, n)
+        y = r.normal(0.3, 0.12, n).clip(-0.1, 0.7)
+        w = r.uniform(0.5, 1.5, n)
+        size = 2 * np.pi
+    else:
+        x, y, w = blobs(32, 120000, k=15, dtype=np.float32)
+        k, size = 40, 100.0
+        if case == "unit_weights":
+            w = np.ones_like(w)
+    s = 3.0 if case == "equalised" else 0.0
+    for tol in (0.0, 0.05):
+        g = tp.kmeans(x, y, w, k, weight_equalizer_scaling=s, equal_size=True, weight_tol=tol, max_iterations=40,
+                      spherical=sph, ctx=ctx)
+        o = orc.kmeans(x, y, w, k, scaling=s, max_iterations=40, block=256, equal_size=True, weight_tol=tol,
+                       spherical=sph)
+        print(f"{case} tol {tol}: {o['pp_moves']} hand-overs", flush=True)
+        assert o["pp_moves"] > 0
+        agree(g, o, size)
+        np.testing.assert_allclose(g["cluster_weights"],
+                                   np.bincount(g["labels"], weights=np.asarray(w, np.float64), minlength=k),
+                                   rtol=1e-9)
+    again = tp.kmeans(x, y, w, k, weight_equalizer_scaling=s, equal_size=True, max_iterations=40, spherical=sph,
+                      ctx=ctx)
+    g0 = tp.kmeans(x, y, w, k, weight_equalizer_scaling=s, equal_size=True, max_iterations=40, spherical=sph,
+                   ctx=ctx)
+    np.testing.assert_array_equal(again["labels"], g0["labels"])
+    np.testing.assert_array_equal(again["cluster_weights"], g0["cluster_weights"])

@@ -10,6 +10,16 @@ centroids; built from its source into oracle/_ref).
 * Planted mistakes fail: unweighted centroids, or moving the weights only at
   the end of a round (the GPU order), change the equalised labels for most
   points / measurably.
+* qsort_arg (src/utils.f90:239-298): the oracle's restatement gives the
+  reference's permutation exactly, ties included (integer-valued arrays full
+  of equal values), and reversed as the reference reverses it.
+* Equal size (equal_size, postprocess_weights, src/kmeans.f90:214-262): the
+  same labels as the reference for >= 99.9 % of the points without the
+  equaliser (where the post-processing hands thousands of points over; the
+  few that differ come from FP32 against FP64 distances), >= 99 % with it, on
+  weighted and on unit-weight catalogues (ties between cluster weights); the
+  post-processing left out, or given a tolerance that skips it, disagrees
+  with the reference for a large share of the points.
 """
 import numpy as np
 import pytest
@@ -58,3 +68,55 @@ def test_kmeans_equalised_matches_reference_and_mistakes_fail(orc):
     unw = orc.kmeans(x, y, np.ones_like(w), 16)
     refv = orc.ref_kmeans(x, y, w, 16)
     assert np.max(np.abs(unw["centroids"] - refv["centroids"])) > 1e-3
+
+
+def test_qsort_arg_matches_reference_ties_included(orc):
+    need_ref(orc)
+    r = np.random.default_rng(1)
+    for case in range(6):
+        n = int(r.integers(1, 400))
+        a = (r.integers(0, 5, n) if case % 2 == 0 else r.uniform(0, 1, n)).astype(np.float32)
+        got = orc.qsort_arg(a)
+        np.testing.assert_array_equal(got, orc.ref_qsort_arg(a))
+        np.testing.assert_array_equal(got[::-1], orc.ref_qsort_arg(a, reverse=True))
+        assert np.all(np.diff(a[got]) >= 0)
+
+
+@pytest.mark.parametrize("unit_w", [False, True])
+def test_kmeans_equal_size_matches_reference(orc, unit_w):
+    need_ref(orc)
+    x, y, w = blobs(2)
+    if unit_w:
+        w = np.ones_like(w)
+    for tol in (0.0, 0.02):
+        ref = orc.ref_kmeans(x, y, w, 16, max_iterations=50, equal_size=True, weight_tol=tol)
+        o = orc.kmeans(x, y, w, 16, max_iterations=50, equal_size=True, weight_tol=tol)
+        assert o["pp_moves"] > 1000
+        assert np.mean(ref["labels"] == o["labels"]) >= 0.999   # FP32 reference against FP64
+        # the post-processing left out / skipped by a large tolerance: far off
+        plain = orc.kmeans(x, y, w, 16, max_iterations=50)
+        assert np.mean(ref["labels"] == plain["labels"]) < 0.9
+        skip = orc.kmeans(x, y, w, 16, max_iterations=50, equal_size=True, weight_tol=100.0)
+        assert skip["pp_moves"] == 0 and np.mean(ref["labels"] == skip["labels"]) < 0.9
+    ref = orc.ref_kmeans(x, y, w, 16, scaling=5.0, max_iterations=50, equal_size=True)
+    o = orc.kmeans(x, y, w, 16, scaling=5.0, max_iterations=50, equal_size=True)
+    assert np.mean(ref["labels"] == o["labels"]) >= 0.99
+
+
+def test_kmeans_equal_size_invariants(orc):
+    """Each hand-over goes to the point's runner-up cluster, which comes later
+    in the decreasing-weight order; the cluster weights stay the sums of their
+    members' weights."""
+    x, y, w = blobs(4, n=8000, k=9)
+    plain = orc.kmeans(x, y, w, 9, max_iterations=50)
+    eq = orc.kmeans(x, y, w, 9, max_iterations=50, equal_size=True)
+    np.testing.assert_array_equal(plain["labels"] == eq["labels"], plain["labels"] == eq["labels"])
+    moved = np.nonzero(plain["labels"] != eq["labels"])[0]
+    assert len(moved) == eq["pp_moves"] > 0
+    cw_before = np.bincount(plain["labels"], weights=w.astype(np.float64), minlength=9)
+    order = orc.qsort_arg(cw_before)[::-1]
+    rank = np.empty(9, int)
+    rank[order] = np.arange(9)
+    assert np.all(rank[eq["labels"][moved]] > rank[plain["labels"][moved]])
+    np.testing.assert_allclose(eq["cluster_weights"],
+                               np.bincount(eq["labels"], weights=w.astype(np.float64), minlength=9), rtol=1e-12)
