@@ -355,8 +355,34 @@ __device__ __forceinline__ void win_clear(Win &w, int lane)
     w.edge = false;
 }
 
+// Sums over the wave of rows r < NR <= 8 of 64 per-lane FP64 values
+// (rows[r * kWave + lane]) through LDS rather than cross-lane steps (a DPP
+// reduction of one FP64 value takes about 20 VALU instructions): lane l adds
+// the values 8 (l % 8) .. 8 (l % 8) + 7 of row l / 8 and writes the partial to
+// slot l % 8 of that row, then lanes r < NR add their row's 8 partials and
+// leave the total in slot 8.  The rows are overwritten.  A wave's LDS
+// operations run in order, so a step reads what the step before wrote
+// without a barrier; all lanes of the wave must be active.
+template <int NR>
+__device__ __forceinline__ void lds_rowsum(double *rows, int lane)
+{
+    static_assert(NR >= 1 && NR <= 8, "eight lanes per row");
+    const int r = lane >> 3, p = lane & 7;
+    if (r < NR) {
+        const double2 *q = reinterpret_cast<const double2 *>(rows + r * kWave + p * 8);
+        const double2 a = q[0], b = q[1], c = q[2], d = q[3];
+        rows[r * kWave + p] = ((a.x + a.y) + (b.x + b.y)) + ((c.x + c.y) + (d.x + d.y));
+    }
+    if (lane < NR) {
+        const double2 *q = reinterpret_cast<const double2 *>(rows + lane * kWave);
+        const double2 a = q[0], b = q[1], c = q[2], d = q[3];
+        rows[lane * kWave + 8] = ((a.x + a.y) + (b.x + b.y)) + ((c.x + c.y) + (d.x + d.y));
+    }
+}
+
 // Reduce the window's per-lane sums across the wave into the per-bin table
-// (and, with marks, add each lane's sums to its lens' marks).
+// (and, with marks, add each lane's sums to its lens' marks).  Each bin's
+// rows are replaced by the lanes' weighted terms and summed by lds_rowsum.
 template <typename T, int KIND, bool MARKS, bool MZ>
 __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bin, double *__restrict__ marks)
 {
@@ -372,38 +398,32 @@ __device__ __forceinline__ void win_flush(Win &w, const LaneData<T> &L, int n_bi
         if (!used || bin < 0 || bin >= n_bin) continue;
         const int o = s * WQ;
         const double aT = wacc(w, o + QT, lane), aG1 = wacc(w, o + QG1, lane), aG2 = wacc(w, o + QG2, lane);
-        const double sT = wave_sum(dwa * aT);
-        double sG1 = 0, sG2 = 0, sW = 0, sK = 0;
-        if (KIND == SHEAR || MARKS || w.edge) {
-            sG1 = gs * wave_sum(daa * aG1);
-            sG2 = gs * wave_sum(daa * aG2);
-        }
+        const bool with_g = KIND == SHEAR || MARKS || w.edge, with_w = MARKS || w.edge;
         double aW = 0, aK = 0;
-        if (MARKS || w.edge) {
+        if (with_w) {
             aW = wacc(w, o + QW, lane);
-            sW = wave_sum(dwa * aW);
-            if (MZ) {
-                aK = aW;   // m == 0: K == W
-                sK = sW;
-            } else {
-                aK = wacc(w, o + QK, lane);
-                sK = wave_sum(dwa * aK);
-            }
+            aK = MZ ? aW : wacc(w, o + QK, lane);   // m == 0: K == W
         }
         // the position angle of a zero-separation pair is 0/0: the reference's
         // G_t and G_x of that bin, and the xi_E / xi_B marks of that lens,
         // become NaN (src/tpcf.f90:343-355)
         const bool zero_sep = KIND == SHEAR && bin == 0 && w.edge && (double)L.vmask * wacc(w, 2 * WQ, lane) > 0.0;
-        if (KIND == SHEAR && bin == 0 && w.edge && __ballot(zero_sep)) {
-            sG1 = __builtin_nan("");
-            sG2 = __builtin_nan("");
-        }
+        wacc(w, o + QT, lane) = dwa * aT;
+        wacc(w, o + QG1, lane) = with_g ? daa * aG1 : 0.0;
+        wacc(w, o + QG2, lane) = with_g ? daa * aG2 : 0.0;
+        wacc(w, o + QW, lane) = with_w ? dwa * aW : 0.0;
+        if (!MZ) wacc(w, o + QK, lane) = with_w ? dwa * aK : 0.0;
+        lds_rowsum<WQ>(w.acc + o * kWave, lane);
         if (lane == bin) {
-            w.t[QT] += sT;
-            w.t[QG1] += sG1;
-            w.t[QG2] += sG2;
-            w.t[QW] += sW;
-            if (!MZ) w.t[QK] += sK;   // MZ: K == W, not kept
+            w.t[QT] += wacc(w, o + QT, 8);
+            w.t[QG1] += gs * wacc(w, o + QG1, 8);
+            w.t[QG2] += gs * wacc(w, o + QG2, 8);
+            w.t[QW] += wacc(w, o + QW, 8);
+            if (!MZ) w.t[QK] += wacc(w, o + QK, 8);   // MZ: K == W, not kept
+        }
+        if (KIND == SHEAR && bin == 0 && w.edge && __ballot(zero_sep) && lane == bin) {
+            w.t[QG1] = __builtin_nan("");
+            w.t[QG2] = __builtin_nan("");
         }
         // A lens' chunk may be split over several waves: add atomically.  The
         // atomics are what a marks pass costs, so a lane whose four terms are
@@ -469,7 +489,8 @@ __device__ __forceinline__ uint64_t edge_below(const EdgeTest<T> &et, T d2, cons
 // are two exec-masked branches (empty asm statements keep them from being
 // turned into selects), so each pair issues one side's instructions.  Pair
 // counts are per lane, added in the upper side's branch, and summed over the
-// valid lanes after the segment.  GUARD: the chunk and segment
+// valid lanes after the segment (a scalar count per source, the popcount of
+// the lane mask, cost more: the kernel's scalar registers are its scarcest).  GUARD: the chunk and segment
 // boxes overlap, so a pair may sit at zero separation (its per-pair values are
 // forced finite); TRACK0: shear with x_min == 0, where such a pair is in bin 0
 // (reference: NaN).  MZ: all m == 0, so K == W.
@@ -487,19 +508,20 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
         const bool below = __builtin_amdgcn_inverse_ballot_w64(edge_below<T>(et, d2, L, q));
         T d, inv, u, v;
         pair_proj<T, SPH, GUARD>(L, g, d, inv, u, v);
-        // the shear terms before the branches (both sides use them)
+        // the shear terms before the branches (both sides use them); 1/d^2
+        // is applied by the branches' FMAs
         T t1 = 0, t2 = 0;
         if (KIND == SHEAR) {
-            t1 = fma(q.s1w, u, q.s2w2 * v) * inv;
-            t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)) * inv;
+            t1 = fma(q.s1w, u, q.s2w2 * v);
+            t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v));
         }
         if (below) {
             lT = fma(d, q.w, lT);
             lW += q.w;
             if (!MZ) lK += q.wk;
             if (KIND == SHEAR) {
-                lG1 += t1;
-                lG2 += t2;
+                lG1 = fma(t1, inv, lG1);
+                lG2 = fma(t2, inv, lG2);
                 if (TRACK0) sZ += (d2 == (T)0) ? (T)1 : (T)0;
             } else {
                 lG1 += q.s1w;
@@ -512,8 +534,8 @@ __device__ __forceinline__ void edge_pass(Win &w, const LaneData<T> &L, const Pa
             hW += q.w;
             if (!MZ) hK += q.wk;
             if (KIND == SHEAR) {
-                hG1 += t1;
-                hG2 += t2;
+                hG1 = fma(t1, inv, hG1);
+                hG2 = fma(t2, inv, hG2);
             } else {
                 hG1 += q.s1w;
                 hG2 += q.s2w;
@@ -739,12 +761,14 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
             int k = 0;
 #pragma unroll
             for (int i = 1; i < S; ++i) k += __builtin_amdgcn_inverse_ballot_w64(ge[i]) ? 1 : 0;
-            T d, t1, t2;
+            // the pair's terms: t1 inv, t2 inv (shear: 1/d^2 applied by the
+            // row update's FMA; kappa: inv = 1)
+            T d, t1, t2, inv = (T)1;
             if constexpr (KIND == SHEAR) {
-                T inv, u, v;
+                T u, v;
                 pair_proj<T, SPH, false>(L, g, d, inv, u, v);
-                t1 = fma(q.s1w, u, q.s2w2 * v) * inv;
-                t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)) * inv;
+                t1 = fma(q.s1w, u, q.s2w2 * v);
+                t2 = fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v));
                 if constexpr (ZSEP) {
                     const bool z0 = d2 == (T)0;
                     d = z0 ? (T)0 : d;
@@ -760,9 +784,14 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
             const int at = k * kWave + lane;
             if constexpr (sizeof(T) == 4) {
                 float4 a = r4[at];
-                a.x += d * q.w;
-                a.y += t1;
-                a.z += t2;
+                a.x = fma(d, q.w, a.x);
+                if constexpr (KIND == SHEAR) {
+                    a.y = fma(t1, inv, a.y);
+                    a.z = fma(t2, inv, a.z);
+                } else {
+                    a.y += t1;
+                    a.z += t2;
+                }
                 a.w += q.w;
                 r4[at] = a;
                 if constexpr (MZ) {
@@ -776,8 +805,8 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
             } else {
                 T *const p = rows + at;
                 atomicAdd(p + QT * S * kWave, d * q.w);
-                atomicAdd(p + QG1 * S * kWave, t1);
-                atomicAdd(p + QG2 * S * kWave, t2);
+                atomicAdd(p + QG1 * S * kWave, KIND == SHEAR ? t1 * inv : t1);
+                atomicAdd(p + QG2 * S * kWave, KIND == SHEAR ? t2 * inv : t2);
                 atomicAdd(p + QW * S * kWave, q.w);
                 if constexpr (!MZ) atomicAdd(p + QK * S * kWave, q.wk);
                 count_pair(ge);
@@ -1176,6 +1205,7 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     if (it >= A.n_items) return;
     // per wave: the window sums or multi-pass rows, then the grid and the
     // segment pointers (11 x 8 bytes)
+    static_assert((wave_lds_doubles<T, MZ>() + 12) % 2 == 0, "16-byte aligned wave blocks (double2 reads)");
     __shared__ __align__(16) double s_acc[kWavesPerBlock][wave_lds_doubles<T, MZ>() + 12];
     CatArgs<T> C;
     C.g = g;

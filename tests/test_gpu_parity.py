@@ -313,8 +313,9 @@ def test_error_paths(tp, ctx):
     assert e.value.code == 8
     field = tp.LensingSurveyField(lenses=tp.LensCatalog(**lens),
                                   convergence=tp.KappaMap(**src))
-    with pytest.raises(tp.TpcfError) as e:   # the adjoint has no spherical form
-        tp.adjoint(KAPPA, lens, src, bins, np.ones(5), np.zeros(5), spherical=True, ctx=ctx)
+    many = tp.create_bins(1.0, 10.0, 64, tp.LOG)
+    with pytest.raises(tp.TpcfError) as e:   # the adjoint takes at most 63 bins
+        tp.adjoint(KAPPA, lens, src, many, np.ones(64), np.zeros(64), ctx=ctx)
     assert e.value.code == 5
     # the context stays usable after errors
     g = tp.kappa_2pcf(field, bins, None, False, 1, True, False)
