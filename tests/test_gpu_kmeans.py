@@ -87,6 +87,13 @@ def test_kmeans_refusals(tp, ctx):
         tp.kmeans(x, y, w, 1, ctx=ctx)
     with pytest.raises(tp.TpcfError):
         tp.kmeans(x, y, w, 2000, ctx=ctx)
+    for bad in (dict(weight_equalizer_scaling=-1.0), dict(weight_equalizer_scaling=float("nan")),
+                dict(equal_size=True, weight_tol=-0.1)):
+        with pytest.raises(tp.TpcfError) as e:
+            tp.kmeans(x, y, w, 8, ctx=ctx, **bad)
+        assert e.value.code == tp._native.ERR_ARG, bad
+    # the context stays usable after the refusals
+    assert tp.kmeans(x, y, w, 8, ctx=ctx)["iterations"] >= 1
 
 
 def test_kmeans_reproducible(tp, ctx):
