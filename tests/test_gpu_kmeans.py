@@ -118,10 +118,10 @@ def test_kmeans_weight_equaliser(tp, orc, ctx, dtype):
     assert np.mean(plain["labels"] == o["labels"]) < 0.99   # the equaliser changes the labels
 
 
-@pytest.mark.parametrize("case", ["weighted", "unit_weights", "equalised", "sphere"])
+@pytest.mark.parametrize("case", ["weighted", "unit_weights", "equalised", "sphere", "sphere_equalised"])
 def test_kmeans_equal_size(tp, orc, ctx, case):
     r = np.random.default_rng(31)
-    sph = case == "sphere"
+    sph = case.startswith("sphere")
     if sph:
         n, k = 60000, 24
         x = r.uniform(1.0, 1.6, n)
@@ -133,7 +133,7 @@ def test_kmeans_equal_size(tp, orc, ctx, case):
         k, size = 40, 100.0
         if case == "unit_weights":
             w = np.ones_like(w)
-    s = 3.0 if case == "equalised" else 0.0
+    s = 3.0 if case.endswith("equalised") else 0.0
     for tol in (0.0, 0.05):
         g = tp.kmeans(x, y, w, k, weight_equalizer_scaling=s, equal_size=True, weight_tol=tol, max_iterations=40,
                       spherical=sph, ctx=ctx)
