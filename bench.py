@@ -502,6 +502,9 @@ def main() -> int:
     ap.add_argument("--host-io", action="store_true",
                     help="host arrays in and out (the C-ABI as a host driver calls it: H2D of the catalogues "
                          "and D2H of every output inside each step)")
+    ap.add_argument("--e2e-steps", type=int, default=2,
+                    help="after the timed steps, time this many calls with host arrays (H2D inside the step) "
+                         "and report them as end_to_end (0: skip)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device (reference kernel for HBM counters)")
     ap.add_argument("--source-hash", action="store_true",
@@ -608,6 +611,20 @@ def main() -> int:
                                                  f"profiled on the kernel of this source hash)")
         else:
             out["roofline"]["traffic_source"] = "no profile of this workload on this pair kernel's sources"
+    if not args.host_io and args.e2e_steps > 0:
+        # the same call end to end through the C-ABI as a host driver makes it:
+        # host arrays in (H2D of the catalogues) and per-bin outputs out; never
+        # `value`, reported beside it
+        host_step()
+        t1 = time.perf_counter()
+        e2e_pairs = 0
+        for _ in range(args.e2e_steps):
+            e2e_pairs += int(host_step().n_pairs.sum())
+        e2e_s = (time.perf_counter() - t1) / args.e2e_steps
+        out["end_to_end"] = {"io": "host arrays in and out (H2D of the inputs inside each step)",
+                             "steps": args.e2e_steps, "ms_per_step": e2e_s * 1e3,
+                             "pairs_per_s": e2e_pairs / args.e2e_steps / e2e_s}
+        log(f"end to end with host arrays: {e2e_s * 1e3:.1f} ms per step")
     # the C port has no spherical mode: the spherical CPU baseline needs the reference build
     if not args.no_cpu and (not wl.spherical or _have_ref()):
         threads = cpu_threads()
