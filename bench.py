@@ -92,6 +92,47 @@ def cpu_info() -> dict:
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
+DEVICE: dict = {}   # the GPU as rocminfo reports it, read before this process touches the GPU
+
+
+def device_peaks() -> dict:
+    """The GPU agent's compute units and maximum engine clock from rocminfo,
+    and the vector peaks they give: 16 lanes per SIMD, 2 FLOPs per FMA; packed
+    FP32 doubles the FP32 rate, and on gfx950 an FP64 FMA issues at the
+    non-packed FP32 rate.  Called before the GPU is initialised (rocminfo runs
+    as a child process)."""
+    import re
+    import subprocess
+    try:
+        txt = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60).stdout
+    except (OSError, subprocess.SubprocessError):
+        return {"source": "rocminfo unavailable"}
+    for agent in re.split(r"^Agent \d+\s*$", txt, flags=re.M)[1:]:
+        f = {}
+        for line in agent.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                f.setdefault(k.strip(), v.strip())
+        if f.get("Device Type") != "GPU":
+            continue
+        try:
+            cus, simds = int(f["Compute Unit"]), int(f["SIMDs per CU"])
+            mhz = int(f["Max Clock Freq. (MHz)"])
+        except (KeyError, ValueError):
+            return {"source": "rocminfo: GPU agent without compute unit, SIMD or clock fields"}
+        fp32 = cus * simds * 16 * 2 * mhz * 1e6 / 1e12
+        return {"source": "rocminfo", "name": f.get("Name"), "marketing_name": f.get("Marketing Name"),
+                "compute_units": cus, "simds_per_cu": simds, "max_clock_mhz": mhz,
+                "fp32_tflops": round(fp32, 2), "fp32_packed_tflops": round(2 * fp32, 2),
+                "fp64_tflops": round(fp32, 2)}
+    return {"source": "rocminfo: no GPU agent"}
+
+
+def peak_note() -> dict:
+    """Where `peak` comes from, beside the figures rocminfo gives."""
+    return {"peak_source": "MI355X published vector peak (FP32 packed, or FP64)", "device": DEVICE}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -253,6 +294,7 @@ def bench_mocks(args) -> int:
                       "cell_target": st["cell_target"], "split": st["split"],
                       "graph_nodes": b.stats["graph_nodes"], "plan_ms": st["ms_plan"]},
            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        **peak_note(),
                         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_pairs (graph replay)",
                         "time_base": "batch run wall time (all kernels of the graph)", "run_ms": run_s * 1e3,
                         "flops_per_pair": {"kappa-kappa": FLOPS_PER_PAIR[1], "shear-scalar": FLOPS_PER_PAIR[0]}}}
@@ -355,6 +397,7 @@ def bench_adjoint(args) -> int:
                                          "pass (its marks give the forward sums), the backward replays it with "
                                          "the source-side marks pass"},
            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        **peak_note(),
                         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_pairs (4 marks passes)",
                         "time_base": "step wall time (forward, chi^2 and backward)",
                         "flops_per_step": flops,
@@ -424,6 +467,7 @@ def bench_kk100k(args) -> int:
                       "theta": [wl.x_min, wl.x_max], "patches": wl.n_patch, "bootstrap_resamples": 200,
                       "pairs_per_step": per_step, "io": "host arrays in and out"},
            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        **peak_note(),
                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_pairs",
                         "kernel_ms": k_ms, "flops_per_pair": FLOPS_PER_PAIR[1]}}
     if not args.no_cpu:
@@ -517,6 +561,7 @@ def main() -> int:
         print("bench.py measures one GPU: the pair-counting path runs on a single MI355X; "
               "sharding over several GPUs is not implemented", file=sys.stderr)
         return 2
+    DEVICE.update(device_peaks())
     if args.config == "kk_100k_auto":
         return bench_kk100k(args)
     if args.config == "mocks_256":
@@ -598,7 +643,7 @@ def main() -> int:
                    "spherical": wl.spherical, "marks": args.marks,
                    "io": "host arrays in and out" if args.host_io else "inputs resident in HBM",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", **peak_note(),
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": fpp,
                      "kernel_source_hash": kernel_source_hash()},
