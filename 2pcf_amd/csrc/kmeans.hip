@@ -18,22 +18,24 @@
 // Weight equaliser (weight_equalizer_scaling s > 0): from the second round a
 // distance to cluster j is scaled by 1 + (W_j / W_mean - 1) s.  The reference
 // moves W_j point by point while a round runs, so its result depends on the
-// visiting order; here W_j is the weight at the end of the previous round
-// (every point of a round sees the same W), which makes the result
-// independent of the order.  Equal size (equal_size): from the second round
-// each point also keeps its runner-up cluster and the ratio of the two
-// distances (src/kmeans.f90:167-180), and after the rounds, when (max W -
-// min W) / W_mean > weight_tol, postprocess_weights (src/kmeans.f90:214-262)
-// runs on the device: clusters in decreasing order of weight (the
-// reference's qsort_arg, restated on the host for the k weights), each
-// handing its members in increasing order of their ratio (a stable radix
-// sort: equal ratios by point index, where the reference's quicksort has its
-// own order) to their runner-up if that cluster comes later, until its weight
-// drops to W_mean (1 + weight_tol).  A member handed over keeps its runner-up
-// as its cluster, so when that cluster's turn comes it is skipped there; each
-// cluster's turn therefore only reads its members at the start of the
-// post-processing (one segment of the sorted points) and the weight it has
-// received, which one workgroup carries from turn to turn.
+// visiting order.  Here one workgroup takes the points in steps of 256
+// consecutive ones: every point of a step sees the weights from before the
+// step, and the step's moves update them before the next step, so the result
+// does not depend on any thread order (k_km_assign_gs).  Equal size
+// (equal_size): from the second round each point also keeps its runner-up
+// cluster and the ratio of the two distances (src/kmeans.f90:167-180), and
+// after the rounds, when (max W - min W) / W_mean > weight_tol,
+// postprocess_weights (src/kmeans.f90:214-262) runs on the device: clusters
+// in decreasing order of weight (the reference's qsort_arg, restated on the
+// host for the k weights), each handing its members in increasing order of
+// their ratio (a stable radix sort: equal ratios by point index, where the
+// reference's quicksort has its own order) to their runner-up if that cluster
+// comes later, until its weight drops to W_mean (1 + weight_tol).  A member
+// handed over keeps its runner-up as its cluster, so when that cluster's turn
+// comes it is skipped there; each cluster's turn therefore only reads its
+// members at the start of the post-processing (one segment of the sorted
+// points) and the weight it has received, which one workgroup carries from
+// turn to turn.
 //
 // One thread per point, the centroids (FP64, k <= 1024) in LDS; per-block
 // partial sums of w (x - x0), w (y - y0), w per cluster in LDS, added to global
@@ -43,7 +45,8 @@
 // of single-precision weights exact), so every add is an integer add: the
 // totals, the centroids and hence the labels do not depend on the order in
 // which the blocks and lanes add, and a run is reproducible bit for bit.  A
-// round is two launches (three with the equaliser) and one 8-byte read-back.
+// round is an assignment launch (one workgroup with the equaliser, one thread
+// per point otherwise), a centroid launch and one 8-byte read-back.
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
@@ -566,6 +569,7 @@ int kmeans(tpcf_ctx *ctx, int64_t n, const T *x, const T *y, const T *w, const t
     if (n < 1 || n >= ((int64_t)1 << 31)) return fail(ctx, TPCF_ERR_ARG, "n must be in [1, 2^31)");
     if (k < 2 || k > kKmMax) return fail(ctx, TPCF_ERR_ARG, "k must be in [2, 1024]");
     if (o->max_iterations < 0) return fail(ctx, TPCF_ERR_ARG, "max_iterations must be >= 0");
+    if (o->init != 0 && o->init != 1) return fail(ctx, TPCF_ERR_ARG, "init must be 0 or 1");
     if (!(o->weight_equalizer_scaling >= 0.0) || !std::isfinite(o->weight_equalizer_scaling))
         return fail(ctx, TPCF_ERR_ARG, "weight_equalizer_scaling must be finite and >= 0");
     if (o->equal_size != 0 && o->equal_size != 1) return fail(ctx, TPCF_ERR_ARG, "equal_size must be 0 or 1");

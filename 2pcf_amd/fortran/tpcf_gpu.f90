@@ -618,8 +618,8 @@ contains
     !> cluster_centroids(k, 2).  given_centroids = .true. starts from
     !> cluster_centroids instead of the fixed grid; spherical coordinates are
     !> longitude / latitude in radians.  equal_size, weight_equalizer_scaling
-    !> and weight_tol as the reference's arguments (the equaliser uses the
-    !> cluster weights of the previous round).
+    !> and weight_tol as the reference's arguments (the equaliser moves the
+    !> cluster weights after every step of 256 points, not after every point).
     subroutine partition_data_gpu(data_points, weights, k, cluster_indicies, cluster_centroids, spherical_coords, &
                                   given_centroids, iterations, reassignment_tol, equal_size, &
                                   weight_equalizer_scaling, weight_tol)

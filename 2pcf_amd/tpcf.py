@@ -587,7 +587,8 @@ def kmeans(x, y, w, k: int, *, max_iterations: int = 1000, reassignment_tol: flo
     cluster weights, the rounds run and the points moved in the last round.
     ``centroids`` (k x 2) replaces the fixed starting grid.
     ``weight_equalizer_scaling`` > 0 scales the distances by the clusters'
-    weights of the previous round; ``equal_size`` adds the post-processing
+    weights, moved after every step of 256 consecutive points (the reference
+    moves them after every point); ``equal_size`` adds the post-processing
     that hands points to their runner-up clusters (``weight_tol``)."""
     ctx = ctx or default_context()
     prec = precision or _precision_of(x)

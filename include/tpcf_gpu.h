@@ -409,15 +409,17 @@ int tpcf_bootstrap(tpcf_ctx *ctx, const tpcf_blocks *blocks, int32_t n_boot, con
  * starting centroids.  labels are 0-based.  TPCF_SPHERICAL: longitude /
  * latitude in radians, great-circle distances.
  * weight_equalizer_scaling s > 0: from the second round the distance to
- * cluster j is scaled by 1 + (W_j / W_mean - 1) s with W_j the cluster's
- * weight at the end of the previous round (the reference updates W_j point
- * by point within a round, which makes its result depend on the visiting
- * order; this form does not).  equal_size 1: the runner-up clusters are
+ * cluster j is scaled by 1 + (W_j / W_mean - 1) s, with the cluster weights
+ * W_j moved after every step of 256 consecutive points (every point of a step
+ * sees the weights from before it; the reference moves them after every
+ * point).  The result does not depend on any thread order.  Such rounds run
+ * in one workgroup.  equal_size 1: the runner-up clusters are
  * tracked from the second round and, when (max W - min W) / W_mean exceeds
  * weight_tol after the rounds, postprocess_weights (src/kmeans.f90:214-262)
  * hands points to their runner-up clusters (equal runner-up ratios are taken
  * in point order).  centroids are those of the last round; cluster_weights
- * those after the hand-overs.  2 <= k <= 1024; FP64 arithmetic. */
+ * those after the hand-overs.  2 <= k <= 1024; init, equal_size in {0, 1};
+ * s finite and >= 0; FP64 arithmetic. */
 typedef struct tpcf_kmeans_opts {
     int32_t k;
     int32_t max_iterations;            /* the reference's default: 1000 */

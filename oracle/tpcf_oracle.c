@@ -501,7 +501,7 @@ void orc_bin_of_d2_array(int64_t n, const double *d2, const double *e2, int n_bi
  * distances move with the points: block = 1 updates them after every point,
  * as the reference's single thread does; block = B > 1 takes the points in
  * blocks of B consecutive ones, each point of a block seeing the weights from
- * before its block (B = 1024: the GPU's order); block <= 0 uses the weights
+ * before its block (B = 256: the GPU's order); block <= 0 uses the weights
  * of the previous round for the whole round.  With s = 0 all are the same.
  * equal_size = 1 (the equal_weight_blocks scheme, src/tpcf.f90:1104-1112):
  * from the second round on each point also keeps its candidate, the

@@ -92,6 +92,15 @@ def test_kmeans_refusals(tp, ctx):
         with pytest.raises(tp.TpcfError) as e:
             tp.kmeans(x, y, w, 8, ctx=ctx, **bad)
         assert e.value.code == tp._native.ERR_ARG, bad
+    # a starting mode other than 0 (grid) or 1 (given centroids), through the C-ABI
+    import ctypes as C
+    N = tp._native
+    o = N.KmeansOpts()
+    o.k, o.max_iterations, o.init = 8, 5, 2
+    lab, cen, cw = np.zeros(len(x), np.int32), np.zeros((8, 2)), np.zeros(8)
+    rc = N.lib().tpcf_kmeans_f64(ctx.handle, len(x), x.ctypes.data, y.ctypes.data, w.ctypes.data, C.byref(o), 0,
+                                 lab.ctypes.data, cen.ctypes.data, cw.ctypes.data, None, None)
+    assert rc == N.ERR_ARG and "init" in ctx.last_error()
     # the context stays usable after the refusals
     assert tp.kmeans(x, y, w, 8, ctx=ctx)["iterations"] >= 1
 
