@@ -47,6 +47,19 @@ __device__ __forceinline__ double rsq_pos(double d2)
 
 __device__ __forceinline__ double sqrt_pos(double d2) { return d2 * rsq_pos(d2); }
 
+// sqrt(d2) for d2 >= 0 from the hardware reciprocal square root r of d2 +
+// 2^-1022 and one Newton-Raphson step on the root itself, s0 = d2 r,
+// s = s0 + (r/2) (d2 - s0^2): a seed error e leaves 3 e^2 / 2, as rsq_pos,
+// in one instruction fewer than d2 rsq_pos(d2).  d2 + 2^-1022 is d2 itself
+// for every d2 >= 2^-969, and d2 == 0 gives 0 without a compare and a
+// select.
+__device__ __forceinline__ double sqrt_nr(double d2)
+{
+    const double r = __builtin_amdgcn_rsq(d2 + 0x1p-1022);
+    const double s0 = d2 * r;
+    return fma(0.5 * r, fma(-s0, s0, d2), s0);
+}
+
 // 1/s for s > 0 from the hardware reciprocal and one Newton-Raphson step
 // r <- r + r (1 - s r): seed within 4.7e-8, result within 2.3e-15 (measured
 // as for rsq_pos).
@@ -81,7 +94,7 @@ struct Geo<double> {
         d = d2 * r;
         inv = r * r;
     }
-    static __device__ inline double dist(double d2) { return d2 > 0.0 ? sqrt_pos(d2) : 0.0; }
+    static __device__ inline double dist(double d2) { return sqrt_nr(d2); }
 };
 
 // d^2 < edge test for the squared edge with index e.  FP32: decided in FP32
@@ -287,6 +300,30 @@ __device__ __forceinline__ double asin_ratio(double x2)
     return fma(t, x2, 1.0);
 }
 
+// a b + c in one VOP3 FMA, with the constant c read from a scalar register
+// pair.  Left to itself the compiler keeps an FP64 constant in vector
+// registers and forms the two-address FMA whose accumulator is the constant,
+// which costs a register copy for every term of a series.
+__device__ __forceinline__ double fma_sc(double a, double b, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+
+// asin_ratio(d2 / 4) as a series in d2 itself: the coefficients divided by
+// 4^k, which changes no bit (every Horner step is the one of asin_ratio
+// scaled by a power of two), and the quartering of d2 is saved.
+__device__ __forceinline__ double asin_ratio_c2(double d2)
+{
+    double t = fma_sc(d2, 231.0 / 13312.0 / 4096.0, 63.0 / 2816.0 / 1024.0);
+    t = fma_sc(t, d2, 35.0 / 1152.0 / 256.0);
+    t = fma_sc(t, d2, 5.0 / 112.0 / 64.0);
+    t = fma_sc(t, d2, 3.0 / 40.0 / 16.0);
+    t = fma_sc(t, d2, 1.0 / 6.0 / 4.0);
+    return fma(t, d2, 1.0);
+}
+
 // 2 asin(x) for x^2 = x2 in [0.025, 1] (18 to 180 degrees): the half-angle
 // step asin(x) = 2 asin(x'), x'^2 = x^2 / (2 (1 + sqrt(1 - x^2))), taken until
 // x'^2 < 0.025 (at most four times: sin^2(pi/32) < 0.025), then the series.
@@ -305,15 +342,17 @@ __device__ __noinline__ double sph_angle_wide(double x2)
 }
 
 // Separation angle from the squared chord c^2 = d2: 2 asin(x) with x = c/2,
-// = c asin(x)/x; the series below x^2 = d2/4 = 0.025 (18 degrees).
+// = c asin(x)/x; the series below x^2 = d2/4 = 0.025 (18 degrees).  A zero
+// chord gives 0 (sqrt_nr).
 __device__ __forceinline__ double sph_angle(double d2)
 {
-    if (d2 < 0.1) return d2 > 0.0 ? sqrt_pos(d2) * asin_ratio(0.25 * d2) : 0.0;
+    if (d2 < 0.1) return sqrt_nr(d2) * asin_ratio_c2(d2);
     return sph_angle_wide(fmin(0.25 * d2, 1.0));
 }
 
-// Separation, 1/d^2 (or 1/sin^2 d) and the projection terms of one pair; d
-// and inv are 0 at zero separation (and at an antipodal pair on the sphere).
+// Separation, 1/d^2 (or 1/sin^2 d) and the projection terms of one pair.  At
+// zero separation d, u and v are 0, and so is inv on the flat sky with GUARD
+// (on the sphere inv stays finite there, and inv u, inv v are 0).
 template <typename T, int SPH, bool GUARD>
 __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T, SPH> &g, T &d, T &inv, T &u, T &v)
 {
@@ -330,7 +369,10 @@ __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T,
         T N = fma(L.fn0, g.a, fma(L.fn1, g.b, L.fn2 * g.c));
         if constexpr (SPH == 2) N = fma((T)0.5 * g.d2, g.pu, -N);
         const T s2 = fma(E, E, N * N);
-        inv = (s2 > (T)0) ? rcp_pos(s2) : (T)0;
+        // s2 == 0 (zero separation, or E and N below 1e-154) leaves u = v =
+        // 0, so the floor at the smallest normal number keeps the pair's
+        // terms at 0 without a compare and a select
+        inv = rcp_pos(fmax(s2, 0x1p-1022));
         u = fma(E, E, -(N * N));
         v = E * N;
         d = sph_angle(g.d2);
@@ -1197,6 +1239,7 @@ __attribute__((amdgpu_waves_per_eu(pair_waves<T, MZ, SPH>(), 8))) void k_pairs(
     unsigned long long *__restrict__ probe, const double *__restrict__ sph_lp, const double *__restrict__ sph_lf, const double2 *__restrict__ sph_segp,
     const double2 *__restrict__ sph_chp)
 {
+    static_assert(!SPH || sizeof(T) == 8, "spherical geometry runs in FP64");
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int64_t b = blockIdx.x;
