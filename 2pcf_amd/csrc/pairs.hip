@@ -24,6 +24,7 @@
 // changes.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "internal.hpp"
 
@@ -253,7 +254,7 @@ struct LaneData {
 // lens, u = E^2 - N^2, v = E N, inv = 1/(E^2 + N^2) = 1/sin^2 d, which are the
 // reference's cos 2phi and sin 2phi from angles_sphere (src/utils.f90:618-639)
 // without trigonometry; the separation angle is 2 asin(chord / 2).
-// SPH == 2 is the sphere with the roles of the catalogues swapped (the
+// sph_rev(SPH) is the sphere with the roles of the catalogues swapped (the
 // adjoint's source-side pass: the lanes hold sources, the stream holds
 // lenses) while the angle stays the one at the lens, i.e. at the streamed
 // object A.  With P the pole and D = u_A - u_B, the direction at A toward B
@@ -261,10 +262,16 @@ struct LaneData {
 // (from e_A = P x u_A / cos lat_A, n_A = u_A x e_A); the lane's frame slots
 // hold u_B x P and P, and pu = P . u_A.  The common factor drops out of u, v
 // and inv.
+// SPH: 0 the flat sky, 1 the sphere, 2 the sphere with the catalogues' roles
+// swapped; plus 4 where every chord of the segment visit is below kSphShort
+// (the pair kernel picks that variant per visit).
+constexpr bool sph_rev(int sph) { return (sph & 3) == 2; }
+constexpr bool sph_short(int sph) { return (sph & 4) != 0; }
+
 template <typename T, int SPH>
 struct PairGeo {
     T a, b, c, d2;
-    T pu;   // SPH == 2: P . u_A of the streamed object
+    T pu;   // sph_rev(SPH): P . u_A of the streamed object
 };
 
 template <typename T, int SPH>
@@ -281,7 +288,7 @@ __device__ __forceinline__ PairGeo<T, SPH> pair_geo(const LaneData<T> &L, const 
         g.b = q.x - L.xa;
         g.c = q.y - L.ya;
         g.d2 = fma(g.a, g.a, fma(g.b, g.b, g.c * g.c));
-        if constexpr (SPH == 2) g.pu = fma(L.fn0, q.s1w2, fma(L.fn1, q.x, L.fn2 * q.y));
+        if constexpr (sph_rev(SPH)) g.pu = fma(L.fn0, q.s1w2, fma(L.fn1, q.x, L.fn2 * q.y));
     }
     return g;
 }
@@ -313,12 +320,22 @@ __device__ __forceinline__ double fma_sc(double a, double b, double c)
 
 // asin_ratio(d2 / 4) as a series in d2 itself: the coefficients divided by
 // 4^k, which changes no bit (every Horner step is the one of asin_ratio
-// scaled by a power of two), and the quartering of d2 is saved.
+// scaled by a power of two), and the quartering of d2 is saved.  SHORT: the
+// terms to x^8 only, for d2 < kSphShort, where the first omitted term,
+// 63/2816 (d2/4)^5, is at most 1.0e-16 of the sum.
+constexpr double kSphShort = 5.4e-3;
+
+template <bool SHORT>
 __device__ __forceinline__ double asin_ratio_c2(double d2)
 {
-    double t = fma_sc(d2, 231.0 / 13312.0 / 4096.0, 63.0 / 2816.0 / 1024.0);
-    t = fma_sc(t, d2, 35.0 / 1152.0 / 256.0);
-    t = fma_sc(t, d2, 5.0 / 112.0 / 64.0);
+    double t;
+    if constexpr (SHORT) {
+        t = fma_sc(d2, 35.0 / 1152.0 / 256.0, 5.0 / 112.0 / 64.0);
+    } else {
+        t = fma_sc(d2, 231.0 / 13312.0 / 4096.0, 63.0 / 2816.0 / 1024.0);
+        t = fma_sc(t, d2, 35.0 / 1152.0 / 256.0);
+        t = fma_sc(t, d2, 5.0 / 112.0 / 64.0);
+    }
     t = fma_sc(t, d2, 3.0 / 40.0 / 16.0);
     t = fma_sc(t, d2, 1.0 / 6.0 / 4.0);
     return fma(t, d2, 1.0);
@@ -343,10 +360,12 @@ __device__ __noinline__ double sph_angle_wide(double x2)
 
 // Separation angle from the squared chord c^2 = d2: 2 asin(x) with x = c/2,
 // = c asin(x)/x; the series below x^2 = d2/4 = 0.025 (18 degrees).  A zero
-// chord gives 0 (sqrt_nr).
+// chord gives 0 (sqrt_nr).  SHORT: d2 < kSphShort is known for every pair.
+template <bool SHORT>
 __device__ __forceinline__ double sph_angle(double d2)
 {
-    if (d2 < 0.1) return sqrt_nr(d2) * asin_ratio_c2(d2);
+    if constexpr (SHORT) return sqrt_nr(d2) * asin_ratio_c2<true>(d2);
+    if (d2 < 0.1) return sqrt_nr(d2) * asin_ratio_c2<false>(d2);
     return sph_angle_wide(fmin(0.25 * d2, 1.0));
 }
 
@@ -367,7 +386,7 @@ __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T,
     } else {
         const T E = fma(L.fe0, g.a, fma(L.fe1, g.b, L.fe2 * g.c));
         T N = fma(L.fn0, g.a, fma(L.fn1, g.b, L.fn2 * g.c));
-        if constexpr (SPH == 2) N = fma((T)0.5 * g.d2, g.pu, -N);
+        if constexpr (sph_rev(SPH)) N = fma((T)0.5 * g.d2, g.pu, -N);
         const T s2 = fma(E, E, N * N);
         // s2 == 0 (zero separation, or E and N below 1e-154) leaves u = v =
         // 0, so the floor at the smallest normal number keeps the pair's
@@ -375,7 +394,7 @@ __device__ __forceinline__ void pair_proj(const LaneData<T> &L, const PairGeo<T,
         inv = rcp_pos(fmax(s2, 0x1p-1022));
         u = fma(E, E, -(N * N));
         v = E * N;
-        d = sph_angle(g.d2);
+        d = sph_angle<sph_short(SPH)>(g.d2);
     }
 }
 
@@ -818,7 +837,7 @@ __device__ __forceinline__ void multi_sweep(Win &w, const LaneData<T> &L, const 
                     t2 = z0 ? (T)__builtin_nan("") : t2;
                 }
             } else {
-                if constexpr (SPH) d = sph_angle(d2);
+                if constexpr (SPH) d = sph_angle<sph_short(SPH)>(d2);
                 else d = Geo<T>::dist(d2);
                 t1 = q.s1w;
                 t2 = q.s2w;
@@ -934,6 +953,42 @@ __device__ __forceinline__ void multi_flush(Win &w, const LaneData<T> &L, int n_
 
 // The work of one item: lens chunk item.chunk against the source segments in
 // its band of cell rows, into the per-XCD accumulator copy `slot`.
+// Runs f with the spherical variant of a segment visit: SPH + 4 (the short
+// angle series) when every chord of the visit lies below kSphShort (hi2 is
+// an upper bound on the visit's squared chords), else SPH.
+template <int SPH, typename T, typename F>
+__device__ __forceinline__ void sph_visit(T hi2, F &&f)
+{
+    static_assert(SPH == 1 || SPH == 2, "spherical passes only");
+    if (hi2 < (T)kSphShort) f(std::integral_constant<int, SPH | 4>());
+    else f(std::integral_constant<int, SPH>());
+}
+
+// Fast class: every pair of the segment is in one bin, so only the terms
+// that depend on the pair are summed here (Theta, and G1, G2 for shear); the
+// others come from the segment's sums.
+template <typename T, int KIND, int SPH>
+__device__ __forceinline__ void fast_sums(const LaneData<T> &L, const SrcRec<T> *__restrict__ r, uint32_t cnt, T &sT,
+                                          T &sG1, T &sG2)
+{
+    if constexpr (KIND == SHEAR) {
+        stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
+            const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+            T d, inv, u, v;
+            pair_proj<T, SPH, false>(L, g, d, inv, u, v);
+            sT = fma(d, q.w, sT);
+            sG1 = fma(fma(q.s1w, u, q.s2w2 * v), inv, sG1);
+            sG2 = fma(fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)), inv, sG2);
+        });
+    } else {
+        stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
+            const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+            if constexpr (SPH) sT = fma(sph_angle<sph_short(SPH)>(g.d2), q.w, sT);
+            else sT = fma(Geo<T>::dist(g.d2), q.w, sT);
+        });
+    }
+}
+
 template <typename T, int KIND, bool MARKS, bool MZ, int SPH>
 __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T> &C, const Item item, const int slot,
                                           const int lane, double *acc, const T e2_lane,
@@ -1097,20 +1152,25 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                         win_open<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, pb, k0);
                     }
                     T sT = 0, sG1 = 0, sG2 = 0;
-                    if (KIND == SHEAR) {
-                        stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
-                            const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
-                            T d, inv, u, v;
-                            pair_proj<T, SPH, false>(L, g, d, inv, u, v);
-                            sT = fma(d, q.w, sT);
-                            sG1 = fma(fma(q.s1w, u, q.s2w2 * v), inv, sG1);
-                            sG2 = fma(fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)), inv, sG2);
-                        });
+                    if constexpr (SPH == 0) {
+                        if (KIND == SHEAR) {
+                            stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
+                                const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+                                T d, inv, u, v;
+                                pair_proj<T, SPH, false>(L, g, d, inv, u, v);
+                                sT = fma(d, q.w, sT);
+                                sG1 = fma(fma(q.s1w, u, q.s2w2 * v), inv, sG1);
+                                sG2 = fma(fma(q.s2w, u, -(s1w2_of<T, SPH>(q) * v)), inv, sG2);
+                            });
+                        } else {
+                            stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
+                                const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
+                                sT = fma(Geo<T>::dist(g.d2), q.w, sT);
+                            });
+                        }
                     } else {
-                        stream2<T>(r, cnt, [&](const SrcRec<T> &q) __attribute__((always_inline)) {
-                            const PairGeo<T, SPH> g = pair_geo<T, SPH>(L, q);
-                            if constexpr (SPH) sT = fma(sph_angle(g.d2), q.w, sT);
-                            else sT = fma(Geo<T>::dist(g.d2), q.w, sT);
+                        sph_visit<SPH>(hi2, [&](auto S) __attribute__((always_inline)) {
+                            fast_sums<T, KIND, decltype(S)::value>(L, r, cnt, sT, sG1, sG2);
                         });
                     }
                     const int o = (kmin - w.wk) * WQ;
@@ -1148,13 +1208,37 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                         // x_max inside the range: only bin n_bin - 1 counts (the
                         // edge pass would derive it as total - the pairs beyond
                         // x_max; at x_min it sums bin 0 directly already)
-                        edge_side_pass<T, KIND, MZ, false, SPH>(w, L, A, r, cnt, n_bin);
+                        if constexpr (SPH == 0) {
+                            edge_side_pass<T, KIND, MZ, false, SPH>(w, L, A, r, cnt, n_bin);
+                        } else {
+                            sph_visit<SPH>(hi2, [&](auto S) __attribute__((always_inline)) {
+                                edge_side_pass<T, KIND, MZ, false, decltype(S)::value>(w, L, A, r, cnt, n_bin);
+                            });
+                        }
                     } else if (lo2 > (T)0) {
-                        edge_pass<T, KIND, MZ, false, false, SPH>(w, L, A, r, cnt, n_bin);
+                        if constexpr (SPH == 0) {
+                            edge_pass<T, KIND, MZ, false, false, SPH>(w, L, A, r, cnt, n_bin);
+                        } else {
+                            sph_visit<SPH>(hi2, [&](auto S) __attribute__((always_inline)) {
+                                edge_pass<T, KIND, MZ, false, false, decltype(S)::value>(w, L, A, r, cnt, n_bin);
+                            });
+                        }
                     } else if (KIND == SHEAR && A.xmin_zero && kmin == 0) {
-                        edge_pass<T, KIND, MZ, true, true, SPH>(w, L, A, r, cnt, n_bin);
+                        if constexpr (SPH == 0) {
+                            edge_pass<T, KIND, MZ, true, true, SPH>(w, L, A, r, cnt, n_bin);
+                        } else {
+                            sph_visit<SPH>(hi2, [&](auto S) __attribute__((always_inline)) {
+                                edge_pass<T, KIND, MZ, true, true, decltype(S)::value>(w, L, A, r, cnt, n_bin);
+                            });
+                        }
                     } else {
-                        edge_pass<T, KIND, MZ, true, false, SPH>(w, L, A, r, cnt, n_bin);
+                        if constexpr (SPH == 0) {
+                            edge_pass<T, KIND, MZ, true, false, SPH>(w, L, A, r, cnt, n_bin);
+                        } else {
+                            sph_visit<SPH>(hi2, [&](auto S) __attribute__((always_inline)) {
+                                edge_pass<T, KIND, MZ, true, false, decltype(S)::value>(w, L, A, r, cnt, n_bin);
+                            });
+                        }
                     }
                 } else {
                     cls = 2;
@@ -1167,12 +1251,25 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
                         // a zero separation is in range only at x_min == 0, in
                         // bin 0, with the boxes overlapping
                         const bool zsep = KIND == SHEAR && A.xmin_zero && k0 == 0 && lo2 == (T)0;
-                        if (zc) {
-                            if (zsep) multi_sweep<T, KIND, MZ, true, true, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
-                            else multi_sweep<T, KIND, MZ, true, false, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                        if constexpr (SPH == 0) {
+                            if (zc) {
+                                if (zsep) multi_sweep<T, KIND, MZ, true, true, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                                else multi_sweep<T, KIND, MZ, true, false, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                            } else {
+                                if (zsep) multi_sweep<T, KIND, MZ, false, true, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                                else multi_sweep<T, KIND, MZ, false, false, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                            }
                         } else {
-                            if (zsep) multi_sweep<T, KIND, MZ, false, true, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
-                            else multi_sweep<T, KIND, MZ, false, false, SPH>(w, L, A, r, zr, cnt, n_bin, k0);
+                            sph_visit<SPH>(hi2, [&](auto S) __attribute__((always_inline)) {
+                                constexpr int S2 = decltype(S)::value;
+                                if (zc) {
+                                    if (zsep) multi_sweep<T, KIND, MZ, true, true, S2>(w, L, A, r, zr, cnt, n_bin, k0);
+                                    else multi_sweep<T, KIND, MZ, true, false, S2>(w, L, A, r, zr, cnt, n_bin, k0);
+                                } else {
+                                    if (zsep) multi_sweep<T, KIND, MZ, false, true, S2>(w, L, A, r, zr, cnt, n_bin, k0);
+                                    else multi_sweep<T, KIND, MZ, false, false, S2>(w, L, A, r, zr, cnt, n_bin, k0);
+                                }
+                            });
                         }
                         multi_flush<T, KIND, MARKS, MZ>(w, L, n_bin, C.marks, k0);
                     }
