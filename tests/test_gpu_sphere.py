@@ -14,7 +14,11 @@ crosses longitude 0 (FP32 inputs, so the FP32 entry point widens them); a
 kappa-kappa auto-correlation across longitude 0; bins from 5 to 150 degrees
 over the whole sphere (separations above 18 degrees take the half-angle path
 of the angle, not the series alone); the Python shear_2pcf wrapper with
-spherical_coords; the adjoint refusing spherical coordinates.
+spherical_coords (the spherical adjoint is tested in
+test_gpu_adjoint_sphere.py).
+The first three cases' bins span the separation of about 4.2 degrees
+(squared chord 5.4e-3) below which a segment visit takes the short angle
+series, so visits of both variants are compared.
 """
 import numpy as np
 import pytest
