@@ -149,6 +149,40 @@ def test_shear_masks_and_zero_weights(tp, orc, ctx):
     check(tp, orc, SHEAR, lens, src, bins, marks=True, ctx=ctx)
 
 
+@pytest.mark.parametrize("kind", [KAPPA, SHEAR])
+def test_tiling_leaves_results_unchanged(tp, orc, kind):
+    """The source cell side (TPCF_OPT_CELL_SIZE) and the work-item band
+    height (TPCF_OPT_SPLIT) only change how the pairs are visited: which
+    segment visits fall into the fast, edge and multi classes, and how a
+    chunk's rows are cut into items.  Counts stay bit-exact against the oracle
+    and the sums within the bar, with zero weights, patches and marks (no z
+    masks: a segment whose z range may fail the z test is always swept as a
+    multi visit), from cells of 1/50 x_max to cells as wide as x_max: the
+    class counters (TPCF_OPT_COUNTERS) show the examined pairs moving from
+    mostly edge visits (one bin edge in the visit's distance range) to mostly
+    multi visits."""
+    lens, src = uniform_case(61 + kind, 6000, 8000, 60.0, kind, zero_w=True)
+    bins = tp.create_bins(1.0, 40.0, 5, tp.LOG)
+    r = _rng(9)
+    pa = r.integers(0, 4, len(lens["x"])).astype(np.int32)
+    pb = r.integers(0, 3, len(src["x"])).astype(np.int32)
+    c = tp.Context()
+    try:
+        c.set_option(2, 1.0)
+        shares = {}
+        for cell, split in ((0.75, 0.2), (3.0, 0.1), (12.0, 0.5), (40.0, 1.0), (0.0, 0.2)):
+            c.set_option(1, cell)
+            c.set_option(4, split)
+            g, _ = check(tp, orc, kind, lens, src, bins, marks=True, patch_A=pa, patch_B=pb, n_pa=4, n_pb=3, ctx=c)
+            pv = np.asarray(g.stats["pair_visits"], np.float64)
+            shares[cell] = pv / pv.sum()
+            print(f"cell {cell} split {split}: fast / edge / multi shares of the examined pairs {shares[cell]}")
+        assert shares[0.75][1] > 0.5 and shares[40.0][2] > 0.5, shares
+        assert shares[40.0][2] > shares[0.75][2] + 0.3, shares
+    finally:
+        c.close()
+
+
 def test_shear_marks_m_zero_cancelling_weights(tp, orc, ctx):
     """m == 0 (the K marks are copied from the W marks after the kernel) and
     lenses whose first bin holds only two sources of weights +w and -w: the
