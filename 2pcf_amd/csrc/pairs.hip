@@ -2,7 +2,7 @@
 // (src/tpcf.f90:266-359) and scalar_kappa_tpcf (src/tpcf.f90:456-526).
 //
 // One wavefront owns one work item: a lens chunk (up to 64 lenses, one per
-// lane) and a round-robin share of its cell rows.  It walks the source
+// lane) and one band of the cell rows it can reach.  It walks the source
 // segments within x_max of the chunk's bounding box, source patch by source
 // patch, row by row.  Source records are
 // wave-uniform, so they are read with scalar loads and every lane pairs its
@@ -18,10 +18,13 @@
 // Each segment's pairs are summed per lane in the input precision (at most
 // kSegMax terms), then added to per-lane FP64 sums for a window of two
 // adjacent bins (fast, edge) or reduced straight from the sweep's rows
-// (multi).  When the window moves these are reduced across the wave (DPP)
-// into FP64 per-bin registers (lane b holds bin b), which go to the
-// per-patch-pair FP64 accumulators with global atomics when the source patch
-// changes.
+// (multi).  When the window moves these are reduced across the wave
+// (through LDS rows) into FP64 per-bin registers (lane b holds bin b), which
+// go to the per-patch-pair FP64 accumulators with global atomics when the
+// source patch changes.  On the sphere (SPH) the same passes run on FP64
+// unit vectors: squared chords against squared chord edges, the position
+// angle from the east and north components at the lens.  Pair counts are
+// integers throughout.
 #include <algorithm>
 #include <cmath>
 #include <type_traits>

@@ -52,14 +52,25 @@ METRIC = "pairs/sec on 1 MI355X vs host-core ref; FP32+FP64-edge vs FP64 at ≤1
 KERNEL_SOURCES = ("2pcf_amd/csrc/pairs.hip", "2pcf_amd/csrc/prepare.hip", "2pcf_amd/csrc/internal.hpp")
 
 
+def code_of(text: str) -> str:
+    """C++ source with its comments and blank lines removed and each line's
+    trailing blanks dropped: the part of a source file the compiler sees (the
+    pair kernel's sources hold no "//" or "/*" inside string literals)."""
+    import re
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return "\n".join(line.rstrip() for line in text.splitlines() if line.strip())
+
+
 def kernel_source_hash() -> str:
-    """sha256 (16 hex digits) over the pair kernel's sources: a profile's
-    traffic figure is reused only for the kernel it was measured on."""
+    """sha256 (16 hex digits) over the code of the pair kernel's sources,
+    comments excluded: a profile's traffic figure is reused only for the
+    kernel it was measured on, and a change to a comment does not change it."""
     import hashlib
     h = hashlib.sha256()
     for p in KERNEL_SOURCES:
-        with open(os.path.join(ROOT, p), "rb") as f:
-            h.update(f.read())
+        with open(os.path.join(ROOT, p), encoding="utf-8") as f:
+            h.update(code_of(f.read()).encode())
     return h.hexdigest()[:16]
 
 
