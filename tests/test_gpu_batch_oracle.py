@@ -11,7 +11,9 @@ oracle gives for that catalogue:
   relative, G against the sum of |terms|); marks within the same bars, the
   signed ones against the lens-bin sum of |terms|;
 * a spherical batch (FP32 and FP64 inputs) the same way, counts exact apart
-  from pairs whose squared chord lies within 1e-9 of an edge (none here);
+  from pairs whose squared chord lies within 1e-9 of an edge (none here):
+  shear with m, kappa-kappa, and shear without an m array (the m == 0
+  kernel);
 * stacking (src/main.f90:678-695): for fields far apart, the weight-stacked
   per-field results of a batch equal one correlation of the union (K = W for
   kappa-kappa and for shear with m = 0, so sum_f W_f g_f / sum_f W_f =
@@ -178,6 +180,8 @@ def test_batch_spherical_against_oracle(tp, orc, ctx, precision):
             lat = np.clip(lat0 + r.uniform(-8, 8, n) * deg, -np.pi / 2, np.pi / 2)
             c[side]["x"] = torch.from_numpy(lon.astype(dt)).to(DEV)
             c[side]["y"] = torch.from_numpy(lat.astype(dt)).to(DEV)
+        if i == 2:
+            c["src"].pop("m")   # no m array: the batch takes the m == 0 kernel for this catalogue
         c["lens"] = {k: v.to(torch.float32 if dt == np.float32 else torch.float64) for k, v in c["lens"].items()}
         c["src"] = {k: v.to(torch.float32 if dt == np.float32 else torch.float64) for k, v in c["src"].items()}
         cats.append(c)
