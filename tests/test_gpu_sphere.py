@@ -13,7 +13,9 @@ patches and marks (FP64 inputs); a cap around the north pole that also
 crosses longitude 0 (FP32 inputs, so the FP32 entry point widens them); a
 kappa-kappa auto-correlation across longitude 0; bins from 5 to 150 degrees
 over the whole sphere (separations above 18 degrees take the half-angle path
-of the angle, not the series alone); the Python shear_2pcf wrapper with
+of the angle, not the series alone); x_min = 0 with every object also on the
+other side (zero separations: angle 0, and NaN position angles for shear as
+in the reference); the Python shear_2pcf wrapper with
 spherical_coords (the spherical adjoint is tested in
 test_gpu_adjoint_sphere.py).
 The first three cases' bins span the separation of about 4.2 degrees
@@ -33,7 +35,8 @@ def unit(lon, lat):
 
 
 def edge_band_pairs(lens, src, e2c, band=1e-9):
-    """Per bin: pairs whose squared chord is within `band` of either edge."""
+    """Per bin: pairs whose squared chord is within `band` of either edge (an
+    edge at 0 is not counted: a zero chord is in bin 0 on both sides)."""
     ua, ub = unit(lens["x"], lens["y"]), unit(src["x"], src["y"])
     out = np.zeros(len(e2c) - 1, dtype=np.int64)
     for i0 in range(0, len(ua), 512):
@@ -41,7 +44,8 @@ def edge_band_pairs(lens, src, e2c, band=1e-9):
         c2 = np.einsum("ijk,ijk->ij", d, d).ravel()
         for k in range(len(e2c) - 1):
             for e in (e2c[k], e2c[k + 1]):
-                out[k] += np.count_nonzero(np.abs(c2 - e) <= band * e)
+                if e > 0:
+                    out[k] += np.count_nonzero(np.abs(c2 - e) <= band * e)
     return out
 
 
@@ -62,8 +66,9 @@ def field(seed, n_a, n_b, kind, lon, lat, masks=True, dtype=np.float64):
     return cast(lens), cast(src)
 
 
-def check(tp, orc, ctx, kind, lens, src, x_min, x_max, n_bin, precision, patches=0, marks=False, tol=1e-10):
-    bins = tp.create_bins(x_min, x_max, n_bin, tp.LOG)
+def check(tp, orc, ctx, kind, lens, src, x_min, x_max, n_bin, precision, patches=0, marks=False, tol=1e-10,
+          spacing=None):
+    bins = tp.create_bins(x_min, x_max, n_bin, tp.LOG if spacing is None else spacing)
     kw = {}
     pa = pb = None
     if patches:
@@ -139,6 +144,51 @@ def test_sphere_wide_angles_all_sky(tp, orc, ctx):
     edges = np.asarray(tp.create_bins(np.radians(5.0), np.radians(150.0), 8, tp.LOG).bin_edges)
     mt = g.sum_theta / g.sum_w
     assert np.all((mt > edges[:-1]) & (mt < edges[1:]))
+
+
+def test_sphere_zero_separation(tp, orc, ctx):
+    """x_min = 0 on linear bins with the same positions on both sides, across
+    longitude 0: every object's self-pair sits at zero separation in bin 0.
+    Kappa-kappa: the self-pairs' angle is 0 and every sum matches the oracle.
+    Shear: their position angle is 0/0, so G_t, G_x of bin 0 and every lens'
+    xi_E / xi_B marks of bin 0 are NaN, as in the oracle (the reference
+    divides by sin d = 0 in angles_sphere, src/utils.f90:618-639); Theta, W,
+    K and the other bins match."""
+    r = np.random.default_rng(36)
+    n = 2000
+    lon = np.mod(r.uniform(-0.06, 0.06, n), 2 * np.pi)
+    lat = r.uniform(0.2, 0.32, n)
+    kap, w = r.normal(0, 0.02, n), r.uniform(0.5, 1.5, n)
+    x_max = np.radians(3.0)
+    check(tp, orc, ctx, KAPPA, dict(x=lon, y=lat, val=kap, w=w),
+          dict(x=lon, y=lat, kappa_E=kap, kappa_B=r.normal(0, 0.02, n), w=w), 0.0, x_max, 6, "f64",
+          spacing=tp.LIN)
+    lens = dict(x=lon, y=lat, z=np.zeros(n), val=kap, w=w)
+    src = dict(x=lon, y=lat, z=np.ones(n), e1=r.normal(0, 0.27, n), e2=r.normal(0, 0.27, n),
+               w=r.uniform(0.5, 1.5, n), m=np.zeros(n))
+    bins = tp.create_bins(0.0, x_max, 6, tp.LIN)
+    g = tp.correlate(SHEAR, lens, src, bins, precision="f64", marks=True, real_bytes=8, ctx=ctx, spherical=True)
+    o = orc.correlate(SHEAR, lens, src, bins.bin_edges, None, None, 1, 1, marks=True, patches=False, spherical=True)
+    amb = edge_band_pairs(lens, src, orc.chord_edges_squared(bins.bin_edges))
+    print("\npairs per bin:", g.n_pairs.tolist(), "near-edge pairs:", amb.tolist())
+    assert amb.sum() == 0
+    np.testing.assert_array_equal(g.n_pairs, o["n_pairs"])
+    assert g.n_pairs[0] >= n
+    for gk, ok in (("sum_theta", "theta"), ("sum_w", "w"), ("sum_k", "k")):
+        assert np.max(np.abs(getattr(g, gk) - o[ok]) / np.abs(o[ok])) <= 1e-10, gk
+    for gk, ok, ak in (("sum_gt", "gt", "gt_abs"), ("sum_gx", "gx", "gx_abs")):
+        have, want = getattr(g, gk), o[ok]
+        assert np.isnan(have[0]) and np.isnan(want[0]), gk
+        assert np.max(np.abs(have[1:] - want[1:]) / o[ak][1:]) <= 1e-10, gk
+    for gk, ok in (("xi_E_marks", "xi_e"), ("xi_B_marks", "xi_b")):
+        a = np.asarray(getattr(g, gk)).T
+        assert np.isnan(a[:, 0]).all() and np.isnan(o[ok][:, 0]).all(), gk
+        scale = np.maximum(np.abs(o[ok][:, 1:]).max(axis=1, keepdims=True), 1e-300)
+        assert np.max(np.abs(a[:, 1:] - o[ok][:, 1:]) / scale) <= 1e-9, gk
+    for gk, ok in (("K_marks", "k_marks"), ("w_marks", "w_marks")):
+        a = np.asarray(getattr(g, gk)).T
+        scale = np.maximum(np.abs(o[ok]).max(axis=1, keepdims=True), 1e-300)
+        assert np.max(np.abs(a - o[ok]) / scale) <= 1e-9, gk
 
 
 def test_sphere_python_wrapper(tp, orc):
