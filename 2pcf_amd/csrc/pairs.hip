@@ -1306,17 +1306,21 @@ __device__ __forceinline__ void pair_item(const PairArgs<T> &A, const CatArgs<T>
     }
 }
 
-// Waves per SIMD the register budget is set for.  The pair loops are
-// latency-bound: on the 20M x 20M shear workload 5 waves take 6.73 s, 6 take
-// 5.92 s, 7 take 5.60 s and 8 take 5.35 s.  FP32 with all m == 0 (MZ) keeps
-// 18.2 KB of LDS per workgroup, so 8 workgroups fit a CU: 8 waves (64 VGPRs, a
-// few cold-path spills); otherwise 22.7 KB allows 7 (72 VGPRs).  FP64 with
-// MZ runs 6 waves (80 VGPRs, some spills): on the FP64 1M x 2M shear workload
-// 4 waves take 208 ms, 5 take 187 ms and 6 take 182 ms (7: the same as 6,
-// with twice the spills).  FP64 with some m != 0 runs 5 waves (96 VGPRs): on
-// that workload with m drawn from N(0, 0.05), 203 ms at 4 waves, 185 ms at 5.
+// Waves per SIMD the register budget is set for.  FP32: 7 waves (72 VGPRs,
+// 94 SGPRs; the LDS of 4-wave workgroups, 18.2 KB with all m == 0 and 22.7 KB
+// otherwise, allows 8 and 7 of them per CU).  The kernel issues VALU work
+// back to back, and at 8 waves (64 VGPRs, 78 SGPRs) its per-visit code
+// spills registers, to vector lanes and to scratch, in instructions that
+// take VALU issue slots: on the 20M x 20M shear workload the pair kernel
+// took 3512 ms at 6 waves, 3506 ms at 7 and 3536 ms at 8 (3537 and 3609 ms
+// with marks at 7 and 8; the mock batch and the adjoint within 0.3 %), all
+// on one GPU.  FP64 with all m == 0 runs 6 waves (80 VGPRs, some spills): on
+// the FP64 1M x 2M shear workload 4 waves took 208 ms, 5 took 187 ms and 6
+// took 182 ms (7: the same as 6, with twice the spills).  FP64 with some
+// m != 0 runs 5 waves (96 VGPRs): on that workload with m drawn from
+// N(0, 0.05), 203 ms at 4 waves, 185 ms at 5.
 template <typename T, bool MZ>
-constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? (MZ ? 8 : 7) : (MZ ? 6 : 5); }
+constexpr int pair_waves_per_simd() { return sizeof(T) == 4 ? 7 : (MZ ? 6 : 5); }
 // the spherical geometry (FP64) holds seven more lane values: 4 waves, up to
 // 128 VGPRs (3 waves were slower on the 20M x 20M sphere workload)
 template <typename T, bool MZ, int SPH>
