@@ -183,6 +183,41 @@ def test_tiling_leaves_results_unchanged(tp, orc, kind):
         c.close()
 
 
+@pytest.mark.parametrize("seed", range(32))
+def test_random_configurations(tp, orc, ctx, seed):
+    """Seeded random cases against the oracle at the bars above: catalogue
+    sizes from 1 to a few thousand (lens chunks of 1, 63, 64 and 65 lenses,
+    single sources), field sides of 5' to 120', 1 to 80 bins of every
+    spacing (more than 63 bins: one pass per group of 63), x_min = 0 where the
+    spacing allows it, both kinds, FP32 and FP64 inputs, patches, marks, z
+    masks and zero weights."""
+    r = _rng(1000 + seed)
+    kind = int(r.integers(0, 2))
+    n_a = int(r.choice([1, 7, 63, 64, 65, 200, 1500, 3000]))
+    n_b = int(r.choice([1, 5, 64, 300, 2000, 4000]))
+    side = float(r.uniform(5.0, 120.0))
+    lens, src = uniform_case(2000 + seed, n_a, n_b, side, kind, z=bool(r.random() < 0.5),
+                             zero_w=bool(r.random() < 0.5))
+    precision = "f64" if r.random() < 0.3 else "f32"
+    if precision == "f64":
+        lens = {k: v.astype(np.float64) for k, v in lens.items()}
+        src = {k: v.astype(np.float64) for k, v in src.items()}
+    spacing = int(r.integers(0, 4))
+    n_bin = int(r.choice([1, 3, 12, 40, 63, 64, 80]))
+    x_min = 0.0 if spacing in (0, 3) and r.random() < 0.5 else float(r.uniform(0.1, 1.5))
+    x_max = float(r.uniform(x_min + 1.0, max(x_min + 2.0, 0.7 * side)))
+    bins = tp.create_bins(x_min, x_max, n_bin, spacing, 0.7)
+    kw = {}
+    if r.random() < 0.5:
+        n_pa, n_pb = int(r.integers(1, 6)), int(r.integers(1, 6))
+        kw = dict(patch_A=r.integers(0, n_pa, n_a).astype(np.int32), patch_B=r.integers(0, n_pb, n_b).astype(np.int32),
+                  n_pa=n_pa, n_pb=n_pb)
+    marks = bool(r.random() < 0.5)
+    print(f"\nseed {seed}: kind {kind}, {n_a} x {n_b} on {side:.1f}', {precision}, spacing {spacing}, "
+          f"{n_bin} bins [{x_min:.3f}, {x_max:.3f}], patches {bool(kw)}, marks {marks}", flush=True)
+    check(tp, orc, kind, lens, src, bins, precision=precision, marks=marks, ctx=ctx, **kw)
+
+
 def test_shear_marks_m_zero_cancelling_weights(tp, orc, ctx):
     """m == 0 (the K marks are copied from the W marks after the kernel) and
     lenses whose first bin holds only two sources of weights +w and -w: the
