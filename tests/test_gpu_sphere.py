@@ -18,6 +18,8 @@ other side (zero separations: angle 0, and NaN position angles for shear as
 in the reference); the Python shear_2pcf wrapper with
 spherical_coords (the spherical adjoint is tested in
 test_gpu_adjoint_sphere.py).
+Seeded random fields anywhere on the sphere, of every size down to a single
+object, with log or linear bins out to up to 60 degrees, close the list.
 The first three cases' bins span the separation of about 4.2 degrees
 (squared chord 5.4e-3) below which a segment visit takes the short angle
 series, so visits of both variants are compared.
@@ -67,7 +69,7 @@ def field(seed, n_a, n_b, kind, lon, lat, masks=True, dtype=np.float64):
 
 
 def check(tp, orc, ctx, kind, lens, src, x_min, x_max, n_bin, precision, patches=0, marks=False, tol=1e-10,
-          spacing=None):
+          spacing=None, min_pairs=10000):
     bins = tp.create_bins(x_min, x_max, n_bin, tp.LOG if spacing is None else spacing)
     kw = {}
     pa = pb = None
@@ -83,7 +85,7 @@ def check(tp, orc, ctx, kind, lens, src, x_min, x_max, n_bin, precision, patches
     amb = edge_band_pairs(lens, src, orc.chord_edges_squared(bins.bin_edges))
     print("\npairs per bin:", g.n_pairs.tolist(), "near-edge pairs:", amb.tolist())
     assert np.all(np.abs(g.n_pairs - o["n_pairs"]) <= amb)
-    assert g.n_pairs.sum() > 10000
+    assert g.n_pairs.sum() >= min_pairs
     exact = amb.sum() == 0
     for gk, ok in (("sum_theta", "theta"), ("sum_w", "w"), ("sum_k", "k")):
         den = np.where(o[ok] != 0, np.abs(o[ok]), 1.0)
@@ -189,6 +191,41 @@ def test_sphere_zero_separation(tp, orc, ctx):
         a = np.asarray(getattr(g, gk)).T
         scale = np.maximum(np.abs(o[ok]).max(axis=1, keepdims=True), 1e-300)
         assert np.max(np.abs(a - o[ok]) / scale) <= 1e-9, gk
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_sphere_random_configurations(tp, orc, ctx, seed):
+    """Seeded random spherical cases against the oracle: fields anywhere on
+    the sphere (near the poles and across longitude 0 included), 1 to 3000
+    objects, field half-widths of 0.02 to 0.4 rad, log or linear bins out to
+    up to 60 degrees (the wide-angle path), both kinds, FP32 and FP64 inputs,
+    patches, marks, masks."""
+    r = np.random.default_rng(3000 + seed)
+    kind = int(r.integers(0, 2))
+    n_a = int(r.choice([1, 63, 65, 400, 1500, 3000]))
+    n_b = int(r.choice([1, 64, 500, 2000, 3000]))
+    lon0, lat0 = float(r.uniform(0, 2 * np.pi)), float(r.uniform(-1.5, 1.5))
+    half = float(r.uniform(0.02, 0.4))
+
+    def lon(rr, n):
+        return np.mod(lon0 + rr.uniform(-half, half, n) / max(np.cos(lat0), 0.1), 2 * np.pi)
+
+    def lat(rr, n):
+        return np.clip(lat0 + rr.uniform(-half, half, n), -np.pi / 2, np.pi / 2)
+
+    dtype = np.float32 if r.random() < 0.4 else np.float64
+    lens, src = field(3100 + seed, n_a, n_b, kind, lon, lat, masks=bool(r.random() < 0.5), dtype=dtype)
+    spacing = tp.LOG if r.random() < 0.7 else tp.LIN
+    x_max = float(r.uniform(0.3, 1.0) * (np.radians(60.0) if r.random() < 0.2 else 2 * half))
+    x_min = float(r.uniform(0.01, 0.2) * x_max)
+    n_bin = int(r.choice([1, 5, 12, 30]))
+    patches = int(r.choice([0, 3]))
+    marks = bool(r.random() < 0.5)
+    print(f"\nseed {seed}: kind {kind}, {n_a} x {n_b} around ({lon0:.2f}, {lat0:.2f}) half {half:.3f}, "
+          f"{np.dtype(dtype).name}, {n_bin} bins [{x_min:.4f}, {x_max:.4f}] spacing {spacing}, patches {patches}, "
+          f"marks {marks}", flush=True)
+    check(tp, orc, ctx, kind, lens, src, x_min, x_max, n_bin, "f32" if dtype == np.float32 else "f64",
+          patches=patches, marks=marks, spacing=spacing, min_pairs=0)
 
 
 def test_sphere_python_wrapper(tp, orc):
