@@ -10,6 +10,9 @@ oracle gives for that catalogue:
   within 1e-6 (FP32; FP64: 1e-12) of each bin's scale (Theta, W, K
   relative, G against the sum of |terms|); marks within the same bars, the
   signed ones against the lens-bin sum of |terms|;
+* seeded random batches (2 to 6 catalogues of either kind, 1 to 3000
+  objects, their own sides and patch counts, m and masks or not) the same
+  way;
 * a spherical batch (FP32 and FP64 inputs) the same way, counts exact apart
   from pairs whose squared chord lies within 1e-9 of an edge (none here):
   shear with m, kappa-kappa, and shear without an m array (the m == 0
@@ -162,6 +165,34 @@ def test_batch_marks_against_oracle(tp, orc, ctx, precision):
                 if k in c["src"]:
                     c["src"][k].copy_(torch.from_numpy(r.normal(0, 0.2, c["src"][k].numel()).astype(dt)))
         torch.cuda.synchronize()
+    b.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_batch_random_configurations(tp, orc, ctx, seed):
+    """Seeded random batches against the oracle: 2 to 6 catalogues of either
+    kind, of 1 to 2500 lenses and 1 to 3000 sources, with their own field
+    sides and patch counts, with or without m and masks, FP32 or FP64, over
+    one set of bins, run twice."""
+    r = np.random.default_rng(4000 + seed)
+    dt = np.float32 if r.random() < 0.6 else np.float64
+    precision = "f32" if dt == np.float32 else "f64"
+    tol = 1e-6 if precision == "f32" else 1e-12
+    bins = tp.create_bins(float(r.uniform(0.2, 1.0)), float(r.uniform(8.0, 30.0)), int(r.choice([3, 12, 20])),
+                          tp.LOG)
+    cats = []
+    for i in range(int(r.integers(2, 7))):
+        cats.append(flat_catalogue(4100 + 10 * seed + i, int(r.integers(0, 2)), int(r.choice([1, 64, 65, 500, 2500])),
+                                   int(r.choice([1, 63, 700, 3000])), float(r.uniform(20.0, 90.0)),
+                                   int(r.integers(1, 5)), dt, m=bool(r.random() < 0.5), masks=bool(r.random() < 0.5)))
+    print(f"\nseed {seed}: {precision}, {len(cats)} catalogues, bins {bins.n_bin}", flush=True)
+    b = tp.Batch([desc(c) for c in cats], bins, precision=precision, patches=True, marks=True, ctx=ctx)
+    for rep in range(2):
+        out = b.run()
+        for c, res in zip(cats, out):
+            o, lens, src = oracle_of(orc, c, bins.bin_edges, marks=True)
+            check_result(res, o, tol)
+            check_marks(res, o, bound_marks(orc, c["kind"], lens, src, bins.bin_edges), tol)
     b.close()
 
 
