@@ -529,7 +529,7 @@ int batch_build(tpcf_ctx *ctx, int32_t n_cat, const tpcf_catalog *cats, const tp
         ctx->opt_split = cand[bi][1];
         b->stats.autotuned = 1;
     }
-    b->stats.cell_target = ctx->opt_cell_target;
+    b->stats.cell_target = cell_target<U>(ctx);
     b->stats.split = ctx->opt_split;
     // pair arguments and a private copy of the squared edges per bin group
     // (later calls on ctx reuse its own edge table)

@@ -202,11 +202,11 @@ struct tpcf_ctx {
     hipStream_t own_stream = nullptr;
     hipEvent_t ev[6] = {};
     double opt_cell = 0.0;
-    // defaults measured on the 20M x 20M clustered shear workload: cell targets
-    // 48-96 are within 1 % of the best (64); work-item bands of 0.1-0.35
-    // x_max are within 0.5 % of each other there, and 0.2-0.25 are the best
-    // on the FP64 1M x 2M workload
-    double opt_cell_target = 64.0;
+    // 0: the automatic cell size uses the precision's own mean sources per
+    // cell (cell_target below).  Work-item bands: 0.12-0.45 x_max are within
+    // 0.6 % of each other on the 20M x 20M clustered shear workload, and 0.2
+    // is the best on the FP64 1M x 2M one (0.3: 1.3 % slower, 0.45: 4 %)
+    double opt_cell_target = 0.0;
     double opt_split = 0.2;       // work-item band height in units of x_max
     bool opt_counters = false;
     tpcf_stats stats{};
@@ -234,6 +234,21 @@ struct tpcf_ctx {
     };
     tpcf::DevBuf buf[B_COUNT];
 };
+
+// Mean sources per cell of the automatic cell size: TPCF_OPT_CELL_TARGET when
+// set, else 128 for FP32 geometry and 64 for FP64.  Measured on one GPU: the
+// 20M x 20M clustered shear workload (FP32) runs 2.7 % faster at 112-160
+// than at 64 (192: 0.8 % slower than 128), the 5M x 5M adjoint (FP32) 3 %
+// faster at 128; the FP64 1M x 2M workload is best at 64-96 (128: 1 %
+// slower, 192: 6 %), the FP64 kappa-kappa 100k call at 64 (128: 5 %), and
+// the sphere (FP64) within 0.4 % from 64 to 160.  The FP64 pair loops cost
+// more per pair against the same per-visit work, which moves the balance
+// between visits and edge pairs to smaller cells.
+template <typename T>
+inline double cell_target(const tpcf_ctx *ctx)
+{
+    return ctx->opt_cell_target > 0.0 ? ctx->opt_cell_target : (sizeof(T) == 4 ? 128.0 : 64.0);
+}
 
 namespace tpcf {
 

@@ -524,7 +524,7 @@ int prepare(tpcf_ctx *ctx, const CallArgs<T> &a, Prepared<T> &p)
     if (!(cell > 0.0)) {
         const double floor_ext = x_max / 64.0;
         const double area = std::max(ext_x, floor_ext) * std::max(ext_y, floor_ext);
-        cell = std::sqrt(ctx->opt_cell_target * area / (double)std::max<int64_t>(a.n_b, 1));
+        cell = std::sqrt(cell_target<T>(ctx) * area / (double)std::max<int64_t>(a.n_b, 1));
         cell = std::min(std::max(cell, x_max / 512.0), 4.0 * x_max);
     }
     Grid g;

@@ -364,6 +364,8 @@ def bench_adjoint(args) -> int:
     kb = torch.randn(n, generator=gen, device=dev) * 0.02
     bins = tp.create_bins(1.0, 200.0, 20, tp.LOG)
     ctx = tp.Context(0)
+    if args.cell_target > 0:
+        ctx.set_option(3, args.cell_target)
 
     def step():
         xi, _ = ag.kappa_kappa(dict(x=kx, y=ky, val=kap, w=kw), dict(x=kx, y=ky, kappa_E=kap, kappa_B=kb, w=kw),
@@ -448,6 +450,8 @@ def bench_kk100k(args) -> int:
     wl = synth.config("kk_100k_auto", args.scale)
     bins = tp.create_bins(wl.x_min, wl.x_max, wl.n_bin, tp.LOG)
     ctx = tp.Context(0)
+    if args.cell_target > 0:
+        ctx.set_option(3, args.cell_target)
 
     def step():
         g = tp.correlate(tp.KAPPA, wl.lens, wl.src, bins, precision="f64", patch_A=wl.patch_A, patch_B=wl.patch_B,
@@ -551,6 +555,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell", type=float, default=0.0, help="source cell side (0: automatic)")
+    ap.add_argument("--cell-target", type=float, default=0.0,
+                    help="automatic cell size: mean sources per cell (0: the library's default, 64)")
+    ap.add_argument("--split", type=float, default=0.0,
+                    help="pair-kernel work-item band height in units of x_max (0: the library's default)")
     ap.add_argument("--no-autotune", action="store_true", help="mocks_256: skip the batch autotuner")
     ap.add_argument("--marks", action="store_true",
                     help="compute the per-lens marks as well (the reference's call always does)")
@@ -601,6 +609,10 @@ def main() -> int:
     ctx = tp.Context(0)
     if args.cell > 0:
         ctx.set_option(1, args.cell)
+    if args.cell_target > 0:
+        ctx.set_option(3, args.cell_target)
+    if args.split > 0:
+        ctx.set_option(4, args.split)
 
     hl = {k: np.ascontiguousarray(v) for k, v in wl.lens.items()}
     hs = {k: np.ascontiguousarray(v) for k, v in wl.src.items()}
@@ -659,7 +671,7 @@ def main() -> int:
                      "kernel": "k_pairs", "kernel_ms": k_ms, "flops_per_pair": fpp,
                      "kernel_source_hash": kernel_source_hash()},
     }
-    if args.scale == 1.0 and args.cell == 0.0 and not args.marks:
+    if args.scale == 1.0 and args.cell == 0.0 and args.cell_target == 0.0 and args.split == 0.0 and not args.marks:
         traffic, src = profiled_traffic(wl.name)
         if traffic is not None:
             out["roofline"]["traffic"] = traffic

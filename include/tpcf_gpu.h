@@ -131,7 +131,8 @@ typedef struct tpcf_stats {
 enum tpcf_option {
     TPCF_OPT_CELL_SIZE = 1,     /* > 0: source cell side; 0: automatic (default) */
     TPCF_OPT_COUNTERS = 2,      /* != 0: collect seg_visits / pair_visits (slower) */
-    TPCF_OPT_CELL_TARGET = 3,   /* automatic cell size: mean sources per cell (default 64) */
+    TPCF_OPT_CELL_TARGET = 3,   /* automatic cell size: mean sources per cell (default: 128 for
+                                   FP32 geometry, 64 for FP64) */
     TPCF_OPT_SPLIT = 4          /* pair-kernel work-item band height in units of x_max (default 0.2) */
 };
 
