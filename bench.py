@@ -557,6 +557,9 @@ def main() -> int:
     ap.add_argument("--cell", type=float, default=0.0, help="source cell side (0: automatic)")
     ap.add_argument("--cell-target", type=float, default=0.0,
                     help="automatic cell size: mean sources per cell (0: the library's default, 64)")
+    ap.add_argument("--counters", action="store_true",
+                    help="after the timed steps, one more call with TPCF_OPT_COUNTERS: segment visits and examined "
+                         "pairs per class (fast / edge / multi) in the line's `classes`")
     ap.add_argument("--split", type=float, default=0.0,
                     help="pair-kernel work-item band height in units of x_max (0: the library's default)")
     ap.add_argument("--no-autotune", action="store_true", help="mocks_256: skip the batch autotuner")
@@ -679,6 +682,19 @@ def main() -> int:
                                                  f"profiled on the kernel of this source hash)")
         else:
             out["roofline"]["traffic_source"] = "no profile of this workload on this pair kernel's sources"
+    if args.counters:
+        # an untimed call that counts the pair kernel's segment visits and
+        # examined lens-source pairs per class
+        ctx.set_option(2, 1)
+        cs = step().stats
+        ctx.set_option(2, 0)
+        sv, pv = np.array(cs["seg_visits"], float), np.array(cs["pair_visits"], float)
+        lanes = stats["n_lens_used"] / max(cs["n_chunks"], 1)
+        out["classes"] = {"segment_visits": sv.tolist(), "examined_pairs": pv.tolist(),
+                          "examined_share": (pv / pv.sum()).tolist(), "in_bin_share": per_step_pairs / pv.sum(),
+                          "sources_per_visit": pv.sum() / sv.sum() / lanes, "lenses_per_chunk": lanes,
+                          "order": "fast, edge, multi"}
+        log(f"classes: visits {sv.tolist()}, examined pairs {np.round(100 * pv / pv.sum(), 1).tolist()} %")
     if not args.host_io and args.e2e_steps > 0:
         # the same call end to end through the C-ABI as a host driver makes it:
         # host arrays in (H2D of the catalogues) and per-bin outputs out; never
