@@ -116,6 +116,27 @@ def test_batch_matches_single_calls_and_replays(tp, ctx, graph):
     b.close()
 
 
+def test_batch_cell_target_reported(tp, ctx):
+    """Without the autotuner a batch plans its catalogues with the automatic
+    cell size of single calls: 128 mean sources per cell for FP32 geometry,
+    64 for FP64, or TPCF_OPT_CELL_TARGET of its context; stats report it."""
+    bins = tp.create_bins(1.0, 30.0, 10, tp.LOG)
+    cats = [make_cat(720 + i, KAPPA if i % 2 else SHEAR, 3000, 4000, 80.0) for i in range(2)]
+    # an FP64 batch reads FP64 arrays
+    cats64 = [dict(c, dl={k: v.double() for k, v in c["dl"].items()}, ds={k: v.double() for k, v in c["ds"].items()})
+              for c in cats]
+    for precision, cs, want in (("f32", cats, 128.0), ("f64", cats64, 64.0)):
+        b = tp.Batch([desc(c) for c in cs], bins, precision=precision, ctx=ctx)
+        assert b.stats["autotuned"] == 0 and b.stats["cell_target"] == want, b.stats
+        b.close()
+    c1 = tp.Context(0)
+    c1.set_option(3, 40.0)
+    b = tp.Batch([desc(c) for c in cats], bins, ctx=c1)
+    assert b.stats["cell_target"] == 40.0, b.stats
+    b.close()
+    c1.close()
+
+
 def test_batch_autotune(tp, ctx):
     bins = tp.create_bins(1.0, 30.0, 10, tp.LOG)
     cats = [make_cat(700 + i, KAPPA if i % 2 else SHEAR, 4000, 5000, 80.0) for i in range(4)]

@@ -284,8 +284,8 @@ def test_mocks_256_batch_against_oracle(tp, orc):
     torch.cuda.synchronize()
     bins = tp.create_bins(1.0, 100.0, 20, tp.LOG)
     ctx = tp.Context(0)
-    ctx.set_option(3, 32.0)    # the cells and items the mocks_256 benchmark's autotuner picks
-    ctx.set_option(4, 0.2)
+    ctx.set_option(3, 64.0)    # the cells and items the mocks_256 benchmark's autotuner picks
+    ctx.set_option(4, 0.25)
     b = tp.Batch([desc(c) for c in cats + extra], bins, patches=True, marks=set(chosen), ctx=ctx)
     out = b.run()
     st = b.stats

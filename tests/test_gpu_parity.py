@@ -183,6 +183,35 @@ def test_tiling_leaves_results_unchanged(tp, orc, kind):
         c.close()
 
 
+def test_automatic_cell_size(tp):
+    """The automatic source cell side is sqrt(t * area / n_B) over the
+    sources' extent (each side at least x_max / 64), clamped to
+    [x_max / 512, 4 x_max], with t = 128 mean sources per cell for FP32
+    geometry and t = 64 for FP64; TPCF_OPT_CELL_TARGET sets t for both."""
+    lens, src = uniform_case(71, 3000, 20000, 100.0, SHEAR)
+    src["y"] = (src["y"] * np.float32(0.3)).astype(np.float32)   # a 100' x 30' field
+    bins = tp.create_bins(1.0, 10.0, 5, tp.LOG)
+
+    def expected(t):
+        x, y = src["x"].astype(np.float64), src["y"].astype(np.float64)
+        fl = bins.x_max / 64.0
+        area = max(np.ptp(x), fl) * max(np.ptp(y), fl)
+        return min(max(np.sqrt(t * area / len(x)), bins.x_max / 512.0), 4.0 * bins.x_max)
+
+    c = tp.Context()
+    try:
+        for target in (None, 32.0):
+            if target is not None:
+                c.set_option(3, target)
+            for precision, t in (("f32", 128.0), ("f64", 64.0)):
+                g = tp.correlate(SHEAR, lens, src, bins, precision=precision, real_bytes=4, ctx=c)
+                want = expected(target or t)
+                print(f"{precision} target {target}: cell {g.stats['cell_size']:.6f}, expected {want:.6f}")
+                assert g.stats["cell_size"] == pytest.approx(want, rel=1e-12)
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("seed", range(32))
 def test_random_configurations(tp, orc, ctx, seed):
     """Seeded random cases against the oracle at the bars above: catalogue
