@@ -273,6 +273,10 @@ def bench_mocks(args) -> int:
     log(f"{len(cats)} catalogues generated on the device in {time.perf_counter() - t0:.1f} s")
     bins = tp.create_bins(1.0, 100.0, 20, tp.LOG)
     ctx = tp.Context(0)
+    if args.cell_target > 0:
+        ctx.set_option(3, args.cell_target)
+    if args.split > 0:
+        ctx.set_option(4, args.split)
     t0 = time.perf_counter()
     b = tp.Batch([synth.batch_desc(c) for c in cats], bins, autotune=not args.no_autotune, ctx=ctx)
     st = b.stats
