@@ -560,7 +560,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cell", type=float, default=0.0, help="source cell side (0: automatic)")
     ap.add_argument("--cell-target", type=float, default=0.0,
-                    help="automatic cell size: mean sources per cell (0: the library's default, 64)")
+                    help="automatic cell size: mean sources per cell (0: the library's default, 128 for FP32 "
+                         "geometry, 64 for FP64)")
     ap.add_argument("--counters", action="store_true",
                     help="after the timed steps, one more call with TPCF_OPT_COUNTERS: segment visits and examined "
                          "pairs per class (fast / edge / multi) in the line's `classes`")
