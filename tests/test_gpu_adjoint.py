@@ -9,7 +9,8 @@ Checked against
 on catalogues with z masks (the reversed z test of the source-side pass),
 zero lens weights, m != 0, and a kappa auto-correlation through autograd
 (the autograd forward records the lens-side pass, its backward replays it);
-the recorded form against the one-call adjoint.
+the recorded form against the one-call adjoint; 12 seeded random
+configurations (sizes, sides, bins, spacings, kind, precision, masks).
 Tolerance: FP32 path within 2e-5 of the largest |gradient| of each array
 (per-pair terms are FP32); FP64 path within 1e-10; finite differences within
 1e-6 relative (+1e-9 absolute)."""
@@ -121,6 +122,52 @@ def test_adjoint_matches_fp64_autograd(tp, orc, ctx, kind, precision):
             err = np.abs(have - want).max()
             print(f"{side}.{k}: max |grad| {scale:.3e}, max err {err:.3e}", flush=True)
             assert err <= tol * scale + 1e-300, (side, k, err, scale)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_adjoint_random_configurations(tp, orc, ctx, seed):
+    """Seeded random cases against the FP64 autograd gradient at the bars
+    above: 1 to 700 lenses (chunks of 1, 7, 64 and 65), 1 to 800 sources,
+    field sides of 5' to 60', 1 to 63 bins of linear (x_min = 0 or not),
+    logarithmic or square-root spacing, both kinds, FP32 and FP64, with and
+    without z masks and zero lens weights.  With a single object on a side
+    (or a single one left by the masks) g = G / K does not depend on its
+    weight: the weight gradient is the difference of two equal terms and the
+    FP64 reference holds only its rounding.  An array whose reference
+    gradient lies below 1e-12 of the call's largest |gradient| is therefore
+    held to the bar against that largest |gradient|; every other array to the
+    bar against its own."""
+    r = np.random.default_rng(5000 + seed)
+    kind = int(r.integers(0, 2))
+    precision = "f32" if r.random() < 0.5 else "f64"
+    n_a = int(r.choice([1, 7, 64, 65, 300, 700, 700]))
+    n_b = int(r.choice([1, 5, 200, 800, 800, 800]))
+    side = float(r.uniform(5.0, 60.0))
+    masks = bool(r.random() < 0.6)
+    lens, src = catalogue(5100 + seed, n_a, n_b, side, kind, masks=masks)
+    spacing = [tp.LIN, tp.LOG, tp.SQRT][int(r.integers(0, 3))]
+    x_min = 0.0 if spacing == tp.LIN and r.random() < 0.4 else float(r.uniform(0.2, 2.0))
+    x_max = float(r.uniform(x_min + 1.0, 1.2 * side + 2.0))
+    n_bin = int(r.integers(1, 64))
+    bins = tp.create_bins(x_min, x_max, n_bin, spacing)
+    print(f"\nseed {seed}: kind {kind}, {precision}, {n_a} x {n_b} on {side:.1f}', masks {masks}, "
+          f"{n_bin} bins {x_min:.2f}-{x_max:.2f} spacing {spacing}", flush=True)
+    gbar1, gbar2 = r.normal(size=n_bin), r.normal(size=n_bin)
+    ref, _, _ = brute_grads(orc, kind, lens, src, bins.bin_edges, gbar1, gbar2)
+    ref = zero_masked(kind, lens, src, ref)
+    dt = np.float32 if precision == "f32" else np.float64
+    got = tp.adjoint(kind, {k: v.astype(dt) for k, v in lens.items()}, {k: v.astype(dt) for k, v in src.items()},
+                     bins, gbar1, gbar2, precision=precision, ctx=ctx)
+    tol = 2e-5 if precision == "f32" else 1e-10
+    top = max(np.abs(v).max() for sd in ref.values() for v in sd.values())
+    for side_ in ("lens", "src"):
+        for k, want in ref[side_].items():
+            err = np.abs(got[side_][k] - want).max()
+            scale = np.abs(want).max()
+            if scale <= 1e-12 * top:
+                scale = top
+            print(f"{side_}.{k}: max |grad| {np.abs(want).max():.3e} (scale {scale:.3e}), max err {err:.3e}")
+            assert err <= tol * scale + 1e-300, (side_, k, err, scale)
 
 
 def test_adjoint_finite_differences(tp, ctx):
