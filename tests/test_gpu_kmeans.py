@@ -9,8 +9,8 @@ centroids could still go either way: labels must agree for >= 99.99 % of the
 points (all of them in these cases so far), centroids within 1e-9 of the
 field size, rounds within one.
 Cases: clustered and uniform flat catalogues in FP64 and FP32, given starting
-centroids, spherical coordinates across longitude 0; k outside [2, 1024] is
-refused.
+centroids, spherical coordinates across longitude 0, 10 seeded random
+configurations; k outside [2, 1024] is refused.
 The weight equaliser and the equal-size post-processing are compared with the
 oracle's form that moves the cluster weights after every block of 256
 consecutive points (block=256, the GPU's steps; the reference moves them
@@ -79,6 +79,30 @@ def test_kmeans_spherical_across_meridian(tp, orc, ctx):
     g = tp.kmeans(lon, lat, w, 16, centroids=c0, spherical=True, max_iterations=30, ctx=ctx)
     o = orc.kmeans(lon, lat, w, 16, centroids=c0, spherical=True, max_iterations=30)
     agree(g, o, 2 * np.pi)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_kmeans_random_configurations(tp, orc, ctx, seed):
+    """Seeded random cases against the oracle at the bars above: 50 to 20000
+    points, k from 2 to 64 (never above n), clustered or uniform fields,
+    FP32 or FP64, the fixed starting grid or given centroids, with or without
+    the weight equaliser (moved per step of 256 points, as on the GPU)."""
+    r = np.random.default_rng(6000 + seed)
+    n = int(r.choice([50, 500, 5000, 20000]))
+    k = int(min(n, r.choice([2, 3, 17, 64])))
+    dt = np.float32 if r.random() < 0.5 else np.float64
+    if r.random() < 0.5:
+        x, y, w = blobs(6100 + seed, n, k=int(r.integers(2, 20)), dtype=dt)
+    else:
+        x, y, w = (r.uniform(0, 100, n).astype(dt), r.uniform(0, 100, n).astype(dt),
+                   r.uniform(0.5, 1.5, n).astype(dt))
+    c0 = np.stack([r.uniform(0, 100, k), r.uniform(0, 100, k)], 1) if r.random() < 0.5 else None
+    s = float(r.choice([0.0, 3.0]))
+    print(f"\nseed {seed}: n {n}, k {k}, {np.dtype(dt).name}, given centroids {c0 is not None}, scaling {s}",
+          flush=True)
+    g = tp.kmeans(x, y, w, k, centroids=c0, weight_equalizer_scaling=s, max_iterations=50, ctx=ctx)
+    o = orc.kmeans(x, y, w, k, centroids=c0, scaling=s, max_iterations=50, block=256 if s > 0 else None)
+    agree(g, o, 100.0)
 
 
 def test_kmeans_refusals(tp, ctx):
