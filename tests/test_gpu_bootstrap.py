@@ -10,6 +10,8 @@
   draws; the same seed draws the same, another seed differently; the pooled
   draws are uniform over the blocks (chi-square below its 99.9 % point); the
   returned multiplicities reproduce the outputs through the oracle.
+* Seeded random configurations (block, bin and sample counts, empty
+  block-bins, given or drawn multiplicities, with or without weights).
 * End to end: a correlation with patches, block_sums, and the bootstrap with
   unit multiplicities returns the correlation's own g_t, g_x (1e-12).
 """
@@ -72,6 +74,49 @@ def test_bootstrap_drawn_multiplicities(tp, orc, ctx):
     chi2 = np.sum((counts - expect) ** 2 / expect)
     assert chi2 < 120.0      # 99.9 % point of chi-square with 63 degrees of freedom is 103
     close(a, orc.bootstrap(blocks, M), blocks, M)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_bootstrap_random_configurations(tp, orc, ctx, seed):
+    """Seeded random cases against the oracle at the bars above: 1 to 700
+    blocks, 1 to 63 bins, 2 to 500 samples, empty block-bins, given
+    multiplicities (with an all-zero sample) or drawn ones (supersampling
+    0.5 to 2), with or without block weights."""
+    r = np.random.default_rng(7000 + seed)
+    P = int(r.choice([1, 2, 37, 128, 700]))
+    nb = int(r.choice([1, 5, 30, 63]))
+    n_boot = int(r.choice([2, 10, 500]))
+    k = r.uniform(1, 5, (P, nb))
+    k[r.random((P, nb)) < 0.1] = 0.0
+    blocks = dict(g_t=r.normal(0, 0.01, (P, nb)) * k, g_x=r.normal(0, 0.01, (P, nb)) * k, k=k)
+    if r.random() < 0.7:
+        blocks["weight"] = r.uniform(10, 20, P)
+    given = r.random() < 0.5
+    print(f"\nseed {seed}: {P} blocks, {nb} bins, {n_boot} samples, given multiplicities {given}", flush=True)
+    if given:
+        M = r.integers(0, 3, (n_boot, P)).astype(np.int32)
+        M[int(r.integers(0, n_boot))] = 0
+        g = tp.bootstrap(blocks, n_boot, multiplicity=M, ctx=ctx)
+        np.testing.assert_array_equal(g["multiplicity"], M)
+    else:
+        ss = float(r.uniform(0.5, 2.0))
+        if int(P * ss) < 1:
+            ss = 1.0   # at least one draw per sample (fewer is refused)
+        g = tp.bootstrap(blocks, n_boot, seed=int(r.integers(0, 2**31)), supersampling=ss, ctx=ctx)
+        M = g["multiplicity"]
+        assert np.all(M.sum(axis=1) == int(P * ss)) and np.all(M >= 0)
+    o = orc.bootstrap(blocks, M)
+    if "weight" in blocks:
+        close(g, o, blocks, M)
+    else:
+        Mf = M.astype(np.float64)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            for key in ("g_t", "g_x"):
+                scale = np.nan_to_num((Mf @ np.abs(blocks[key])) / (Mf @ blocks["k"]))
+                assert np.all(np.abs(g[key] - o[key]) <= 1e-12 * scale + 1e-300), key
+    for key, src in (("cov_t", "g_t"), ("cov_x", "g_x")):
+        c = orc.covariance(o[src])
+        assert np.max(np.abs(g[key] - c)) <= 1e-12 * np.max(np.abs(c)) + 1e-300, key
 
 
 def test_bootstrap_end_to_end_unit_multiplicities(tp, orc, ctx):
