@@ -4,7 +4,7 @@
   (src/tpcf.f90:1029-1049, through the single-precision find_bin_index the
   oracle pins to the reference's own outputs) for FP32 and FP64 inputs, host
   and device arrays, several grids and offsets, with objects on the box's
-  edges and maxima.
+  edges and maxima; 8 seeded random grids, boxes and offsets.
 * Errors: fewer than 2 blocks, labels past the field's range (n_x < n_y),
   non-finite coordinates.
 * End to end (FP64 catalogue): labels as lens patches, a correlation with
@@ -39,6 +39,31 @@ def test_labels_equal_oracle(tp, orc, ctx, dtype, nx, ny, offset):
     want = orc.rect_block_labels(x.astype(np.float32), y.astype(np.float32), nx, ny, offset)
     np.testing.assert_array_equal(lab, want)
     assert lab.min() >= offset and lab.max() < offset + nx * ny
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_labels_random_configurations(tp, orc, ctx, seed):
+    """Seeded random grids against the oracle, bit-exact: 6 to 200000
+    objects in a box of random position and size (0.01' to 300' a side) with
+    its corners among them, 1 <= n_y <= n_x < 16 blocks, offsets to 200,
+    FP32 or FP64 inputs."""
+    r = np.random.default_rng(8000 + seed)
+    n = int(r.choice([6, 100, 5000, 200000]))
+    ny = int(r.integers(1, 12))
+    nx = int(r.integers(max(ny, 2), 16))
+    offset = int(r.integers(0, 200))
+    x0, y0 = r.uniform(-500.0, 500.0, 2)
+    wx, wy = r.uniform(0.01, 300.0, 2)
+    dt = np.float32 if r.random() < 0.5 else np.float64
+    x, y = r.uniform(x0, x0 + wx, n), r.uniform(y0, y0 + wy, n)
+    x[:4] = [x0, x0 + wx, x0, x0 + wx]
+    y[:4] = [y0, y0 + wy, y0 + wy, y0]
+    x, y = x.astype(dt), y.astype(dt)
+    print(f"\nseed {seed}: {n} objects, {nx} x {ny} + {offset}, box {wx:.3f} x {wy:.3f}, {np.dtype(dt).name}",
+          flush=True)
+    lab = tp.rect_blocks(x, y, nx, ny, offset=offset, ctx=ctx)
+    np.testing.assert_array_equal(lab, orc.rect_block_labels(x.astype(np.float32), y.astype(np.float32), nx, ny,
+                                                             offset))
 
 
 def test_labels_device_arrays(tp, orc, ctx):
