@@ -125,6 +125,41 @@ def test_sphere_adjoint_matches_fp64_autograd(tp, orc, ctx, field, kind, precisi
     assert_grads(got, ref, 1e-9, field)
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_sphere_adjoint_random_configurations(tp, orc, ctx, seed):
+    """Seeded random fields against the FP64 gradient at the bar above:
+    centres anywhere in longitude and at latitudes up to 1.4 rad (the fixed
+    cases above cross lon = 0), half-sides of 0.005 to 0.05 rad, 1 to 400 lenses, 1 to 600
+    sources, 1 to 40 logarithmic or linear bins, both kinds and precisions.
+    With a single object on a side its weight gradient vanishes identically,
+    so an array whose reference gradient lies below 1e-12 of the call's
+    largest is held to the bar against that largest."""
+    r = np.random.default_rng(9000 + seed)
+    kind = int(r.integers(0, 2))
+    precision = "f32" if r.random() < 0.5 else "f64"
+    dt = np.float32 if precision == "f32" else np.float64
+    lon0, lat0, half = float(r.uniform(0, 2 * np.pi)), float(r.uniform(-1.4, 1.4)), float(r.uniform(0.005, 0.05))
+    n_a, n_b = int(r.choice([1, 64, 65, 400])), int(r.choice([1, 200, 600, 600]))
+    lens, src = sph_catalogue(9100 + seed, n_a, n_b, kind, lon0, lat0, half, dtype=dt)
+    spacing = tp.LOG if r.random() < 0.6 else tp.LIN
+    x_min = float(r.uniform(0.0005, 0.005)) if spacing == tp.LOG else 0.0
+    bins = tp.create_bins(x_min, float(r.uniform(2 * x_min + 0.005, 3 * half)), int(r.integers(1, 41)), spacing)
+    print(f"\nseed {seed}: kind {kind}, {precision}, {n_a} x {n_b}, centre ({lon0:.3f}, {lat0:.3f}), half {half:.4f}, "
+          f"{bins.n_bin} bins to {bins.x_max:.4f}", flush=True)
+    gbar1, gbar2 = r.normal(size=bins.n_bin), r.normal(size=bins.n_bin)
+    ref, _, _, n_pairs = brute_sph_grads(orc, kind, lens, src, bins.bin_edges, gbar1, gbar2)
+    got = tp.adjoint(kind, lens, src, bins, gbar1, gbar2, precision=precision, spherical=True, ctx=ctx)
+    top = max(np.abs(v).max() for sd in ref.values() for v in sd.values())
+    for side in ("lens", "src"):
+        for k, want in ref[side].items():
+            scale = np.abs(want).max()
+            if scale <= 1e-12 * top:
+                scale = top
+            err = np.abs(got[side][k] - want).max()
+            print(f"{n_pairs} pairs {side}.{k}: scale {scale:.3e}, max err {err:.3e}")
+            assert err <= 1e-9 * scale + 1e-300, (side, k, err, scale)
+
+
 @pytest.mark.parametrize("kind", [SHEAR, KAPPA])
 def test_sphere_autograd_record_replay(tp, orc, ctx, kind):
     """The autograd form on the sphere (record, then replay in backward):
